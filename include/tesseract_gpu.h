@@ -1,0 +1,108 @@
+/* The GPU engine: the C ABI a host registers as `--engine gpu`.
+
+   Plain C types only: programs are the layout-identical ts_program (reference src/isa.h:36-46),
+   state crosses as the reference's 64-byte lines (src/lanes.h:13-15), results as ts_enum_result
+   (src/enum.h:26-35). Every function returns 1 on success and 0 with a message in err on failure;
+   nothing in the library calls exit() or asserts on caller input.
+
+   Threading: one gpu_engine per device owns its stream; gpu_code is immutable after gpu_compile
+   and may be shared; a gpu_state has one owner. Calls on one engine are serialised by the caller.
+   A gpu_code keeps no pointer into the caller's program. */
+#ifndef TESSERACT_GPU_H
+#define TESSERACT_GPU_H
+
+#include "tesseract_host.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gpu_engine gpu_engine;
+typedef struct gpu_code   gpu_code;
+typedef struct gpu_state  gpu_state;
+
+/* Opens device `device` (0 = the first GPU). There is no CPU fallback: without a usable GPU this
+   fails with the HIP error. */
+int  gpu_engine_open(int device, gpu_engine **out, char *err, size_t errlen);
+void gpu_engine_close(gpu_engine *e);
+/* "gfx950, 256 CUs, ..." */
+const char *gpu_engine_describe(const gpu_engine *e);
+
+/* Tuning knobs of one compiled program; zero fields take the defaults. */
+typedef struct {
+    unsigned block;        /* threads per workgroup (multiple of 64) */
+    unsigned slice_nodes;  /* AIG nodes per slice for segmented programs (0 = default) */
+    unsigned lut3;         /* 1 = map to v_bitop3_b32 (default), 2 = plain AND/NOT */
+    unsigned grid;         /* workgroups for enumeration kernels (0 = default) */
+} gpu_tune;
+
+typedef struct {
+    uint32_t segments;     /* launches per pass */
+    uint32_t resident;     /* 1 = the pass loop runs in-kernel with the state in registers */
+    uint32_t slices;       /* largest number of slices in a segment */
+    uint64_t bitop3;       /* LUT instructions in the generated code (all slices) */
+    uint64_t values;       /* values the generated code materialises */
+    double   compile_seconds;
+    uint32_t dims;         /* cube dimensions */
+} gpu_code_info;
+
+/* Compiles p to gfx950 code through hipRTC: the counterpart of
+   jit_compile(const Program*, int literal, JitCode*, char **listing, char*, size_t)
+   (reference src/jit.h:46). listing, if not NULL, receives the generated HIP source (free()). */
+int  gpu_compile(gpu_engine *e, const ts_program *p, const gpu_tune *tune, gpu_code **out,
+                 char **listing, char *err, size_t errlen);
+void gpu_code_free(gpu_code *c);
+int  gpu_code_stats(const gpu_code *c, gpu_code_info *info);
+
+/* Device state for `groups` groups of 512 machines, in the power-on state (reference
+   lane_init/lane_reset, src/lanes.c:110-192; lane_cube_init, src/lanes.c:341-370). For a cube
+   program groups must be a multiple of its 2^(dims-9) groups; consecutive blocks are separate
+   cubes. */
+int  gpu_state_new(gpu_code *c, unsigned groups, gpu_state **out, char *err, size_t errlen);
+void gpu_state_free(gpu_state *s);
+int  gpu_state_reset(gpu_state *s, char *err, size_t errlen);
+/* Copies ngroups groups from / to the reference layout: mem[(g - group0) * ncells + cell] and
+   rr[g - group0] are lines, machine k of group g being bit k % 64 of w[k / 64] (the memory a
+   LaneMachine holds, src/lanes.h:38-51). put ignores read-only cells; get returns ZERO as 0, ONE
+   as all ones, id cells as their pattern and link cells as 0, as a LaneMachine holds them. */
+int  gpu_state_put(gpu_state *s, unsigned group0, unsigned ngroups, const ts_line *mem,
+                   const ts_line *rr, char *err, size_t errlen);
+int  gpu_state_get(gpu_state *s, unsigned group0, unsigned ngroups, ts_line *mem, ts_line *rr,
+                   char *err, size_t errlen);
+
+/* `passes` whole passes of every machine of the state, cubes in lockstep: the counterpart of
+   jit_run (src/jit.h:54) and jit_cube_run (src/jit.h:59). */
+int  gpu_run(gpu_code *c, gpu_state *s, uint64_t passes, char *err, size_t errlen);
+/* Milliseconds of device time the last gpu_run / gpu_enum_run took (HIP events on the engine's
+   stream). */
+double gpu_last_ms(const gpu_engine *e);
+
+/* Exhaustive enumeration on the device: the counterpart of enum_run (src/enum.h:37) with the same
+   validation messages, the same tried / hits / hit[16] / ops and the same share rule for
+   --part (chunks of 64 passes of 512 assignments, chunk c in share c % parts,
+   src/enum.c:232-234). The spec is the text the reference parses (src/spec.h); NULL for equiv. */
+typedef struct {
+    int               mode;      /* TS_EN_VERIFY, TS_EN_EQUIV, TS_EN_SEARCH */
+    const ts_program *a, *b;     /* b: equiv only, inputs and outputs matched by name */
+    const char       *spec;
+    unsigned          ticks;     /* passes per assignment, >= 1 */
+    int               part, part_end, parts;
+    const gpu_tune   *tune;      /* NULL = defaults */
+} gpu_enum;
+
+int  gpu_enum_run(gpu_engine *e, const gpu_enum *en, ts_enum_result *r, char *err, size_t errlen);
+
+/* Listings without a device (the counterpart of `alpha jit`, reference src/main.c:468-496):
+   the HIP source gpu_compile / gpu_enum_run would build, and a hipRTC compile of any such source
+   for an architecture ("gfx950"), which needs no GPU. */
+int  gpu_source(const ts_program *p, const gpu_tune *tune, char **source, gpu_code_info *info,
+                char *err, size_t errlen);
+int  gpu_enum_source(const gpu_enum *en, char **source, char *err, size_t errlen);
+int  gpu_rtc_check(const char *source, const char *arch, double *seconds, size_t *code_bytes,
+                   char *err, size_t errlen);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

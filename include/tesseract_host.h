@@ -1,0 +1,162 @@
+/* Host side of the engine: everything that produces or describes the programs the GPU engine runs.
+   Program helpers, the gate builder, the workload generators, verify/search specifications, random
+   programs for the cycle tester, and Beneš routing. Plain C, no device code.
+
+   Each function names the reference routine whose behaviour it reproduces. */
+#ifndef TESSERACT_HOST_H
+#define TESSERACT_HOST_H
+
+#include <stdio.h>
+
+#include "tesseract_isa.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- program helpers (reference src/asm.c:767-820) ------------------------------------------ */
+
+void     ts_program_free(ts_program *p);
+int      ts_program_find_cell(const ts_program *p, const char *name, uint32_t *addr);
+int      ts_program_readonly(const ts_program *p, uint32_t cell);
+/* The plain/constant/id cell a read of `cell` reaches through links; *mask = XOR of the dims. */
+uint32_t ts_program_resolve(const ts_program *p, uint32_t cell, unsigned *mask);
+/* 9, or one more than the highest dimension an id or link cell names. */
+unsigned ts_program_dims(const ts_program *p);
+int      ts_program_id_name(const char *name);  /* ID0..ID15 -> bit, else -1 */
+unsigned ts_addr_bits(uint32_t ncells);
+/* A deep copy (so a caller's program can be freed while the copy lives). */
+int      ts_program_copy(ts_program *dst, const ts_program *src);
+/* Writes the program as assembly (reference src/build.c:432-444). */
+void     ts_program_print(FILE *f, const ts_program *p);
+
+/* ---- gate builder (reference src/build.h) ---------------------------------------------------- */
+
+enum { TS_G_BUF, TS_G_NOT, TS_G_AND, TS_G_NAND, TS_G_OR, TS_G_NOR, TS_G_XOR, TS_G_XNOR, TS_G_COUNT };
+
+typedef struct ts_builder ts_builder;
+
+ts_builder *ts_b_new(void);
+uint32_t    ts_b_cell(ts_builder *b, const char *name);
+int         ts_b_has(const ts_builder *b, const char *name);
+void        ts_b_input(ts_builder *b, uint32_t cell);
+void        ts_b_output(ts_builder *b, uint32_t cell);
+void        ts_b_gate(ts_builder *b, int type, uint32_t out, const uint32_t *in, int n);
+void        ts_b_gate2(ts_builder *b, int type, uint32_t out, uint32_t x, uint32_t y);
+void        ts_b_reg(ts_builder *b, uint32_t q, uint32_t d);
+void        ts_b_mux(ts_builder *b, uint32_t out, uint32_t s, uint32_t ns, uint32_t a1, uint32_t a0);
+void        ts_b_half_adder(ts_builder *b, uint32_t s, uint32_t k, uint32_t x, uint32_t y);
+void        ts_b_full_adder(ts_builder *b, uint32_t s, uint32_t k, uint32_t x, uint32_t y, uint32_t c);
+void        ts_b_select(ts_builder *b, uint32_t out, const uint32_t *sel, const uint32_t *val, int n);
+uint32_t    ts_b_link(ts_builder *b, const char *name, uint32_t src, int dim);
+size_t      ts_b_len(const ts_builder *b);
+/* Raw instructions, for generators that hand-schedule (e.g. the cube library programs). */
+void        ts_b_nand(ts_builder *b, uint32_t cell);
+void        ts_b_sto(ts_builder *b, uint32_t cell);
+void        ts_b_finish(ts_builder *b, ts_program *out);  /* frees the builder */
+const char *ts_gate_name(int type);
+int         ts_gate_type(const char *name);
+
+/* ---- generators (reference src/gen.h) -------------------------------------------------------- */
+
+void     ts_gen_mul(ts_program *p, int bits);
+int      ts_gen_hash(ts_program *p, int bits, int rounds);
+uint64_t ts_toyhash(uint64_t x, int bits, int rounds);
+void     ts_gen_life(ts_program *p, int w, int h, uint32_t *board);
+unsigned ts_grid_machine(unsigned row, unsigned col, int cbits);
+void     ts_gen_life_cube(ts_program *p, int w, int h, int rbits, int cbits, uint32_t *board);
+void     ts_gen_rank(ts_program *p, int dims);
+void     ts_gen_wht(ts_program *p, int bits);
+void     ts_gen_xorconv(ts_program *p);
+/* An n-bit ripple-carry adder with carry in: inputs a0.., b0.., c; outputs s0.. and c (the carry
+   out replaces the carry in), 19 instructions per bit (the shape of reference examples/adder8.asm). */
+void     ts_gen_adder(ts_program *p, int bits);
+/* Cube library programs (the semantics of reference lib/cube.inc PERM8 and SORT8):
+   ts_gen_perm8: inputs v0..v7 and c0..c16; 17 Beneš rounds over dims 0..8..0; round s swaps v with
+   the machine across benes_dim(s) where c<s> is 1. Output v0..v7.
+   ts_gen_sort8: input v0..v7; a bitonic sort over the 512 machines, machine k ends with the k-th
+   smallest value. Output v0..v7. */
+void     ts_gen_perm8(ts_program *p);
+void     ts_gen_sort8(ts_program *p);
+
+/* ---- specifications (reference src/spec.h) --------------------------------------------------- */
+
+typedef struct ts_spec ts_spec;
+
+ts_spec *ts_spec_parse(const char *text, const ts_program *p, int inputs_allowed, char *err,
+                       size_t errlen);
+void     ts_spec_free(ts_spec *s);
+int      ts_spec_width(const ts_spec *s);
+uint32_t ts_spec_out_cell(const ts_spec *s, int bit);
+uint64_t ts_spec_value(const ts_spec *s, uint64_t x);
+int      ts_vec_resolve(const ts_program *p, const uint32_t *list, size_t n, const char *name,
+                        int *idx);
+
+/* The lowered right-hand side, for engines that evaluate it themselves: an RPN program over
+   64-bit words. Variables are input vectors; bit b of variable k is bit var_pos[k][b] of the
+   assignment x. */
+enum { TS_T_NUM, TS_T_VAR, TS_T_ADD, TS_T_SUB, TS_T_MUL, TS_T_AND, TS_T_OR, TS_T_XOR, TS_T_SHL,
+       TS_T_SHR, TS_T_NOT, TS_T_NEG };
+typedef struct {
+    int      op;
+    uint64_t v;
+} ts_spec_item;
+int                 ts_spec_nrpn(const ts_spec *s);
+const ts_spec_item *ts_spec_rpn(const ts_spec *s);
+int                 ts_spec_nvars(const ts_spec *s);
+int                 ts_spec_var_bits(const ts_spec *s, int k);
+int                 ts_spec_var_pos(const ts_spec *s, int k, int bit);
+
+/* ---- enumeration results (reference src/enum.h) ----------------------------------------------- */
+
+enum { TS_EN_VERIFY = 0, TS_EN_EQUIV = 1, TS_EN_SEARCH = 2 };
+#define TS_EN_SHOW   16
+#define TS_EN_MAX_IN 46
+#define TS_EN_CHUNK  64  /* passes of 512 assignments per hand-out (reference src/enum.c:11) */
+
+/* Layout-identical to the reference EnumResult (src/enum.h:26-35). */
+typedef struct {
+    uint64_t space;
+    uint64_t tried;
+    uint64_t hits;
+    uint64_t hit[TS_EN_SHOW];
+    int      nhit;
+    double   seconds;
+    uint64_t ops;
+    int      threads;
+} ts_enum_result;
+
+void ts_enum_add_hit(ts_enum_result *r, uint64_t x);
+/* equiv: in_map[k] / out_map[k] = index in b's inputs / outputs of a's input / output k, by
+   case-insensitive name. Returns 0 with the reference's message on a mismatch. */
+int  ts_enum_match(const ts_program *a, const ts_program *b, int *in_map, int *out_map, char *err,
+                   size_t errlen);
+
+/* ---- random programs and start states (reference src/tester.c:51-91) ------------------------ */
+
+uint64_t ts_splitmix64(uint64_t *state);
+void     ts_random_program(ts_program *p, uint32_t cells, size_t len, uint64_t seed);
+uint64_t ts_t1_program_seed(uint64_t seed, uint32_t cells, uint64_t k);
+/* Start states for `groups` groups of 512 machines: mem[g * ncells + cell] and rr[g] lines.
+   Group 0 is the reference's t1_start(seed) exactly; later groups continue the same splitmix64
+   stream (cells, then RR, group after group). Read-only cells are left as they are. */
+void     ts_t1_start(const ts_program *p, uint64_t seed, unsigned groups, ts_line *mem,
+                     ts_line *rr);
+#define TS_T1_BUCKETS 64
+int      ts_t1_bucket(uint64_t lam);
+
+/* ---- Beneš routing (reference src/benes.h) --------------------------------------------------- */
+
+#define TS_BENES_ROUNDS (2 * TS_CUBE_DIMS - 1)
+int  ts_benes_dim(int round);
+void ts_benes_route(const uint16_t *dest, uint8_t ctrl[TS_BENES_ROUNDS][TS_CUBE_SIZE]);
+
+/* ---- ISCAS .bench netlists (reference src/net.c) ---------------------------------------------- */
+
+int  ts_net_parse(const char *text, const char *name, ts_program *out, char *err, size_t errlen);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

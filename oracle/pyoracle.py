@@ -1,0 +1,123 @@
+"""ORACLE bindings — test infrastructure only.
+
+ctypes access to liboracle.so (the C restatement of the reference machine and closed forms, see
+oracle.h for how each function is pinned) and, when it was built, to _ref/libalpha_ref.so (the
+reference's own sources compiled by oracle/Makefile). Only tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg import this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libalpha_ref.so")
+
+_o = None
+
+
+class OracleEnumResult(C.Structure):
+    _fields_ = [("space", C.c_uint64), ("tried", C.c_uint64), ("hits", C.c_uint64),
+                ("hit", C.c_uint64 * 16), ("nhit", C.c_int)]
+
+
+def lib():
+    global _o
+    if _o is None:
+        if not os.path.exists(ORACLE_LIB):
+            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+        L = C.CDLL(ORACLE_LIB)
+        vp, u64 = C.c_void_p, C.c_uint64
+        L.oracle_run_lines.argtypes = [vp, C.c_uint, vp, vp, u64]
+        L.oracle_run_lines.restype = C.c_int
+        L.oracle_reset_lines.argtypes = [vp, C.c_uint, vp, vp]
+        L.oracle_random_lines.argtypes = [vp, C.c_uint, u64, vp, vp]
+        L.oracle_enum.argtypes = [C.c_int, vp, vp, vp, vp, vp, C.c_int, vp, u64, C.c_uint, C.c_int,
+                                  C.c_int, C.c_int, C.POINTER(OracleEnumResult)]
+        L.oracle_enum.restype = C.c_int
+        L.oracle_brent_lines.argtypes = [vp, vp, vp, u64, vp]
+        L.oracle_brent_lines.restype = u64
+        L.oracle_exact_cycle.argtypes = [vp, vp, vp, C.c_uint]
+        L.oracle_exact_cycle.restype = u64
+        L.oracle_life_scalar.argtypes = [vp, C.c_int, C.c_int]
+        L.oracle_benes_apply.argtypes = [vp, vp]
+        L.oracle_toyhash.argtypes = [u64, C.c_int, C.c_int]
+        L.oracle_toyhash.restype = u64
+        _o = L
+    return _o
+
+
+def _addr(p):
+    return C.cast(p.ptr, C.c_void_p)
+
+
+def reset_lines(prog, groups):
+    mem = np.zeros((groups, prog.ncells, 8), np.uint64)
+    rr = np.zeros((groups, 8), np.uint64)
+    lib().oracle_reset_lines(_addr(prog), groups, mem.ctypes.data, rr.ctypes.data)
+    return mem, rr
+
+
+def random_lines(prog, groups, seed):
+    mem = np.zeros((groups, prog.ncells, 8), np.uint64)
+    rr = np.zeros((groups, 8), np.uint64)
+    lib().oracle_random_lines(_addr(prog), groups, seed, mem.ctypes.data, rr.ctypes.data)
+    return mem, rr
+
+
+def run_lines(prog, mem, rr, passes):
+    """Returns new (mem, rr) after `passes` passes on the reference machine."""
+    mem = np.array(mem, dtype=np.uint64, order="C")
+    rr = np.array(rr, dtype=np.uint64, order="C")
+    if not lib().oracle_run_lines(_addr(prog), rr.shape[0], mem.ctypes.data, rr.ctypes.data, passes):
+        raise ValueError("groups not a multiple of the cube")
+    return mem, rr
+
+
+def enum(mode, a, b=None, b_in=None, b_out=None, out_cells=None, expect=None, target=0, ticks=1,
+         part=0, part_end=0, parts=1):
+    r = OracleEnumResult()
+    arr = lambda x, t: None if x is None else np.ascontiguousarray(x, dtype=t)
+    b_in, b_out = arr(b_in, np.uint32), arr(b_out, np.uint32)
+    out_cells, expect = arr(out_cells, np.uint32), arr(expect, np.uint64)
+    ptr = lambda x: None if x is None else x.ctypes.data
+    ok = lib().oracle_enum(mode, _addr(a), _addr(b) if b is not None else None, ptr(b_in), ptr(b_out),
+                           ptr(out_cells), 0 if out_cells is None else len(out_cells), ptr(expect),
+                           target, ticks, part, part_end, parts, C.byref(r))
+    if not ok:
+        raise ValueError("oracle_enum refused the program")
+    return {"space": r.space, "tried": r.tried, "hits": r.hits, "hit": list(r.hit[:r.nhit]), "nhit": r.nhit}
+
+
+def brent_lines(prog, mem, rr, max_passes):
+    lam = np.zeros(512, np.uint64)
+    mem = np.ascontiguousarray(mem, dtype=np.uint64)
+    rr = np.ascontiguousarray(rr, dtype=np.uint64)
+    lib().oracle_brent_lines(_addr(prog), mem.ctypes.data, rr.ctypes.data, max_passes, lam.ctypes.data)
+    return lam
+
+
+def exact_cycle(prog, mem, rr, lane):
+    mem = np.ascontiguousarray(mem, dtype=np.uint64)
+    rr = np.ascontiguousarray(rr, dtype=np.uint64)
+    return int(lib().oracle_exact_cycle(_addr(prog), mem.ctypes.data, rr.ctypes.data, lane))
+
+
+def life_scalar(board, gens=1):
+    b = np.ascontiguousarray(board, dtype=np.uint8).copy()
+    h, w = b.shape
+    for _ in range(gens):
+        lib().oracle_life_scalar(b.ctypes.data, w, h)
+    return b
+
+
+def benes_apply(ctrl, v):
+    c = np.ascontiguousarray(ctrl, dtype=np.uint8)
+    x = np.ascontiguousarray(v, dtype=np.uint64).copy()
+    lib().oracle_benes_apply(c.ctypes.data, x.ctypes.data)
+    return x
+
+
+def toyhash(x, bits, rounds):
+    return int(lib().oracle_toyhash(x, bits, rounds))

@@ -1,0 +1,169 @@
+"""Bindings to the reference's own engine, compiled from its sources by `make -C oracle ref` into
+oracle/_ref/libalpha_ref.so. Test infrastructure only: it pins the oracle and gives the CPU
+baseline its "reference" kind. Absent (and every caller skips) where the build could not run.
+
+Structs mirrored here follow the reference headers: Program (src/isa.h:36-46), Enum
+(src/enum.h:17-24), EnumResult (src/enum.h:26-35), RefMachine (src/ref.h:11-20).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from oracle.pyoracle import REF_LIB
+
+_r = None
+
+
+def available():
+    return os.path.exists(REF_LIB)
+
+
+class RefProgram(C.Structure):
+    _fields_ = [("code", C.c_void_p), ("len", C.c_size_t), ("ncells", C.c_uint32),
+                ("cell_names", C.POINTER(C.c_char_p)), ("inputs", C.POINTER(C.c_uint32)),
+                ("ninputs", C.c_size_t), ("outputs", C.POINTER(C.c_uint32)),
+                ("noutputs", C.c_size_t), ("special", C.c_void_p)]
+
+
+class RefEnum(C.Structure):
+    _fields_ = [("mode", C.c_int), ("a", C.c_void_p), ("b", C.c_void_p), ("spec", C.c_void_p),
+                ("ticks", C.c_uint), ("threads", C.c_int), ("idle", C.c_int),
+                ("excluded", C.c_void_p), ("part", C.c_int), ("part_end", C.c_int),
+                ("parts", C.c_int)]
+
+
+class RefEnumResult(C.Structure):
+    _fields_ = [("space", C.c_uint64), ("tried", C.c_uint64), ("hits", C.c_uint64),
+                ("hit", C.c_uint64 * 16), ("nhit", C.c_int), ("seconds", C.c_double),
+                ("ops", C.c_uint64), ("threads", C.c_int)]
+
+
+class RefMachine(C.Structure):
+    _fields_ = [("prog", C.c_void_p), ("mem", C.POINTER(C.c_uint8)), ("rr", C.c_uint8),
+                ("pc", C.c_size_t), ("steps", C.c_uint64), ("ticks", C.c_uint64),
+                ("lane", C.c_uint), ("cube", C.c_void_p)]
+
+
+def lib():
+    global _r
+    if _r is None:
+        L = C.CDLL(REF_LIB)
+        vp = C.c_void_p
+        for name in ("gen_mul", "gen_rank", "gen_wht"):
+            getattr(L, name).argtypes = [vp, C.c_int]
+        L.gen_hash.argtypes = [vp, C.c_int, C.c_int]
+        L.gen_hash.restype = C.c_int
+        L.gen_life.argtypes = [vp, C.c_int, C.c_int, vp]
+        L.gen_life_cube.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]
+        L.gen_xorconv.argtypes = [vp]
+        L.random_program.argtypes = [vp, C.c_uint32, C.c_size_t, C.c_uint64]
+        L.t1_program_seed.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64]
+        L.t1_program_seed.restype = C.c_uint64
+        L.program_free.argtypes = [vp]
+        L.toyhash.argtypes = [C.c_uint64, C.c_int, C.c_int]
+        L.toyhash.restype = C.c_uint64
+        L.spec_parse.argtypes = [C.c_char_p, vp, C.c_int, C.c_char_p, C.c_size_t]
+        L.spec_parse.restype = vp
+        L.spec_free.argtypes = [vp]
+        L.spec_value.argtypes = [vp, C.c_uint64]
+        L.spec_value.restype = C.c_uint64
+        L.enum_run.argtypes = [C.POINTER(RefEnum), C.POINTER(RefEnumResult), C.c_char_p, C.c_size_t]
+        L.enum_run.restype = C.c_int
+        L.net_parse.argtypes = [C.c_char_p, C.c_char_p, vp, C.c_char_p, C.c_size_t]
+        L.net_parse.restype = C.c_int
+        L.ref_cube_init.argtypes = [vp, vp]
+        L.ref_cube_init.restype = C.c_int
+        L.ref_cube_free.argtypes = [vp]
+        L.ref_cube_tick.argtypes = [vp]
+        L.ref_cube_machines.argtypes = [vp]
+        L.ref_cube_machines.restype = C.c_uint
+        L.ref_set.argtypes = [vp, C.c_uint32, C.c_uint8]
+        L.ref_get.argtypes = [vp, C.c_uint32]
+        L.ref_get.restype = C.c_uint8
+        L.benes_route.argtypes = [vp, vp]
+        L.t1_cycles.argtypes = [vp, C.c_uint64, C.c_uint64, vp, vp, vp]
+        L.t1_cycles.restype = C.c_uint64
+        L.cpu_has_avx2.restype = C.c_int
+        _r = L
+    return _r
+
+
+class Prog:
+    """A program made by the reference's generators (owned by the reference library)."""
+
+    def __init__(self):
+        self.s = RefProgram()
+
+    def __del__(self):
+        if _r is not None:
+            _r.program_free(C.byref(self.s))
+
+    @property
+    def ptr(self):
+        return C.cast(C.pointer(self.s), C.c_void_p)
+
+    def code(self):
+        n = self.s.len
+        raw = np.ctypeslib.as_array(C.cast(self.s.code, C.POINTER(C.c_uint32)), shape=(2 * n,)).reshape(n, 2)
+        return (raw[:, 0] & 0xFF).astype(np.uint8), raw[:, 1].copy()
+
+    def names(self):
+        return [self.s.cell_names[i].decode() for i in range(self.s.ncells)]
+
+
+def make(name, *args):
+    p = Prog()
+    getattr(lib(), name)(p.ptr, *args)
+    return p
+
+
+def enum_run(mode, a_ptr, b_ptr=None, spec_text=None, spec_inputs=True, ticks=1, part=0, part_end=0,
+             parts=1, threads=0):
+    L = lib()
+    err = C.create_string_buffer(512)
+    spec = None
+    if spec_text is not None:
+        spec = L.spec_parse(spec_text.encode(), a_ptr, int(spec_inputs), err, len(err))
+        if not spec:
+            raise ValueError(err.value.decode())
+    e = RefEnum(mode, a_ptr, b_ptr, spec, ticks, threads, 0, None, part, part_end, parts)
+    r = RefEnumResult()
+    ok = L.enum_run(C.byref(e), C.byref(r), err, len(err))
+    if spec:
+        L.spec_free(spec)
+    if not ok:
+        raise ValueError(err.value.decode())
+    return {"space": r.space, "tried": r.tried, "hits": r.hits, "hit": list(r.hit[:r.nhit]),
+            "nhit": r.nhit, "seconds": r.seconds, "ops": r.ops, "threads": r.threads}
+
+
+def cube_run(prog_ptr, ncells, mem, rr, passes):
+    """The reference machine cube (ref_cube_*) from line state, as the oracle's run_lines."""
+    L = lib()
+    n = L.ref_cube_machines(prog_ptr)
+    cube = (RefMachine * n)()
+    if not L.ref_cube_init(C.cast(cube, C.c_void_p), prog_ptr):
+        raise MemoryError
+    mem = np.array(mem, dtype=np.uint64)
+    rr = np.array(rr, dtype=np.uint64)
+    bits = np.unpackbits(mem.view(np.uint8), bitorder="little").reshape(mem.shape[0], ncells, 512)
+    rbits = np.unpackbits(rr.view(np.uint8), bitorder="little").reshape(rr.shape[0], 512)
+    for k in range(n):
+        m = C.cast(C.byref(cube, k * C.sizeof(RefMachine)), C.c_void_p)
+        for c in range(2, ncells):
+            L.ref_set(m, c, int(bits[k // 512, c, k % 512]))
+        cube[k].rr = int(rbits[k // 512, k % 512])
+    for _ in range(passes):
+        L.ref_cube_tick(C.cast(cube, C.c_void_p))
+    out = bits.copy()
+    outr = rbits.copy()
+    for k in range(n):
+        m = C.cast(C.byref(cube, k * C.sizeof(RefMachine)), C.c_void_p)
+        for c in range(2, ncells):
+            out[k // 512, c, k % 512] = cube[k].mem[c]
+        outr[k // 512, k % 512] = cube[k].rr
+    L.ref_cube_free(C.cast(cube, C.c_void_p))
+    mem2 = np.packbits(out, axis=-1, bitorder="little").view(np.uint64).reshape(mem.shape)
+    rr2 = np.packbits(outr, axis=-1, bitorder="little").view(np.uint64).reshape(rr.shape)
+    return mem2, rr2

@@ -1,0 +1,567 @@
+"""tesseract_amd: an MI355X engine for the bit-sliced NAND/STO machine.
+
+Python mirror of the C interfaces in include/: programs and their generators (the reference's
+gen_mul, gen_hash, gen_life_cube, ...), verify/search specs, and the GPU engine (gpu_compile,
+gpu_run, gpu_enum_run). Everything runs in libtesseract.so; this module only binds it with ctypes.
+There is no CPU fallback: without the built library, or without a GPU, the engine calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtesseract.so")
+
+OP_NAND, OP_STO = 0, 1
+CELL_ZERO, CELL_ONE, CELL_FIRST_FREE = 0, 1, 2
+CK_PLAIN, CK_ID, CK_LINK = 0, 1, 2
+EN_VERIFY, EN_EQUIV, EN_SEARCH = 0, 1, 2
+G_BUF, G_NOT, G_AND, G_NAND, G_OR, G_NOR, G_XOR, G_XNOR = range(8)
+CUBE_DIMS, CUBE_SIZE = 9, 512
+BENES_ROUNDS = 17
+
+
+class Instr(C.Structure):
+    _fields_ = [("op", C.c_uint8), ("addr", C.c_uint32)]
+
+
+class CellInfo(C.Structure):
+    _fields_ = [("kind", C.c_uint8), ("dim", C.c_uint8), ("src", C.c_uint32)]
+
+
+class ProgramStruct(C.Structure):
+    """Layout-identical to the reference's Program (src/isa.h:36-46)."""
+    _fields_ = [("code", C.POINTER(Instr)), ("len", C.c_size_t), ("ncells", C.c_uint32),
+                ("cell_names", C.POINTER(C.c_char_p)), ("inputs", C.POINTER(C.c_uint32)),
+                ("ninputs", C.c_size_t), ("outputs", C.POINTER(C.c_uint32)),
+                ("noutputs", C.c_size_t), ("special", C.POINTER(CellInfo))]
+
+
+class EnumResultStruct(C.Structure):
+    """Layout-identical to the reference's EnumResult (src/enum.h:26-35)."""
+    _fields_ = [("space", C.c_uint64), ("tried", C.c_uint64), ("hits", C.c_uint64),
+                ("hit", C.c_uint64 * 16), ("nhit", C.c_int), ("seconds", C.c_double),
+                ("ops", C.c_uint64), ("threads", C.c_int)]
+
+
+class GpuTune(C.Structure):
+    _fields_ = [("block", C.c_uint), ("slice_nodes", C.c_uint), ("lut3", C.c_uint), ("grid", C.c_uint)]
+
+
+class GpuCodeInfo(C.Structure):
+    _fields_ = [("segments", C.c_uint32), ("resident", C.c_uint32), ("slices", C.c_uint32),
+                ("bitop3", C.c_uint64), ("values", C.c_uint64), ("compile_seconds", C.c_double),
+                ("dims", C.c_uint32)]
+
+
+class GpuEnumStruct(C.Structure):
+    _fields_ = [("mode", C.c_int), ("a", C.POINTER(ProgramStruct)), ("b", C.POINTER(ProgramStruct)),
+                ("spec", C.c_char_p), ("ticks", C.c_uint), ("part", C.c_int), ("part_end", C.c_int),
+                ("parts", C.c_int), ("tune", C.POINTER(GpuTune))]
+
+
+_lib = None
+
+
+def lib():
+    """The native library; raises if it has not been built (python -m tesseract_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libtesseract.so is not built: run python -m tesseract_amd.build")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER(ProgramStruct)
+        vp, sz, u32, u64, i32 = C.c_void_p, C.c_size_t, C.c_uint32, C.c_uint64, C.c_int
+        sig = {
+            "ts_program_free": (None, [P]),
+            "ts_program_dims": (C.c_uint, [P]),
+            "ts_program_copy": (i32, [P, P]),
+            "ts_gen_mul": (None, [P, i32]),
+            "ts_gen_adder": (None, [P, i32]),
+            "ts_gen_hash": (i32, [P, i32, i32]),
+            "ts_toyhash": (u64, [u64, i32, i32]),
+            "ts_gen_life": (None, [P, i32, i32, C.POINTER(u32)]),
+            "ts_gen_life_cube": (None, [P, i32, i32, i32, i32, C.POINTER(u32)]),
+            "ts_grid_machine": (C.c_uint, [C.c_uint, C.c_uint, i32]),
+            "ts_gen_rank": (None, [P, i32]),
+            "ts_gen_wht": (None, [P, i32]),
+            "ts_gen_xorconv": (None, [P]),
+            "ts_gen_perm8": (None, [P]),
+            "ts_gen_sort8": (None, [P]),
+            "ts_random_program": (None, [P, u32, sz, u64]),
+            "ts_t1_program_seed": (u64, [u64, u32, u64]),
+            "ts_t1_start": (None, [P, u64, C.c_uint, vp, vp]),
+            "ts_t1_bucket": (i32, [u64]),
+            "ts_splitmix64": (u64, [C.POINTER(u64)]),
+            "ts_net_parse": (i32, [C.c_char_p, C.c_char_p, P, C.c_char_p, sz]),
+            "ts_spec_parse": (vp, [C.c_char_p, P, i32, C.c_char_p, sz]),
+            "ts_spec_free": (None, [vp]),
+            "ts_spec_width": (i32, [vp]),
+            "ts_spec_value": (u64, [vp, u64]),
+            "ts_spec_out_cell": (u32, [vp, i32]),
+            "ts_benes_route": (None, [C.POINTER(C.c_uint16), vp]),
+            "ts_benes_dim": (i32, [i32]),
+            "ts_b_new": (vp, []),
+            "ts_b_cell": (u32, [vp, C.c_char_p]),
+            "ts_b_has": (i32, [vp, C.c_char_p]),
+            "ts_b_link": (u32, [vp, C.c_char_p, u32, i32]),
+            "ts_b_input": (None, [vp, u32]),
+            "ts_b_output": (None, [vp, u32]),
+            "ts_b_nand": (None, [vp, u32]),
+            "ts_b_sto": (None, [vp, u32]),
+            "ts_b_gate": (None, [vp, i32, u32, C.POINTER(u32), i32]),
+            "ts_b_reg": (None, [vp, u32, u32]),
+            "ts_b_len": (sz, [vp]),
+            "ts_b_finish": (None, [vp, P]),
+            "ts_enum_add_hit": (None, [C.POINTER(EnumResultStruct), u64]),
+            "gpu_engine_open": (i32, [i32, C.POINTER(vp), C.c_char_p, sz]),
+            "gpu_engine_close": (None, [vp]),
+            "gpu_engine_describe": (C.c_char_p, [vp]),
+            "gpu_last_ms": (C.c_double, [vp]),
+            "gpu_compile": (i32, [vp, P, C.POINTER(GpuTune), C.POINTER(vp), C.POINTER(C.c_char_p),
+                                  C.c_char_p, sz]),
+            "gpu_code_free": (None, [vp]),
+            "gpu_code_stats": (i32, [vp, C.POINTER(GpuCodeInfo)]),
+            "gpu_state_new": (i32, [vp, C.c_uint, C.POINTER(vp), C.c_char_p, sz]),
+            "gpu_state_free": (None, [vp]),
+            "gpu_state_reset": (i32, [vp, C.c_char_p, sz]),
+            "gpu_state_put": (i32, [vp, C.c_uint, C.c_uint, vp, vp, C.c_char_p, sz]),
+            "gpu_state_get": (i32, [vp, C.c_uint, C.c_uint, vp, vp, C.c_char_p, sz]),
+            "gpu_run": (i32, [vp, vp, u64, C.c_char_p, sz]),
+            "gpu_enum_run": (i32, [vp, C.POINTER(GpuEnumStruct), C.POINTER(EnumResultStruct),
+                                   C.c_char_p, sz]),
+            "gpu_source": (i32, [P, C.POINTER(GpuTune), C.POINTER(vp), C.POINTER(GpuCodeInfo),
+                                 C.c_char_p, sz]),
+            "gpu_enum_source": (i32, [C.POINTER(GpuEnumStruct), C.POINTER(vp), C.c_char_p, sz]),
+            "gpu_rtc_check": (i32, [C.c_char_p, C.c_char_p, C.POINTER(C.c_double),
+                                    C.POINTER(sz), C.c_char_p, sz]),
+            "free": (None, [vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name, None)
+            if f is None:
+                continue
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _err():
+    return C.create_string_buffer(1024)
+
+
+class Program:
+    """A NAND/STO program owned by the native library (a ts_program)."""
+
+    def __init__(self):
+        self._p = ProgramStruct()
+        self._owned = True
+
+    def __del__(self):
+        if getattr(self, "_owned", False) and _lib is not None:
+            _lib.ts_program_free(C.byref(self._p))
+            self._owned = False
+
+    @property
+    def ptr(self):
+        return C.pointer(self._p)
+
+    @property
+    def len(self):
+        return int(self._p.len)
+
+    @property
+    def ncells(self):
+        return int(self._p.ncells)
+
+    def name(self, cell):
+        return self._p.cell_names[cell].decode()
+
+    @property
+    def inputs(self):
+        return [self._p.inputs[i] for i in range(self._p.ninputs)]
+
+    @property
+    def outputs(self):
+        return [self._p.outputs[i] for i in range(self._p.noutputs)]
+
+    @property
+    def has_special(self):
+        return bool(self._p.special)
+
+    def kind(self, cell):
+        return self._p.special[cell].kind if self._p.special else CK_PLAIN
+
+    def code(self):
+        """(ops, addrs) as numpy arrays."""
+        n = self.len
+        if n == 0:
+            return np.zeros(0, np.uint8), np.zeros(0, np.uint32)
+        raw = np.ctypeslib.as_array(C.cast(self._p.code, C.POINTER(C.c_uint32)), shape=(n * 2,))
+        raw = raw.reshape(n, 2)
+        return (raw[:, 0] & 0xFF).astype(np.uint8), raw[:, 1].copy()
+
+    def cell(self, name):
+        for i in range(self.ncells):
+            if self.name(i).lower() == name.lower():
+                return i
+        raise KeyError(name)
+
+    @property
+    def dims(self):
+        return int(lib().ts_program_dims(self.ptr))
+
+    @property
+    def groups(self):
+        """Groups of 512 machines in one cube of this program."""
+        return 1 << (self.dims - CUBE_DIMS)
+
+
+def _make(fn, *args):
+    p = Program()
+    r = fn(p.ptr, *args)
+    return p, r
+
+
+def gen_mul(bits):
+    return _make(lib().ts_gen_mul, bits)[0]
+
+
+def gen_adder(bits):
+    return _make(lib().ts_gen_adder, bits)[0]
+
+
+def gen_hash(bits, rounds):
+    p, ok = _make(lib().ts_gen_hash, bits, rounds)
+    if not ok:
+        raise ValueError("no bit permutation for %d bits" % bits)
+    return p
+
+
+def toyhash(x, bits, rounds):
+    return int(lib().ts_toyhash(x, bits, rounds))
+
+
+def gen_life(w, h):
+    board = (C.c_uint32 * (w * h))()
+    p = _make(lib().ts_gen_life, w, h, board)[0]
+    return p, np.array(board, dtype=np.uint32)
+
+
+def gen_life_cube(w, h, rbits, cbits):
+    board = (C.c_uint32 * (w * h))()
+    p = _make(lib().ts_gen_life_cube, w, h, rbits, cbits, board)[0]
+    return p, np.array(board, dtype=np.uint32)
+
+
+def grid_machine(row, col, cbits):
+    return int(lib().ts_grid_machine(row, col, cbits))
+
+
+def gen_rank(dims):
+    return _make(lib().ts_gen_rank, dims)[0]
+
+
+def gen_wht(bits):
+    return _make(lib().ts_gen_wht, bits)[0]
+
+
+def gen_xorconv():
+    return _make(lib().ts_gen_xorconv)[0]
+
+
+def gen_perm8():
+    return _make(lib().ts_gen_perm8)[0]
+
+
+def gen_sort8():
+    return _make(lib().ts_gen_sort8)[0]
+
+
+def random_program(cells, length, seed):
+    return _make(lib().ts_random_program, cells, length, seed)[0]
+
+
+def t1_program_seed(seed, cells, k):
+    return int(lib().ts_t1_program_seed(seed, cells, k))
+
+
+def t1_start(prog, seed, groups=1):
+    """Start states (mem [groups, ncells, 8] u64, rr [groups, 8] u64): group 0 = reference t1_start."""
+    mem = np.zeros((groups, prog.ncells, 8), np.uint64)
+    rr = np.zeros((groups, 8), np.uint64)
+    mem[:, CELL_ONE, :] = np.uint64(0xFFFFFFFFFFFFFFFF)
+    lib().ts_t1_start(prog.ptr, seed, groups, mem.ctypes.data, rr.ctypes.data)
+    return mem, rr
+
+
+def t1_bucket(lam):
+    return int(lib().ts_t1_bucket(lam))
+
+
+def net_parse(text, name="<netlist>"):
+    p = Program()
+    e = _err()
+    if not lib().ts_net_parse(text.encode(), name.encode(), p.ptr, e, len(e)):
+        p._owned = False
+        raise ValueError(e.value.decode())
+    return p
+
+
+class Builder:
+    """The gate builder (reference src/build.h), for hand-made programs."""
+
+    def __init__(self):
+        self._b = lib().ts_b_new()
+
+    def cell(self, name):
+        return int(lib().ts_b_cell(self._b, name.encode()))
+
+    def has(self, name):
+        return bool(lib().ts_b_has(self._b, name.encode()))
+
+    def link(self, name, src, dim):
+        return int(lib().ts_b_link(self._b, name.encode(), src, dim))
+
+    def input(self, c):
+        lib().ts_b_input(self._b, c)
+
+    def output(self, c):
+        lib().ts_b_output(self._b, c)
+
+    def nand(self, c):
+        lib().ts_b_nand(self._b, c)
+
+    def sto(self, c):
+        lib().ts_b_sto(self._b, c)
+
+    def gate(self, kind, out, ins):
+        arr = (C.c_uint32 * len(ins))(*ins)
+        lib().ts_b_gate(self._b, kind, out, arr, len(ins))
+
+    def reg(self, q, d):
+        lib().ts_b_reg(self._b, q, d)
+
+    def __len__(self):
+        return int(lib().ts_b_len(self._b))
+
+    def finish(self):
+        p = Program()
+        lib().ts_b_finish(self._b, p.ptr)
+        self._b = None
+        return p
+
+
+class Spec:
+    def __init__(self, text, prog, inputs_allowed=True):
+        e = _err()
+        self._s = lib().ts_spec_parse(text.encode(), prog.ptr, int(inputs_allowed), e, len(e))
+        if not self._s:
+            raise ValueError(e.value.decode())
+        self.prog = prog
+
+    def __del__(self):
+        if getattr(self, "_s", None) and _lib is not None:
+            _lib.ts_spec_free(self._s)
+
+    @property
+    def width(self):
+        return int(lib().ts_spec_width(self._s))
+
+    def value(self, x):
+        return int(lib().ts_spec_value(self._s, x))
+
+    def out_cell(self, bit):
+        return int(lib().ts_spec_out_cell(self._s, bit))
+
+
+def benes_route(dest):
+    dest = np.ascontiguousarray(dest, dtype=np.uint16)
+    ctrl = np.zeros((BENES_ROUNDS, CUBE_SIZE), np.uint8)
+    lib().ts_benes_route(dest.ctypes.data_as(C.POINTER(C.c_uint16)), ctrl.ctypes.data)
+    return ctrl
+
+
+def benes_dim(r):
+    return int(lib().ts_benes_dim(r))
+
+
+def _take_string(ptr):
+    s = C.cast(ptr, C.c_char_p).value.decode()
+    lib().free(ptr)
+    return s
+
+
+def source(prog, tune=None):
+    """The HIP source gpu_compile would build for prog, and its code statistics (no device)."""
+    out = C.c_void_p()
+    info = GpuCodeInfo()
+    e = _err()
+    if not lib().gpu_source(prog.ptr, _tune(tune), C.byref(out), C.byref(info), e, len(e)):
+        raise RuntimeError(e.value.decode())
+    return _take_string(out), {f: getattr(info, f) for f, _ in GpuCodeInfo._fields_}
+
+
+def _enum_struct(mode, a, b, spec, ticks, part, part_end, parts, tune):
+    en = GpuEnumStruct()
+    en.mode = mode
+    en.a = a.ptr
+    en.b = b.ptr if b is not None else None
+    en.spec = spec.encode() if spec is not None else None
+    en.ticks = ticks
+    en.part, en.part_end, en.parts = part, part_end, parts
+    en.tune = _tune(tune)
+    return en
+
+
+def enum_source(mode, a, b=None, spec=None, ticks=1, part=0, part_end=0, parts=1, tune=None):
+    """The HIP source gpu_enum_run would build (no device)."""
+    en = _enum_struct(mode, a, b, spec, ticks, part, part_end, parts, tune)
+    out = C.c_void_p()
+    e = _err()
+    if not lib().gpu_enum_source(C.byref(en), C.byref(out), e, len(e)):
+        raise ValueError(e.value.decode())
+    return _take_string(out)
+
+
+def rtc_check(text, arch="gfx950"):
+    """hipRTC-compiles HIP source for arch (no device needed); returns (seconds, code bytes)."""
+    secs = C.c_double()
+    size = C.c_size_t()
+    e = _err()
+    if not lib().gpu_rtc_check(text.encode(), arch.encode(), C.byref(secs), C.byref(size), e, len(e)):
+        raise RuntimeError(e.value.decode())
+    return secs.value, size.value
+
+
+def _result(r):
+    return {"space": r.space, "tried": r.tried, "hits": r.hits, "hit": list(r.hit[:r.nhit]),
+            "nhit": r.nhit, "seconds": r.seconds, "ops": r.ops}
+
+
+# ---- the GPU engine --------------------------------------------------------------------------
+
+def _tune(tune):
+    if tune is None:
+        return None
+    t = GpuTune()
+    for k, v in tune.items():
+        setattr(t, k, v)
+    return C.pointer(t)
+
+
+class Engine:
+    """gpu_engine: one per device."""
+
+    def __init__(self, device=0):
+        e = _err()
+        h = C.c_void_p()
+        if not lib().gpu_engine_open(device, C.byref(h), e, len(e)):
+            raise RuntimeError("gpu_engine_open: " + e.value.decode())
+        self._e = h
+
+    def close(self):
+        if self._e:
+            lib().gpu_engine_close(self._e)
+            self._e = None
+
+    def __del__(self):
+        if getattr(self, "_e", None) and _lib is not None:
+            _lib.gpu_engine_close(self._e)
+            self._e = None
+
+    def describe(self):
+        return lib().gpu_engine_describe(self._e).decode()
+
+    @property
+    def last_ms(self):
+        return float(lib().gpu_last_ms(self._e))
+
+    def compile(self, prog, tune=None, listing=False):
+        return Code(self, prog, tune, listing)
+
+    def enum_run(self, mode, a, b=None, spec=None, ticks=1, part=0, part_end=0, parts=1, tune=None):
+        en = _enum_struct(mode, a, b, spec, ticks, part, part_end, parts, tune)
+        r = EnumResultStruct()
+        e = _err()
+        if not lib().gpu_enum_run(self._e, C.byref(en), C.byref(r), e, len(e)):
+            raise ValueError(e.value.decode())
+        return _result(r)
+
+
+class Code:
+    """gpu_code: one compiled program."""
+
+    def __init__(self, engine, prog, tune=None, listing=False):
+        self.engine = engine
+        self.prog = prog
+        e = _err()
+        h = C.c_void_p()
+        lst = C.c_char_p()
+        if not lib().gpu_compile(engine._e, prog.ptr, _tune(tune), C.byref(h),
+                                 C.byref(lst) if listing else None, e, len(e)):
+            raise RuntimeError("gpu_compile: " + e.value.decode())
+        self._c = h
+        self.listing = lst.value.decode() if listing and lst.value else None
+
+    def __del__(self):
+        if getattr(self, "_c", None) and _lib is not None:
+            _lib.gpu_code_free(self._c)
+            self._c = None
+
+    def stats(self):
+        i = GpuCodeInfo()
+        lib().gpu_code_stats(self._c, C.byref(i))
+        return {f: getattr(i, f) for f, _ in GpuCodeInfo._fields_}
+
+    def state(self, groups=None):
+        return State(self, groups if groups is not None else self.prog.groups)
+
+
+class State:
+    """gpu_state: device-resident machines of one program."""
+
+    def __init__(self, code, groups):
+        self.code = code
+        self.groups = groups
+        e = _err()
+        h = C.c_void_p()
+        if not lib().gpu_state_new(code._c, groups, C.byref(h), e, len(e)):
+            raise RuntimeError("gpu_state_new: " + e.value.decode())
+        self._s = h
+
+    def __del__(self):
+        if getattr(self, "_s", None) and _lib is not None:
+            _lib.gpu_state_free(self._s)
+            self._s = None
+
+    def put(self, mem, rr, group0=0):
+        mem = np.ascontiguousarray(mem, dtype=np.uint64)
+        rr = np.ascontiguousarray(rr, dtype=np.uint64)
+        n = rr.shape[0]
+        assert mem.shape == (n, self.code.prog.ncells, 8) and rr.shape == (n, 8)
+        e = _err()
+        if not lib().gpu_state_put(self._s, group0, n, mem.ctypes.data, rr.ctypes.data, e, len(e)):
+            raise RuntimeError("gpu_state_put: " + e.value.decode())
+
+    def get(self, group0=0, ngroups=None):
+        n = self.groups - group0 if ngroups is None else ngroups
+        mem = np.zeros((n, self.code.prog.ncells, 8), np.uint64)
+        rr = np.zeros((n, 8), np.uint64)
+        e = _err()
+        if not lib().gpu_state_get(self._s, group0, n, mem.ctypes.data, rr.ctypes.data, e, len(e)):
+            raise RuntimeError("gpu_state_get: " + e.value.decode())
+        return mem, rr
+
+    def reset(self):
+        e = _err()
+        if not lib().gpu_state_reset(self._s, e, len(e)):
+            raise RuntimeError(e.value.decode())
+
+    def run(self, passes):
+        e = _err()
+        if not lib().gpu_run(self.code._c, self._s, passes, e, len(e)):
+            raise RuntimeError("gpu_run: " + e.value.decode())
+        return self.code.engine.last_ms

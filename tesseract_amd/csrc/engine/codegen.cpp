@@ -1,0 +1,832 @@
+// HIP source generation for lowered programs. See lower.hpp.
+//
+// Mapping: every AND node of a cone gets its 3-feasible cuts enumerated; an area-flow pass picks
+// one cut per node and the cover is taken from the roots down, so each chosen cut becomes one
+// v_bitop3_b32 whose 8-bit truth table absorbs every complemented edge inside the cut. Values that
+// leave the LUT network (stores, cross-lane moves) are kept in the polarity most of their uses
+// want, so a NAND result stored to a cell costs no separate NOT.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <sstream>
+
+#include "lower.hpp"
+#include "tesseract_host.h"
+
+namespace ts {
+
+namespace {
+
+struct Cut {
+    uint8_t  n = 0;
+    uint32_t leaf[3];
+    float    flow = 0;
+};
+
+// One cone mapped to 3-input LUTs.
+struct Mapping {
+    std::vector<uint8_t>  in_cone;   // node in the cone of the roots
+    std::vector<uint8_t>  is_lut;    // AND node materialised as a LUT
+    std::vector<Cut>      best;      // chosen cut per LUT node
+    std::vector<uint8_t>  tt;        // truth table per LUT node (already in stored polarity)
+    std::vector<uint8_t>  inv;       // the variable holds the complement of the node
+};
+
+static bool merge(const Cut &a, const Cut &b, Cut &out)
+{
+    uint32_t tmp[6];
+    int n = 0;
+    for (int i = 0; i < a.n; i++)
+        tmp[n++] = a.leaf[i];
+    for (int i = 0; i < b.n; i++) {
+        bool dup = false;
+        for (int k = 0; k < a.n; k++)
+            dup |= tmp[k] == b.leaf[i];
+        if (!dup)
+            tmp[n++] = b.leaf[i];
+    }
+    if (n > 3)
+        return false;
+    std::sort(tmp, tmp + n);
+    out.n = (uint8_t)n;
+    for (int i = 0; i < n; i++)
+        out.leaf[i] = tmp[i];
+    return true;
+}
+
+static bool same_cut(const Cut &a, const Cut &b)
+{
+    if (a.n != b.n)
+        return false;
+    for (int i = 0; i < a.n; i++)
+        if (a.leaf[i] != b.leaf[i])
+            return false;
+    return true;
+}
+
+// Truth table (over leaf patterns 0xF0, 0xCC, 0xAA) of node n above the given cut.
+static uint8_t simulate(const Aig &g, uint32_t n, const Cut &c, const std::vector<uint8_t> &inv,
+                        std::vector<int16_t> &memo, std::vector<uint32_t> &touched)
+{
+    static const uint8_t pattern[3] = {0xF0, 0xCC, 0xAA};
+    for (int i = 0; i < c.n; i++)
+        if (c.leaf[i] == n)
+            return (uint8_t)(inv[n] ? ~pattern[i] : pattern[i]);
+    if (memo[n] >= 0)
+        return (uint8_t)memo[n];
+    const Node &nd = g.nodes[n];
+    uint8_t v;
+    if (nd.kind == N_CONST) {
+        v = 0;
+    } else {
+        uint8_t x = simulate(g, nd.a >> 1, c, inv, memo, touched);
+        uint8_t y = simulate(g, nd.b >> 1, c, inv, memo, touched);
+        if (nd.a & 1)
+            x = (uint8_t)~x;
+        if (nd.b & 1)
+            y = (uint8_t)~y;
+        v = x & y;
+    }
+    memo[n] = v;
+    touched.push_back(n);
+    return v;
+}
+
+// roots: literals whose values leave the network (stores, hit logic, cross-lane operands).
+static void map_luts(const Aig &g, const std::vector<Lit> &roots, bool lut3, Mapping &m)
+{
+    const size_t N = g.nodes.size();
+    m.in_cone.assign(N, 0);
+    m.is_lut.assign(N, 0);
+    m.best.assign(N, Cut());
+    m.tt.assign(N, 0);
+    m.inv.assign(N, 0);
+
+    // the cone, through PERM operands too
+    std::vector<uint32_t> stack;
+    for (Lit r : roots)
+        stack.push_back(r >> 1);
+    while (!stack.empty()) {
+        uint32_t n = stack.back();
+        stack.pop_back();
+        if (m.in_cone[n])
+            continue;
+        m.in_cone[n] = 1;
+        const Node &nd = g.nodes[n];
+        if (nd.kind == N_AND) {
+            stack.push_back(nd.a >> 1);
+            stack.push_back(nd.b >> 1);
+        } else if (nd.kind == N_PERM) {
+            stack.push_back(nd.a >> 1);
+        }
+    }
+
+    // fanout estimates for area flow
+    std::vector<uint32_t> fanout(N, 0);
+    for (size_t n = 0; n < N; n++) {
+        if (!m.in_cone[n])
+            continue;
+        const Node &nd = g.nodes[n];
+        if (nd.kind == N_AND) {
+            fanout[nd.a >> 1]++;
+            fanout[nd.b >> 1]++;
+        } else if (nd.kind == N_PERM) {
+            fanout[nd.a >> 1]++;
+        }
+    }
+    // nodes that must be materialised whatever the cut choice
+    std::vector<uint8_t> forced(N, 0);
+    for (Lit r : roots)
+        forced[r >> 1] = 1;
+    for (size_t n = 0; n < N; n++)
+        if (m.in_cone[n] && g.nodes[n].kind == N_PERM)
+            forced[g.nodes[n].a >> 1] = 1;
+
+    // cut enumeration in topological (index) order
+    const int KEEP = lut3 ? 8 : 1;
+    std::vector<std::vector<Cut>> cuts(N);
+    std::vector<float> flow(N, 0.f);
+    for (size_t n = 0; n < N; n++) {
+        if (!m.in_cone[n])
+            continue;
+        const Node &nd = g.nodes[n];
+        Cut trivial;
+        trivial.n = 1;
+        trivial.leaf[0] = (uint32_t)n;
+        if (nd.kind != N_AND) {
+            flow[n] = 0;
+            cuts[n].push_back(trivial);
+            continue;
+        }
+        uint32_t a = nd.a >> 1, b = nd.b >> 1;
+        std::vector<Cut> cand;
+        if (!lut3) {
+            Cut c;
+            c.n = 2;
+            c.leaf[0] = std::min(a, b);
+            c.leaf[1] = std::max(a, b);
+            if (a == b)
+                c.n = 1;
+            cand.push_back(c);
+        } else {
+            for (const Cut &x : cuts[a])
+                for (const Cut &y : cuts[b]) {
+                    Cut c;
+                    if (!merge(x, y, c))
+                        continue;
+                    bool seen = false;
+                    for (const Cut &z : cand)
+                        seen |= same_cut(z, c);
+                    if (!seen)
+                        cand.push_back(c);
+                }
+        }
+        for (Cut &c : cand) {
+            float f = 1.f;
+            for (int i = 0; i < c.n; i++) {
+                uint32_t l = c.leaf[i];
+                if (!forced[l])  // a materialised node is a free leaf
+                    f += flow[l] / (float)std::max<uint32_t>(1, fanout[l]);
+            }
+            c.flow = f;
+        }
+        std::sort(cand.begin(), cand.end(), [](const Cut &x, const Cut &y) {
+            return x.flow < y.flow || (x.flow == y.flow && x.n < y.n);
+        });
+        if ((int)cand.size() > KEEP)
+            cand.resize(KEEP);
+        m.best[n] = cand.front();
+        flow[n] = cand.front().flow;
+        cuts[n] = cand;
+        cuts[n].push_back(trivial);  // the node itself as a leaf of a bigger cut
+    }
+
+    // cover from the roots down
+    for (size_t n = 0; n < N; n++)
+        if (m.in_cone[n] && forced[n] && g.nodes[n].kind == N_AND)
+            m.is_lut[n] = 1;
+    for (size_t i = N; i-- > 0;) {
+        if (!m.is_lut[i])
+            continue;
+        const Cut &c = m.best[i];
+        for (int k = 0; k < c.n; k++)
+            if (g.nodes[c.leaf[k]].kind == N_AND)
+                m.is_lut[c.leaf[k]] = 1;
+    }
+
+    // polarity: uses outside the LUT network (roots and PERM operands) decide
+    std::vector<int> pos(N, 0), neg(N, 0);
+    for (Lit r : roots)
+        (r & 1 ? neg : pos)[r >> 1]++;
+    for (size_t n = 0; n < N; n++)
+        if (m.in_cone[n] && g.nodes[n].kind == N_PERM)
+            (g.nodes[n].a & 1 ? neg : pos)[g.nodes[n].a >> 1]++;
+    for (size_t n = 0; n < N; n++)
+        if (m.is_lut[n] && neg[n] > pos[n])
+            m.inv[n] = 1;
+
+    std::vector<int16_t> memo(N, -1);
+    std::vector<uint32_t> touched;
+    for (size_t n = 0; n < N; n++) {
+        if (!m.is_lut[n])
+            continue;
+        uint8_t t = simulate(g, (uint32_t)n, m.best[n], m.inv, memo, touched);
+        for (uint32_t k : touched)
+            memo[k] = -1;
+        touched.clear();
+        m.tt[n] = m.inv[n] ? (uint8_t)~t : t;
+    }
+}
+
+std::string hex32(uint32_t v)
+{
+    char b[16];
+    snprintf(b, sizeof b, "0x%08xu", v);
+    return b;
+}
+
+// The word held by wave lane (lane ^ x), 0 < x < 64: DPP quad permutations for x < 4,
+// ds_swizzle's xor mode inside 32 lanes, ds_bpermute across the halves.
+std::string xl_expr(const std::string &v, uint32_t x)
+{
+    char b[96];
+    static const unsigned quad[4] = {0, 0xB1, 0x4E, 0x1B};
+    if (x < 4)
+        snprintf(b, sizeof b, "(u32)__builtin_amdgcn_mov_dpp((int)(%s), 0x%X, 0xF, 0xF, false)",
+                 v.c_str(), quad[x]);
+    else if (x < 32)
+        snprintf(b, sizeof b, "(u32)__builtin_amdgcn_ds_swizzle((int)(%s), 0x%X)", v.c_str(),
+                 (x << 10) | 0x1Fu);
+    else
+        snprintf(b, sizeof b, "ts_xl_far(%s, %uu)", v.c_str(), x);
+    return b;
+}
+
+// Common device helpers, emitted once per module.
+const char *kPrelude = R"(typedef unsigned int u32;
+typedef unsigned long long u64;
+#define TS_DEV __device__ static inline __attribute__((always_inline))
+// lane k of a word takes lane k ^ m (m < 32): bit permutations inside the word
+TS_DEV u32 ts_pw(u32 v, u32 m) {
+  if (m & 1u)  v = ((v & 0x55555555u) << 1) | ((v >> 1) & 0x55555555u);
+  if (m & 2u)  v = ((v & 0x33333333u) << 2) | ((v >> 2) & 0x33333333u);
+  if (m & 4u)  v = ((v & 0x0F0F0F0Fu) << 4) | ((v >> 4) & 0x0F0F0F0Fu);
+  if (m & 8u)  v = ((v & 0x00FF00FFu) << 8) | ((v >> 8) & 0x00FF00FFu);
+  if (m & 16u) v = (v << 16) | (v >> 16);
+  return v;
+}
+// the word held by wave lane (lane ^ x), x >= 32: any lane through LDS permute hardware
+TS_DEV u32 ts_xl_far(u32 v, u32 x) {
+  return (u32)__builtin_amdgcn_ds_bpermute((int)(((threadIdx.x & 63u) ^ x) << 2), (int)v);
+}
+)";
+
+// Emits the mapped cone as statements "u32 vN = ...;". leaf(n) gives the expression of a non-AND
+// node (loads, inputs, ids); PERM nodes are emitted here from their operand.
+struct Emitter {
+    const Aig &g;
+    const Mapping &m;
+    std::ostringstream &o;
+    std::string prefix;  // variable name prefix
+    size_t luts = 0, values = 0;
+
+    Emitter(const Aig &g_, const Mapping &m_, std::ostringstream &o_, std::string p)
+        : g(g_), m(m_), o(o_), prefix(std::move(p)) {}
+
+    std::string var(uint32_t n) const { return prefix + std::to_string(n); }
+
+    // expression of literal l as a plain value
+    std::string val(Lit l) const
+    {
+        uint32_t n = l >> 1;
+        if (g.nodes[n].kind == N_CONST)
+            return (l & 1) ? "0xffffffffu" : "0u";
+        bool c = (l & 1) != 0;
+        if (g.nodes[n].kind == N_AND && m.inv[n])
+            c = !c;
+        return c ? "(~" + var(n) + ")" : var(n);
+    }
+
+    template <class LeafFn>
+    void emit(LeafFn leaf)
+    {
+        for (size_t n = 1; n < g.nodes.size(); n++) {
+            if (!m.in_cone[n])
+                continue;
+            const Node &nd = g.nodes[n];
+            if (nd.kind == N_AND) {
+                if (!m.is_lut[n])
+                    continue;
+                const Cut &c = m.best[n];
+                uint8_t tt = m.tt[n];
+                o << "  u32 " << var((uint32_t)n) << " = ";
+                if (c.n == 3) {
+                    o << "__builtin_amdgcn_bitop3_b32(" << var(c.leaf[0]) << ", " << var(c.leaf[1])
+                      << ", " << var(c.leaf[2]) << ", " << (unsigned)tt << "u);\n";
+                } else if (c.n == 2) {
+                    o << "__builtin_amdgcn_bitop3_b32(" << var(c.leaf[0]) << ", " << var(c.leaf[1])
+                      << ", " << var(c.leaf[0]) << ", " << (unsigned)tt << "u);\n";
+                } else {
+                    // one leaf: the table is 0xF0 (copy) or 0x0F (complement), or constant
+                    if (tt == 0xF0)
+                        o << var(c.leaf[0]) << ";\n";
+                    else if (tt == 0x0F)
+                        o << "~" << var(c.leaf[0]) << ";\n";
+                    else
+                        o << (tt ? "0xffffffffu" : "0u") << ";\n";
+                }
+                luts++;
+            } else if (nd.kind == N_PERM) {
+                std::string x = val(nd.a);
+                uint32_t inword = nd.b & 31u, lanes = nd.b >> 5;
+                std::string e = x;
+                if (inword)
+                    e = "ts_pw(" + e + ", " + std::to_string(inword) + "u)";
+                if (lanes)
+                    e = xl_expr(e, lanes);
+                o << "  u32 " << var((uint32_t)n) << " = " << e << ";\n";
+            } else {
+                o << "  u32 " << var((uint32_t)n) << " = " << leaf((uint32_t)n, nd) << ";\n";
+            }
+            values++;
+        }
+    }
+};
+
+}  // namespace
+
+// ---- run kernels (streamed and resident) ------------------------------------------------------
+
+void codegen_run(const Lowered &lw, const CodegenOptions &opt, KernelSource &ks)
+{
+    std::ostringstream o;
+    o << kPrelude;
+    const unsigned cube_wl = lw.dims - 5;  // words per cube = 2^cube_wl
+    const char *ptr_of[3] = {"cur", "prev", "init"};
+
+    auto id_expr = [&](uint32_t d) -> std::string {
+        static const uint32_t pat[5] = {0xAAAAAAAAu, 0xCCCCCCCCu, 0xF0F0F0F0u, 0xFF00FF00u,
+                                        0xFFFF0000u};
+        if (d < 5)
+            return hex32(pat[d]);
+        return "(0u - ((cw >> " + std::to_string(d - 5) + ") & 1u))";
+    };
+
+    if (lw.resident) {
+        const Segment &s = lw.segs[0];
+        std::vector<Lit> roots;
+        for (const SegOut &out : s.outs)
+            roots.push_back(out.v);
+        Mapping m;
+        map_luts(s.g, roots, opt.lut3, m);
+        // state variables: every version -1 that is read, every written cell
+        std::vector<uint8_t> is_state(lw.ncells + 1, 0);
+        for (const Version &v : lw.versions)
+            is_state[v.cell] = 1;
+        for (const SegOut &out : s.outs)
+            is_state[out.cell] = 1;
+        o << "extern \"C\" __global__ void __launch_bounds__(" << opt.block
+          << ") ts_pass(const u32* __restrict__ prev, u32* __restrict__ cur, const u32* "
+             "__restrict__ init, u32 nw, u32 passes) {\n"
+             "  const u32 w = blockIdx.x * "
+          << opt.block << "u + threadIdx.x;\n  if (w >= nw) return;\n"
+          << "  const u32 cw = w & " << ((1u << cube_wl) - 1) << "u;\n  (void)cw;\n";
+        for (uint32_t c = 0; c <= lw.ncells; c++) {
+            if (!is_state[c])
+                continue;
+            int32_t fs = lw.final_slot[c];
+            if (fs >= 0)
+                o << "  u32 s" << c << " = prev[(size_t)" << fs << "u * nw + w];\n";
+            else
+                o << "  u32 s" << c << " = init[(size_t)" << lw.init_slot[c] << "u * nw + w];\n";
+        }
+        o << "  for (u32 pass = 0; pass < passes; pass++) {\n";
+        Emitter e(s.g, m, o, "v");
+        e.emit([&](uint32_t, const Node &nd) -> std::string {
+            if (nd.kind == N_LOAD) {
+                const Version &ver = lw.versions[nd.a];
+                std::string x = "s" + std::to_string(ver.cell);
+                uint32_t inword = nd.b & 31u, lanes = nd.b >> 5;
+                if (inword)
+                    x = "ts_pw(" + x + ", " + std::to_string(inword) + "u)";
+                if (lanes)
+                    x = xl_expr(x, lanes);
+                return x;
+            }
+            if (nd.kind == N_IDBIT)
+                return id_expr(nd.a);
+            return "0u";
+        });
+        // all new values first, then the state update (every read above saw the old state)
+        for (size_t i = 0; i < s.outs.size(); i++)
+            o << "    u32 n" << i << " = " << e.val(s.outs[i].v) << ";\n";
+        for (size_t i = 0; i < s.outs.size(); i++)
+            o << "    s" << s.outs[i].cell << " = n" << i << ";\n";
+        o << "  }\n";
+        for (const SegOut &out : s.outs)
+            o << "  cur[(size_t)" << lw.final_slot[out.cell] << "u * nw + w] = s" << out.cell
+              << ";\n";
+        o << "}\n";
+        ks.names.push_back("ts_pass");
+        ks.bitop3 += e.luts;
+        ks.values += e.values;
+        ks.text = o.str();
+        return;
+    }
+
+    // streamed: one kernel per segment, slices switch on the thread's slice index
+    for (size_t si = 0; si < lw.segs.size(); si++) {
+        const Segment &s = lw.segs[si];
+        std::string name = "ts_seg" + std::to_string(si);
+        ks.names.push_back(name);
+        o << "extern \"C\" __global__ void __launch_bounds__(" << opt.block << ") " << name
+          << "(const u32* __restrict__ prev, u32* __restrict__ cur, const u32* __restrict__ "
+             "init, u32 nw) {\n"
+             "  const u32 t = blockIdx.x * "
+          << opt.block << "u + threadIdx.x;\n"
+          << "  const u32 slice = t / nw, w = t - slice * nw;\n"
+          << "  if (slice >= " << s.slices.size() << "u) return;\n"
+          << "  const u32 cw = w & " << ((1u << cube_wl) - 1) << "u;\n  (void)cw;\n"
+          << "  switch (slice) {\n";
+        for (size_t k = 0; k < s.slices.size(); k++) {
+            std::vector<Lit> roots;
+            for (uint32_t oi : s.slices[k])
+                roots.push_back(s.outs[oi].v);
+            Mapping m;
+            map_luts(s.g, roots, opt.lut3, m);
+            o << "  case " << k << "u: {\n";
+            Emitter e(s.g, m, o, "v");
+            e.emit([&](uint32_t, const Node &nd) -> std::string {
+                if (nd.kind == N_LOAD) {
+                    uint32_t where = lw.ver_where[nd.a], slot = (uint32_t)lw.ver_slot[nd.a];
+                    uint32_t inword = nd.b & 31u, wx = nd.b >> 5;
+                    std::string idx = wx ? "(w ^ " + std::to_string(wx) + "u)" : "w";
+                    std::string x = std::string(ptr_of[where]) + "[(size_t)" +
+                                    std::to_string(slot) + "u * nw + " + idx + "]";
+                    if (inword)
+                        x = "ts_pw(" + x + ", " + std::to_string(inword) + "u)";
+                    return x;
+                }
+                if (nd.kind == N_IDBIT)
+                    return id_expr(nd.a);
+                return "0u";
+            });
+            for (uint32_t oi : s.slices[k]) {
+                const SegOut &out = s.outs[oi];
+                o << "    cur[(size_t)" << out.slot << "u * nw + w] = " << e.val(out.v) << ";\n";
+            }
+            o << "  } break;\n";
+            ks.bitop3 += e.luts;
+            ks.values += e.values;
+        }
+        o << "  default: break;\n  }\n}\n";
+    }
+    ks.text = o.str();
+}
+
+// ---- exhaustive enumeration ---------------------------------------------------------------
+
+namespace {
+
+// Symbolic execution of a link-free pass. cur[c] holds the literal of every cell (RR = ncells) on
+// entry and on exit; written[c] is set for cells the pass stores to.
+void execute_linkfree(const ts_program *p, Aig &g, std::vector<Lit> &cur, std::vector<uint8_t> *written)
+{
+    const uint32_t RR = p->ncells;
+    for (size_t i = 0; i < p->len; i++) {
+        const ts_instr in = p->code[i];
+        if (in.op == TS_OP_STO) {
+            if (in.addr >= TS_CELL_FIRST_FREE) {
+                cur[in.addr] = cur[RR];
+                if (written)
+                    (*written)[in.addr] = 1;
+            }
+            continue;
+        }
+        Lit v = in.addr == TS_CELL_ZERO ? 0 : in.addr == TS_CELL_ONE ? 1 : cur[in.addr];
+        cur[RR] = lit_not(g.And(cur[RR], v));
+        if (written)
+            (*written)[RR] = 1;
+    }
+}
+
+// The per-lane expected value for verify, as C source. x bits 0..4 are the lane (compile-time
+// constant per unrolled lane), bits >= 5 come from the word index.
+struct SpecCode {
+    std::string prelude;                 // per word: the var bases
+    std::vector<std::string> lane_expr;  // per lane
+    bool u32 = false;
+};
+
+bool lower_spec(const ts_spec *sp, SpecCode &sc, std::string &err)
+{
+    const int nv = ts_spec_nvars(sp), nr = ts_spec_nrpn(sp), width = ts_spec_width(sp);
+    const ts_spec_item *rpn = ts_spec_rpn(sp);
+    // u32 arithmetic gives the same low 32 bits when nothing shifts right and every left shift
+    // is by a constant
+    bool u32 = width <= 32;
+    for (int i = 0; i < nr && u32; i++) {
+        if (rpn[i].op == TS_T_SHR)
+            u32 = false;
+        if (rpn[i].op == TS_T_SHL && (i == 0 || rpn[i - 1].op != TS_T_NUM))
+            u32 = false;
+    }
+    sc.u32 = u32;
+    const char *T = u32 ? "u32" : "u64";
+    std::ostringstream pre;
+    for (int k = 0; k < nv; k++) {
+        int bits = ts_spec_var_bits(sp, k);
+        std::string acc = "0ull";
+        int b = 0;
+        while (b < bits) {
+            int p0 = ts_spec_var_pos(sp, k, b);
+            if (p0 < 5) {
+                b++;
+                continue;
+            }
+            int len = 1, dir = 0;
+            if (b + 1 < bits) {
+                int p1 = ts_spec_var_pos(sp, k, b + 1);
+                if (p1 == p0 + 1 && p1 >= 5)
+                    dir = 1;
+                else if (p1 == p0 - 1 && p1 >= 5)
+                    dir = -1;
+            }
+            if (dir)
+                while (b + len < bits && ts_spec_var_pos(sp, k, b + len) == p0 + dir * len &&
+                       ts_spec_var_pos(sp, k, b + len) >= 5)
+                    len++;
+            unsigned long long mask = len == 64 ? ~0ull : (1ull << len) - 1;
+            char term[160];
+            if (dir >= 0)
+                snprintf(term, sizeof term, "(((xb >> %d) & 0x%llxull) << %d)", p0, mask, b);
+            else
+                snprintf(term, sizeof term, "(((xr >> %d) & 0x%llxull) << %d)", 63 - p0, mask, b);
+            acc += std::string(" | ") + term;
+            b += len;
+        }
+        pre << "    const u64 vb" << k << " = " << acc << ";\n";
+    }
+    sc.prelude = pre.str();
+    sc.lane_expr.resize(32);
+    for (int l = 0; l < 32; l++) {
+        std::vector<std::string> st;
+        for (int i = 0; i < nr; i++) {
+            const ts_spec_item &it = rpn[i];
+            char b[96];
+            switch (it.op) {
+            case TS_T_NUM:
+                snprintf(b, sizeof b, "((%s)0x%llxull)", T, (unsigned long long)it.v);
+                st.push_back(b);
+                break;
+            case TS_T_VAR: {
+                int k = (int)it.v;
+                unsigned long long c = 0;
+                for (int bit = 0; bit < ts_spec_var_bits(sp, k); bit++) {
+                    int p = ts_spec_var_pos(sp, k, bit);
+                    if (p < 5 && (l >> p & 1))
+                        c |= 1ull << bit;
+                }
+                snprintf(b, sizeof b, "((%s)(vb%d | 0x%llxull))", T, k, c);
+                st.push_back(b);
+                break;
+            }
+            case TS_T_NOT:
+                st.back() = "(~" + st.back() + ")";
+                break;
+            case TS_T_NEG:
+                st.back() = "((" + std::string(T) + ")0 - " + st.back() + ")";
+                break;
+            default: {
+                if (st.size() < 2) {
+                    err = "internal: bad spec program";
+                    return false;
+                }
+                std::string r = st.back();
+                st.pop_back();
+                std::string &x = st.back();
+                const char *op = it.op == TS_T_ADD   ? "+"
+                                 : it.op == TS_T_SUB ? "-"
+                                 : it.op == TS_T_MUL ? "*"
+                                 : it.op == TS_T_AND ? "&"
+                                 : it.op == TS_T_OR  ? "|"
+                                 : it.op == TS_T_XOR ? "^"
+                                                     : nullptr;
+                if (op)
+                    x = "(" + x + " " + op + " " + r + ")";
+                else if (it.op == TS_T_SHL)
+                    x = std::string(u32 ? "ts_shl32(" : "ts_shl64(") + x + ", " + r + ")";
+                else
+                    x = "ts_shr64(" + x + ", " + r + ")";
+            }
+            }
+        }
+        if (st.size() != 1) {
+            err = "internal: bad spec program";
+            return false;
+        }
+        sc.lane_expr[l] = st[0];
+    }
+    return true;
+}
+
+void emit_transpose(std::ostringstream &o, const char *e)
+{
+    static const uint32_t masks[5] = {0x0000FFFFu, 0x00FF00FFu, 0x0F0F0F0Fu, 0x33333333u, 0x55555555u};
+    int j = 16;
+    for (int st = 0; st < 5; st++, j >>= 1)
+        for (int k = 0; k < 32; k++)
+            if (!(k & j))
+                o << "    { u32 t = ((" << e << k << " >> " << j << ") ^ " << e << k + j << ") & "
+                  << hex32(masks[st]) << "; " << e << k << " ^= t << " << j << "; " << e << k + j
+                  << " ^= t; }\n";
+}
+
+}  // namespace
+
+bool codegen_enum(const ts_program *a, const ts_program *b, const int *in_map, const int *out_map,
+                  const EnumLowering &el, const CodegenOptions &opt, KernelSource &ks,
+                  std::string &err, const ts_spec *spec)
+{
+    const unsigned n = el.n;
+    Aig g;
+    const bool loop = el.ticks > 1;
+    // outputs to compare: o_k words
+    std::vector<uint32_t> cells_a, cells_b;
+    if (el.mode == TS_EN_EQUIV) {
+        for (size_t o = 0; o < a->noutputs; o++) {
+            cells_a.push_back(a->outputs[o]);
+            cells_b.push_back(b->outputs[out_map[o]]);
+        }
+    } else {
+        cells_a = el.out_cells_a;
+    }
+
+    std::ostringstream o;
+    o << kPrelude;
+    o << "TS_DEV u32 ts_shl32(u32 a, u32 s) { return s < 32u ? a << s : 0u; }\n"
+         "TS_DEV u64 ts_shl64(u64 a, u64 s) { return s < 64ull ? a << s : 0ull; }\n"
+         "TS_DEV u64 ts_shr64(u64 a, u64 s) { return s < 64ull ? a >> s : 0ull; }\n";
+    o << "extern \"C\" __global__ void __launch_bounds__(" << opt.block
+      << ") ts_enum(u64* __restrict__ out, u64* __restrict__ list, u64 cap, u64 i0, u64 i1, "
+         "u64 space, u32 parts, u32 part, u32 nshare) {\n"
+         "  for (u64 i = i0 + (u64)blockIdx.x * "
+      << opt.block << "u + threadIdx.x; i < i1; i += (u64)gridDim.x * " << opt.block
+      << "u) {\n"
+         "    u64 gw = i;\n"
+         "    if (parts > 1u) {  // active chunk (64 passes of 512 = 1024 words) -> chunk index\n"
+         "      const u64 ci = i >> 10, q = ci / nshare, r = part + ci % nshare;\n"
+         "      gw = ((q * parts + r) << 10) | (i & 1023u);\n"
+         "    }\n"
+         "    const u64 xb = gw << 5;\n"
+         "    if (xb >= space) continue;\n"
+         "    const u32 valid = space - xb >= 32ull ? 0xffffffffu : ((1u << (u32)(space - xb)) - 1u);\n";
+    // input bit j of x: j < 5 lane pattern, else a bit of the word index
+    static const uint32_t pat[5] = {0xAAAAAAAAu, 0xCCCCCCCCu, 0xF0F0F0F0u, 0xFF00FF00u, 0xFFFF0000u};
+    auto inbit = [&](uint32_t j) -> std::string {
+        if (j < 5)
+            return hex32(pat[j]);
+        return "(0u - (u32)((gw >> " + std::to_string(j - 5) + ") & 1ull))";
+    };
+
+    std::vector<Lit> ra(a->ncells + 1, 0), rb;
+    std::vector<uint8_t> wa(a->ncells + 1, 0), wb;
+    auto init_cells = [&](const ts_program *p, std::vector<Lit> &cur, const uint32_t *in_cells,
+                          bool symbolic, uint32_t base) {
+        cur.assign(p->ncells + 1, 0);
+        for (uint32_t c = 0; c <= p->ncells; c++)
+            cur[c] = symbolic ? g.Load(base + c, 0) : 0;
+        if (!symbolic)
+            for (unsigned k = 0; k < n; k++)
+                cur[in_cells[k]] = g.InBit(n - 1 - k);
+        cur[TS_CELL_ZERO] = 0;
+        cur[TS_CELL_ONE] = 1;
+    };
+    std::vector<uint32_t> b_in;
+    if (el.mode == TS_EN_EQUIV)
+        for (unsigned k = 0; k < n; k++)
+            b_in.push_back(b->inputs[in_map[k]]);
+    init_cells(a, ra, a->inputs, loop, 0);
+    execute_linkfree(a, g, ra, &wa);
+    if (el.mode == TS_EN_EQUIV) {
+        wb.assign(b->ncells + 1, 0);
+        init_cells(b, rb, b_in.data(), loop, a->ncells + 1);
+        execute_linkfree(b, g, rb, &wb);
+    }
+
+    // roots: ticks == 1 the compared outputs; ticks > 1 every written cell (state update)
+    std::vector<Lit> roots;
+    if (!loop) {
+        for (uint32_t c : cells_a)
+            roots.push_back(ra[c]);
+        for (uint32_t c : cells_b)
+            roots.push_back(rb[c]);
+    } else {
+        for (uint32_t c = 0; c <= a->ncells; c++)
+            if (wa[c])
+                roots.push_back(ra[c]);
+        for (uint32_t c = 0; el.mode == TS_EN_EQUIV && c <= b->ncells; c++)
+            if (wb[c])
+                roots.push_back(rb[c]);
+    }
+    Mapping m;
+    map_luts(g, roots, opt.lut3, m);
+
+    auto leaf = [&](uint32_t, const Node &nd) -> std::string {
+        if (nd.kind == N_INBIT)
+            return inbit(nd.a);
+        if (nd.kind == N_LOAD)
+            return "s" + std::to_string(nd.a);
+        return "0u";
+    };
+    std::vector<std::string> oa, ob;  // expressions of the compared output words
+    if (!loop) {
+        Emitter e(g, m, o, "v");
+        e.emit(leaf);
+        ks.bitop3 += e.luts;
+        for (uint32_t c : cells_a)
+            oa.push_back(e.val(ra[c]));
+        for (uint32_t c : cells_b)
+            ob.push_back(e.val(rb[c]));
+    } else {
+        // state s<c> (a) and s<ncells_a + 1 + c> (b): power-on with the inputs set
+        auto init_state = [&](const ts_program *p, const uint32_t *in_cells, uint32_t base) {
+            for (uint32_t c = 0; c <= p->ncells; c++) {
+                std::string v = "0u";
+                if (c == TS_CELL_ONE)
+                    v = "0xffffffffu";
+                for (unsigned k = 0; k < n; k++)
+                    if (in_cells[k] == c && c >= TS_CELL_FIRST_FREE)
+                        v = inbit(n - 1 - k);
+                o << "    u32 s" << base + c << " = " << v << ";\n";
+            }
+        };
+        init_state(a, a->inputs, 0);
+        if (el.mode == TS_EN_EQUIV)
+            init_state(b, b_in.data(), a->ncells + 1);
+        o << "    for (u32 tick = 0; tick < " << el.ticks << "u; tick++) {\n";
+        Emitter e(g, m, o, "v");
+        e.emit(leaf);
+        ks.bitop3 += e.luts;
+        std::vector<std::pair<uint32_t, std::string>> upd;
+        for (uint32_t c = 0; c <= a->ncells; c++)
+            if (wa[c])
+                upd.push_back({c, e.val(ra[c])});
+        for (uint32_t c = 0; el.mode == TS_EN_EQUIV && c <= b->ncells; c++)
+            if (wb[c])
+                upd.push_back({a->ncells + 1 + c, e.val(rb[c])});
+        for (size_t i = 0; i < upd.size(); i++)
+            o << "      u32 n" << i << " = " << upd[i].second << ";\n";
+        for (size_t i = 0; i < upd.size(); i++)
+            o << "      s" << upd[i].first << " = n" << i << ";\n";
+        o << "    }\n";
+        for (uint32_t c : cells_a)
+            oa.push_back("s" + std::to_string(c));
+        for (uint32_t c : cells_b)
+            ob.push_back("s" + std::to_string(a->ncells + 1 + c));
+    }
+
+    // hit word
+    o << "    u32 hit;\n";
+    if (el.mode == TS_EN_SEARCH) {
+        o << "    hit = valid";
+        for (int bit = 0; bit < el.width; bit++)
+            o << " & " << ((el.target >> bit & 1) ? "" : "~") << oa[bit];
+        o << ";\n";
+    } else if (el.mode == TS_EN_EQUIV) {
+        o << "    hit = 0u";
+        for (size_t k = 0; k < oa.size(); k++)
+            o << " | (" << oa[k] << " ^ " << ob[k] << ")";
+        o << ";\n    hit &= valid;\n";
+    } else {
+        SpecCode sc;
+        if (!lower_spec(spec, sc, err))
+            return false;
+        o << "    const u64 xr = __builtin_bitreverse64(xb);\n    (void)xr;\n" << sc.prelude;
+        const char *T = sc.u32 ? "u32" : "u64";
+        for (int l = 0; l < 32; l++)
+            o << "    const " << T << " e" << l << " = " << sc.lane_expr[l] << ";\n";
+        o << "    u32 miss = 0u;\n";
+        for (int half = 0; half < (el.width > 32 ? 2 : 1); half++) {
+            o << "    {\n";
+            for (int l = 0; l < 32; l++)
+                o << "    u32 t" << l << " = (u32)(e" << l << (half ? " >> 32" : "") << ");\n";
+            emit_transpose(o, "t");
+            for (int bit = 32 * half; bit < el.width && bit < 32 * half + 32; bit++)
+                o << "    miss |= t" << bit - 32 * half << " ^ " << oa[bit] << ";\n";
+            o << "    }\n";
+        }
+        o << "    hit = miss & valid;\n";
+    }
+    o << "    if (hit) {\n"
+         "      atomicAdd(&out[0], (u64)__builtin_popcount(hit));\n"
+         "      const u64 k = atomicAdd(&out[1], 1ull);\n"
+         "      if (k < cap) { list[2 * k] = gw; list[2 * k + 1] = hit; }\n"
+         "    }\n"
+         "  }\n}\n";
+    ks.names.push_back("ts_enum");
+    ks.text = o.str();
+    return true;
+}
+
+}  // namespace ts

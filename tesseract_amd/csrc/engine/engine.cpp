@@ -1,0 +1,733 @@
+// The GPU engine behind include/tesseract_gpu.h: hipRTC compilation of lowered programs, device
+// state in the word layout, pass launches and exhaustive enumeration.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "lower.hpp"
+#include "tesseract_gpu.h"
+
+using namespace ts;
+
+struct gpu_engine {
+    int         device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t  ev0 = nullptr, ev1 = nullptr;
+    int         cus = 0;
+    std::string arch, desc;
+    double      last_ms = 0;
+};
+
+struct gpu_code {
+    gpu_engine            *e = nullptr;
+    Lowered                lw;
+    hipModule_t            mod = nullptr;
+    std::vector<hipFunction_t> fns;
+    std::vector<uint32_t>  slices;      // per segment
+    gpu_code_info          info{};
+    gpu_tune               tune{};
+    uint32_t               ncells = 0;
+    std::vector<uint8_t>   kind;        // per cell: TS_CK_*; constants as plain
+    std::vector<uint8_t>   id_dim;
+};
+
+struct gpu_state {
+    gpu_code *c = nullptr;
+    unsigned  groups = 0;
+    uint32_t  nw = 0;        // words = groups * 16
+    uint32_t *slots = nullptr, *init = nullptr;
+    int       latest = 1;    // parity holding the newest values of written cells
+};
+
+namespace {
+
+void say(char *err, size_t errlen, const char *fmt, ...)
+{
+    if (!err || !errlen)
+        return;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(err, errlen, fmt, ap);
+    va_end(ap);
+}
+
+#define HIPCHK(call)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            say(err, errlen, "%s failed: %s", #call, hipGetErrorString(e_));                \
+            return 0;                                                                       \
+        }                                                                                   \
+    } while (0)
+
+gpu_tune defaults(const gpu_engine *e, const gpu_tune *t)
+{
+    gpu_tune d{};
+    if (t)
+        d = *t;
+    if (!d.block || d.block % 64 || d.block > 1024)
+        d.block = 256;
+    if (!d.slice_nodes)
+        d.slice_nodes = 1500;
+    if (!d.lut3)
+        d.lut3 = 1;
+    if (!d.grid)
+        d.grid = (unsigned)std::max(1, e->cus) * 8;
+    return d;
+}
+
+// hipRTC: source -> gfx950 code object (needs no device)
+int rtc_compile(const std::string &src, const std::string &arch_name, std::vector<char> &code,
+                double *seconds, char *err, size_t errlen)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "tesseract_gen.hip", 0, nullptr, nullptr) !=
+        HIPRTC_SUCCESS) {
+        say(err, errlen, "hiprtcCreateProgram failed");
+        return 0;
+    }
+    std::string arch = "--offload-arch=" + arch_name;
+    const char *opts[] = {arch.c_str(), "-O3", "-Xclang", "-target-feature", "-Xclang",
+                          "-packed-fp32-ops"};
+    hiprtcResult r = hiprtcCompileProgram(prog, 6, opts);
+    if (r != HIPRTC_SUCCESS) {
+        size_t n = 0;
+        hiprtcGetProgramLogSize(prog, &n);
+        std::string log(n + 1, 0);
+        hiprtcGetProgramLog(prog, &log[0]);
+        say(err, errlen, "hipRTC: %s: %.900s", hiprtcGetErrorString(r), log.c_str());
+        hiprtcDestroyProgram(&prog);
+        return 0;
+    }
+    size_t size = 0;
+    hiprtcGetCodeSize(prog, &size);
+    code.resize(size);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    if (seconds)
+        *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return 1;
+}
+
+int build_module(gpu_engine *e, const std::string &src, hipModule_t *mod, double *seconds, char *err,
+                 size_t errlen)
+{
+    std::vector<char> code;
+    if (!rtc_compile(src, e->arch, code, seconds, err, errlen))
+        return 0;
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipModuleLoadData(mod, code.data()));
+    return 1;
+}
+
+int launch(gpu_engine *e, hipFunction_t f, unsigned grid, unsigned block, void **args, char *err,
+           size_t errlen)
+{
+    HIPCHK(hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, 0, e->stream, args, nullptr));
+    return 1;
+}
+
+uint32_t id_pattern_word(const gpu_code *c, uint32_t cell, unsigned group, unsigned j)
+{
+    // word j of group g holds machines 32 j .. 32 j + 31 of that group, numbered within the cube
+    unsigned cube_groups = 1u << (c->lw.dims - TS_CUBE_DIMS);
+    unsigned d = c->id_dim[cell];
+    uint32_t w = 0;
+    for (unsigned b = 0; b < 32; b++) {
+        unsigned machine = (group % cube_groups) * TS_CUBE_SIZE + 32 * j + b;
+        w |= (uint32_t)(machine >> d & 1) << b;
+    }
+    return w;
+}
+
+}  // namespace
+
+// ---- engine ---------------------------------------------------------------------------------
+
+extern "C" int gpu_engine_open(int device, gpu_engine **out, char *err, size_t errlen)
+{
+    *out = nullptr;
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) {
+        say(err, errlen, "no GPU %d (%d visible)", device, n);
+        return 0;
+    }
+    std::unique_ptr<gpu_engine> e(new gpu_engine());
+    e->device = device;
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    e->cus = prop.multiProcessorCount;
+    e->arch = prop.gcnArchName;
+    size_t colon = e->arch.find(':');
+    if (colon != std::string::npos)
+        e->arch.resize(colon);
+    char d[256];
+    snprintf(d, sizeof d, "%s, %d CUs, %.0f MHz, %.1f GB", e->arch.c_str(), e->cus,
+             prop.clockRate / 1000.0, prop.totalGlobalMem / 1e9);
+    e->desc = d;
+    HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&e->ev0));
+    HIPCHK(hipEventCreate(&e->ev1));
+    *out = e.release();
+    return 1;
+}
+
+extern "C" void gpu_engine_close(gpu_engine *e)
+{
+    if (!e)
+        return;
+    hipSetDevice(e->device);
+    if (e->ev0)
+        hipEventDestroy(e->ev0);
+    if (e->ev1)
+        hipEventDestroy(e->ev1);
+    if (e->stream)
+        hipStreamDestroy(e->stream);
+    delete e;
+}
+
+extern "C" const char *gpu_engine_describe(const gpu_engine *e) { return e->desc.c_str(); }
+extern "C" double gpu_last_ms(const gpu_engine *e) { return e->last_ms; }
+
+// ---- compile --------------------------------------------------------------------------------
+
+namespace {
+
+// program -> lowered passes -> HIP source (host only)
+int generate(const ts_program *p, const gpu_tune &tune, Lowered &lw, KernelSource &ks, char *err,
+             size_t errlen)
+{
+    LowerOptions lo;
+    unsigned dims = ts_program_dims(p);
+    lo.cube_words_log2 = dims - 5;
+    lo.slice_nodes = tune.slice_nodes;
+    std::string msg;
+    if (!lower_program(p, lo, lw, msg)) {
+        say(err, errlen, "%s", msg.c_str());
+        return 0;
+    }
+    CodegenOptions co;
+    co.block = tune.block;
+    co.lut3 = tune.lut3 == 1;
+    codegen_run(lw, co, ks);
+    return 1;
+}
+
+void fill_info(const Lowered &lw, const KernelSource &ks, gpu_code_info &info)
+{
+    uint32_t most = 0;
+    for (const Segment &s : lw.segs)
+        most = std::max<uint32_t>(most, (uint32_t)s.slices.size());
+    info.segments = lw.resident ? 1 : (uint32_t)lw.segs.size();
+    info.resident = lw.resident;
+    info.slices = most;
+    info.bitop3 = ks.bitop3;
+    info.values = ks.values;
+    info.dims = lw.dims;
+}
+
+}  // namespace
+
+extern "C" int gpu_compile(gpu_engine *e, const ts_program *p, const gpu_tune *tune, gpu_code **out,
+                           char **listing, char *err, size_t errlen)
+{
+    *out = nullptr;
+    if (listing)
+        *listing = nullptr;
+    std::unique_ptr<gpu_code> c(new gpu_code());
+    c->e = e;
+    c->tune = defaults(e, tune);
+    c->ncells = p->ncells;
+    c->kind.assign(p->ncells, TS_CK_PLAIN);
+    c->id_dim.assign(p->ncells, 0);
+    for (uint32_t i = 0; p->special && i < p->ncells; i++) {
+        c->kind[i] = p->special[i].kind;
+        c->id_dim[i] = p->special[i].dim;
+    }
+    KernelSource ks;
+    if (!generate(p, c->tune, c->lw, ks, err, errlen))
+        return 0;
+    if (listing)
+        *listing = strdup(ks.text.c_str());
+    double secs = 0;
+    if (!build_module(e, ks.text, &c->mod, &secs, err, errlen))
+        return 0;
+    for (const std::string &name : ks.names) {
+        hipFunction_t f;
+        HIPCHK(hipModuleGetFunction(&f, c->mod, name.c_str()));
+        c->fns.push_back(f);
+    }
+    for (const Segment &s : c->lw.segs)
+        c->slices.push_back((uint32_t)s.slices.size());
+    fill_info(c->lw, ks, c->info);
+    c->info.compile_seconds = secs;
+    *out = c.release();
+    return 1;
+}
+
+extern "C" int gpu_source(const ts_program *p, const gpu_tune *tune, char **source,
+                          gpu_code_info *info, char *err, size_t errlen)
+{
+    gpu_engine none;
+    none.cus = 256;
+    gpu_tune t = defaults(&none, tune);
+    Lowered lw;
+    KernelSource ks;
+    if (!generate(p, t, lw, ks, err, errlen))
+        return 0;
+    if (info) {
+        memset(info, 0, sizeof *info);
+        fill_info(lw, ks, *info);
+    }
+    *source = strdup(ks.text.c_str());
+    return 1;
+}
+
+extern "C" int gpu_rtc_check(const char *source, const char *arch, double *seconds,
+                             size_t *code_bytes, char *err, size_t errlen)
+{
+    std::vector<char> code;
+    if (!rtc_compile(source, arch ? arch : "gfx950", code, seconds, err, errlen))
+        return 0;
+    if (code_bytes)
+        *code_bytes = code.size();
+    return 1;
+}
+
+extern "C" void gpu_code_free(gpu_code *c)
+{
+    if (!c)
+        return;
+    if (c->mod) {
+        hipSetDevice(c->e->device);
+        hipModuleUnload(c->mod);
+    }
+    delete c;
+}
+
+extern "C" int gpu_code_stats(const gpu_code *c, gpu_code_info *info)
+{
+    *info = c->info;
+    return 1;
+}
+
+// ---- state ----------------------------------------------------------------------------------
+
+extern "C" int gpu_state_new(gpu_code *c, unsigned groups, gpu_state **out, char *err, size_t errlen)
+{
+    *out = nullptr;
+    unsigned cube_groups = 1u << (c->lw.dims - TS_CUBE_DIMS);
+    if (!groups || groups % cube_groups) {
+        say(err, errlen, "a cube of this program has %u groups of 512 machines; %u is not a multiple",
+            cube_groups, groups);
+        return 0;
+    }
+    if ((uint64_t)groups * 16 > 0xFFFFFFFFull / 2) {
+        say(err, errlen, "too many groups");
+        return 0;
+    }
+    std::unique_ptr<gpu_state> s(new gpu_state());
+    s->c = c;
+    s->groups = groups;
+    s->nw = groups * 16;
+    HIPCHK(hipSetDevice(c->e->device));
+    size_t slot_bytes = (size_t)2 * c->lw.nslots * s->nw * 4, init_bytes = (size_t)c->lw.ninit * s->nw * 4;
+    HIPCHK(hipMalloc(&s->slots, std::max<size_t>(slot_bytes, 4)));
+    HIPCHK(hipMalloc(&s->init, std::max<size_t>(init_bytes, 4)));
+    *out = s.release();
+    return gpu_state_reset(*out, err, errlen);
+}
+
+extern "C" void gpu_state_free(gpu_state *s)
+{
+    if (!s)
+        return;
+    hipSetDevice(s->c->e->device);
+    hipFree(s->slots);
+    hipFree(s->init);
+    delete s;
+}
+
+namespace {
+
+// Staging copies of the groups [g0, g0 + n): per stored cell, n * 16 words, cell-major.
+struct Staging {
+    std::vector<uint32_t> slot_words, init_words;
+};
+
+int put_lines(gpu_state *s, unsigned g0, unsigned n, const ts_line *mem, const ts_line *rr,
+              char *err, size_t errlen)
+{
+    gpu_code *c = s->c;
+    const Lowered &lw = c->lw;
+    const size_t per = (size_t)n * 16;
+    std::vector<uint32_t> slot_words((size_t)lw.nslots * per), init_words((size_t)lw.ninit * per);
+    for (uint32_t cell = 0; cell <= lw.ncells; cell++) {
+        int32_t fs = lw.final_slot[cell], is = lw.init_slot[cell];
+        if (fs < 0 && is < 0)
+            continue;
+        uint32_t *dst = fs >= 0 ? &slot_words[(size_t)fs * per] : &init_words[(size_t)is * per];
+        for (unsigned g = 0; g < n; g++) {
+            const ts_line *line = cell == lw.ncells ? &rr[g] : &mem[(size_t)g * lw.ncells + cell];
+            memcpy(dst + (size_t)g * 16, line->w, 64);
+        }
+    }
+    hipStream_t st = c->e->stream;
+    if (lw.nslots)
+        HIPCHK(hipMemcpy2DAsync(s->slots + (size_t)s->latest * lw.nslots * s->nw + (size_t)g0 * 16,
+                                (size_t)s->nw * 4, slot_words.data(), per * 4, per * 4, lw.nslots,
+                                hipMemcpyHostToDevice, st));
+    if (lw.ninit)
+        HIPCHK(hipMemcpy2DAsync(s->init + (size_t)g0 * 16, (size_t)s->nw * 4, init_words.data(),
+                                per * 4, per * 4, lw.ninit, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 1;
+}
+
+}  // namespace
+
+extern "C" int gpu_state_reset(gpu_state *s, char *err, size_t errlen)
+{
+    // power-on: all 0, RR 0; ONE and id cells are constants the kernels synthesise
+    gpu_code *c = s->c;
+    HIPCHK(hipSetDevice(c->e->device));
+    s->latest = 1;
+    size_t slot_bytes = (size_t)2 * c->lw.nslots * s->nw * 4, init_bytes = (size_t)c->lw.ninit * s->nw * 4;
+    if (slot_bytes)
+        HIPCHK(hipMemsetAsync(s->slots, 0, slot_bytes, c->e->stream));
+    if (init_bytes)
+        HIPCHK(hipMemsetAsync(s->init, 0, init_bytes, c->e->stream));
+    HIPCHK(hipStreamSynchronize(c->e->stream));
+    return 1;
+}
+
+extern "C" int gpu_state_put(gpu_state *s, unsigned group0, unsigned ngroups, const ts_line *mem,
+                             const ts_line *rr, char *err, size_t errlen)
+{
+    if ((uint64_t)group0 + ngroups > s->groups) {
+        say(err, errlen, "groups %u..%u outside the state's %u", group0, group0 + ngroups, s->groups);
+        return 0;
+    }
+    HIPCHK(hipSetDevice(s->c->e->device));
+    return put_lines(s, group0, ngroups, mem, rr, err, errlen);
+}
+
+extern "C" int gpu_state_get(gpu_state *s, unsigned group0, unsigned ngroups, ts_line *mem,
+                             ts_line *rr, char *err, size_t errlen)
+{
+    if ((uint64_t)group0 + ngroups > s->groups) {
+        say(err, errlen, "groups %u..%u outside the state's %u", group0, group0 + ngroups, s->groups);
+        return 0;
+    }
+    gpu_code *c = s->c;
+    const Lowered &lw = c->lw;
+    HIPCHK(hipSetDevice(c->e->device));
+    const size_t per = (size_t)ngroups * 16;
+    std::vector<uint32_t> slot_words((size_t)lw.nslots * per), init_words((size_t)lw.ninit * per);
+    hipStream_t st = c->e->stream;
+    if (lw.nslots)
+        HIPCHK(hipMemcpy2DAsync(slot_words.data(), per * 4,
+                                s->slots + (size_t)s->latest * lw.nslots * s->nw + (size_t)group0 * 16,
+                                (size_t)s->nw * 4, per * 4, lw.nslots, hipMemcpyDeviceToHost, st));
+    if (lw.ninit)
+        HIPCHK(hipMemcpy2DAsync(init_words.data(), per * 4, s->init + (size_t)group0 * 16,
+                                (size_t)s->nw * 4, per * 4, lw.ninit, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (unsigned g = 0; g < ngroups; g++) {
+        for (uint32_t cell = 0; cell <= lw.ncells; cell++) {
+            ts_line *line = cell == lw.ncells ? &rr[g] : &mem[(size_t)g * lw.ncells + cell];
+            int32_t fs = lw.final_slot[cell], is = lw.init_slot[cell];
+            if (fs >= 0 || is >= 0) {
+                const uint32_t *src = fs >= 0 ? &slot_words[(size_t)fs * per] : &init_words[(size_t)is * per];
+                memcpy(line->w, src + (size_t)g * 16, 64);
+            } else if (cell == TS_CELL_ONE) {
+                memset(line->w, 0xFF, 64);
+            } else if (cell < lw.ncells && c->kind[cell] == TS_CK_ID) {
+                uint32_t words[16];
+                for (unsigned j = 0; j < 16; j++)
+                    words[j] = id_pattern_word(c, cell, group0 + g, j);
+                memcpy(line->w, words, 64);
+            } else {
+                memset(line->w, 0, 64);
+            }
+        }
+    }
+    return 1;
+}
+
+// ---- run ------------------------------------------------------------------------------------
+
+extern "C" int gpu_run(gpu_code *c, gpu_state *s, uint64_t passes, char *err, size_t errlen)
+{
+    if (s->c != c) {
+        say(err, errlen, "the state belongs to another program");
+        return 0;
+    }
+    gpu_engine *e = c->e;
+    HIPCHK(hipSetDevice(e->device));
+    const Lowered &lw = c->lw;
+    const unsigned block = c->tune.block;
+    HIPCHK(hipEventRecord(e->ev0, e->stream));
+    uint32_t nw = s->nw;
+    if (lw.resident) {
+        uint64_t left = passes;
+        while (left) {
+            uint32_t now = (uint32_t)std::min<uint64_t>(left, 1u << 30);
+            uint32_t *prev = s->slots + (size_t)s->latest * lw.nslots * nw;
+            uint32_t *cur = s->slots + (size_t)(1 - s->latest) * lw.nslots * nw;
+            void *args[] = {&prev, &cur, &s->init, &nw, &now};
+            if (!launch(e, c->fns[0], (nw + block - 1) / block, block, args, err, errlen))
+                return 0;
+            s->latest = 1 - s->latest;
+            left -= now;
+        }
+    } else {
+        for (uint64_t p = 0; p < passes; p++) {
+            uint32_t *prev = s->slots + (size_t)s->latest * lw.nslots * nw;
+            uint32_t *cur = s->slots + (size_t)(1 - s->latest) * lw.nslots * nw;
+            for (size_t k = 0; k < c->fns.size(); k++) {
+                if (!c->slices[k])
+                    continue;
+                uint64_t threads = (uint64_t)c->slices[k] * nw;
+                void *args[] = {&prev, &cur, &s->init, &nw};
+                if (!launch(e, c->fns[k], (unsigned)((threads + block - 1) / block), block, args, err,
+                            errlen))
+                    return 0;
+            }
+            s->latest = 1 - s->latest;
+        }
+    }
+    HIPCHK(hipEventRecord(e->ev1, e->stream));
+    HIPCHK(hipEventSynchronize(e->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    e->last_ms = ms;
+    return 1;
+}
+
+// ---- enumeration ----------------------------------------------------------------------------
+
+namespace {
+
+struct ShareInfo {
+    uint64_t space, passes, nchunks, tried, passes_run, nact_words;
+    unsigned nshare;
+};
+
+// reference src/enum.c:215-242: passes of 512 in chunks of 64; chunk c belongs to share c % parts
+ShareInfo shares(unsigned n, int part, int part_end, int parts)
+{
+    ShareInfo si{};
+    si.space = 1ull << n;
+    si.passes = (si.space + 511) / 512;
+    si.nchunks = (si.passes + 63) / 64;
+    si.nshare = (unsigned)(part_end - part + 1);
+    const uint64_t last = si.nchunks - 1;
+    for (int r = part; r <= part_end; r++) {
+        if ((uint64_t)r >= si.nchunks)
+            continue;
+        uint64_t count = (si.nchunks - (uint64_t)r + (uint64_t)parts - 1) / (uint64_t)parts;
+        si.tried += count << 15;
+        si.passes_run += count * 64;
+    }
+    if ((int)(last % (uint64_t)parts) >= part && (int)(last % (uint64_t)parts) <= part_end) {
+        uint64_t size = si.space - (last << 15);
+        si.tried -= (1ull << 15) - size;
+        si.passes_run -= 64 - (si.passes - last * 64);
+    }
+    if (parts == 1)
+        si.nact_words = (si.space + 31) / 32;
+    else
+        si.nact_words = ((si.nchunks + (uint64_t)parts - 1) / (uint64_t)parts) * si.nshare * 1024;
+    return si;
+}
+
+}  // namespace
+
+namespace {
+
+// validation (the reference's messages, src/enum.c:244-268), spec parsing and code generation
+int enum_prepare(const gpu_engine *e, const gpu_enum *en, gpu_tune &tune, KernelSource &ks,
+                 char *err, size_t errlen)
+{
+    const ts_program *a = en->a, *b = en->b;
+    if (a->special || (en->mode == TS_EN_EQUIV && b && b->special)) {
+        say(err, errlen, "id and link cells connect the lanes, but every input needs a lane of its "
+                         "own here");
+        return 0;
+    }
+    size_t n = a->ninputs;
+    if (n > TS_EN_MAX_IN) {
+        say(err, errlen, "%zu inputs is too many to try them all (at most %d)", n, TS_EN_MAX_IN);
+        return 0;
+    }
+    if (en->parts < 1 || en->part < 0 || en->part > en->part_end || en->part_end >= en->parts ||
+        en->ticks < 1) {
+        say(err, errlen, "bad --part or --ticks");
+        return 0;
+    }
+    if (en->mode == TS_EN_EQUIV && !b) {
+        say(err, errlen, "equiv needs two programs");
+        return 0;
+    }
+    std::vector<int> in_map(n + 1), out_map(a->noutputs + 1);
+    if (en->mode == TS_EN_EQUIV &&
+        !ts_enum_match(a, b, in_map.data(), out_map.data(), err, errlen))
+        return 0;
+    ts_spec *spec = nullptr;
+    EnumLowering el;
+    el.mode = en->mode;
+    el.n = (unsigned)n;
+    el.ticks = en->ticks;
+    if (en->mode != TS_EN_EQUIV) {
+        spec = ts_spec_parse(en->spec ? en->spec : "", a, en->mode == TS_EN_VERIFY, err, errlen);
+        if (!spec)
+            return 0;
+        el.width = ts_spec_width(spec);
+        for (int bit = 0; bit < el.width; bit++)
+            el.out_cells_a.push_back(ts_spec_out_cell(spec, bit));
+        if (en->mode == TS_EN_SEARCH)
+            el.target = ts_spec_value(spec, 0);
+    }
+    tune = defaults(e, en->tune);
+    CodegenOptions co;
+    co.block = tune.block;
+    co.lut3 = tune.lut3 == 1;
+    std::string msg;
+    bool ok = codegen_enum(a, b, in_map.data(), out_map.data(), el, co, ks, msg, spec);
+    ts_spec_free(spec);
+    if (!ok) {
+        say(err, errlen, "%s", msg.c_str());
+        return 0;
+    }
+    return 1;
+}
+
+}  // namespace
+
+extern "C" int gpu_enum_source(const gpu_enum *en, char **source, char *err, size_t errlen)
+{
+    gpu_engine none;
+    none.cus = 256;
+    gpu_tune tune;
+    KernelSource ks;
+    if (!enum_prepare(&none, en, tune, ks, err, errlen))
+        return 0;
+    *source = strdup(ks.text.c_str());
+    return 1;
+}
+
+extern "C" int gpu_enum_run(gpu_engine *e, const gpu_enum *en, ts_enum_result *r, char *err,
+                            size_t errlen)
+{
+    memset(r, 0, sizeof *r);
+    const ts_program *a = en->a, *b = en->b;
+    const size_t n = a->ninputs;
+    gpu_tune tune;
+    KernelSource ks;
+    if (!enum_prepare(e, en, tune, ks, err, errlen))
+        return 0;
+    hipModule_t mod = nullptr;
+    double csecs = 0;
+    if (!build_module(e, ks.text, &mod, &csecs, err, errlen))
+        return 0;
+    struct ModGuard {
+        hipModule_t m;
+        ~ModGuard() { if (m) hipModuleUnload(m); }
+    } guard{mod};
+    hipFunction_t fn;
+    HIPCHK(hipModuleGetFunction(&fn, mod, "ts_enum"));
+
+    ShareInfo si = shares((unsigned)n, en->part, en->part_end, en->parts);
+    r->space = si.space;
+    r->tried = si.tried;
+    r->ops = si.passes_run * en->ticks * (a->len + (en->mode == TS_EN_EQUIV ? b->len : 0)) * 512;
+    r->threads = 1;
+
+    const uint64_t cap = 1u << 16;
+    uint64_t *d_out = nullptr, *d_list = nullptr;
+    HIPCHK(hipMalloc(&d_out, 2 * sizeof(uint64_t)));
+    HIPCHK(hipMalloc(&d_list, 2 * cap * sizeof(uint64_t)));
+    struct Free {
+        void *p;
+        ~Free() { if (p) hipFree(p); }
+    } f1{d_out}, f2{d_list};
+
+    auto run = [&](uint64_t i0, uint64_t i1, uint64_t *hits, uint64_t *nlist,
+                   std::vector<uint64_t> &list, bool timed) -> int {
+        HIPCHK(hipMemsetAsync(d_out, 0, 2 * sizeof(uint64_t), e->stream));
+        uint64_t blocks = (i1 - i0 + tune.block - 1) / tune.block;
+        unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, tune.grid));
+        uint64_t space = si.space, capv = cap;
+        uint32_t parts = (uint32_t)en->parts, part = (uint32_t)en->part, nshare = si.nshare;
+        void *args[] = {&d_out, &d_list, &capv, &i0, &i1, &space, &parts, &part, &nshare};
+        if (timed)
+            HIPCHK(hipEventRecord(e->ev0, e->stream));
+        if (!launch(e, fn, grid, tune.block, args, err, errlen))
+            return 0;
+        if (timed)
+            HIPCHK(hipEventRecord(e->ev1, e->stream));
+        uint64_t host[2];
+        HIPCHK(hipMemcpyAsync(host, d_out, sizeof host, hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+        if (timed) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+            e->last_ms = ms;
+        }
+        *hits = host[0];
+        *nlist = host[1];
+        uint64_t keep = std::min<uint64_t>(host[1], cap);
+        list.resize(2 * keep);
+        if (keep)
+            HIPCHK(hipMemcpy(list.data(), d_list, 2 * keep * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        return 1;
+    };
+    auto collect = [&](const std::vector<uint64_t> &list) {
+        for (size_t i = 0; i + 1 < list.size(); i += 2)
+            for (uint32_t m = (uint32_t)list[i + 1]; m; m &= m - 1)
+                ts_enum_add_hit(r, list[i] * 32 + (uint64_t)__builtin_ctz(m));
+    };
+    // Words are visited in increasing x within the share, so a range's hits are all smaller than
+    // the next range's: when a range has more hit words than the list holds, its left half is
+    // listed first and the right half only while fewer than 16 hits are known.
+    std::vector<uint64_t> list;
+    std::function<int(uint64_t, uint64_t)> collect_range = [&](uint64_t i0, uint64_t i1) -> int {
+        if (i0 >= i1 || r->nhit >= TS_EN_SHOW)
+            return 1;
+        uint64_t h = 0, nl = 0;
+        if (!run(i0, i1, &h, &nl, list, false))
+            return 0;
+        if (nl <= cap) {
+            collect(list);
+            return 1;
+        }
+        uint64_t mid = i0 + (i1 - i0) / 2;
+        return collect_range(i0, mid) && collect_range(mid, i1);
+    };
+
+    uint64_t hits = 0, nlist = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    if (!run(0, si.nact_words, &hits, &nlist, list, true))
+        return 0;
+    r->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    r->hits = hits;
+    if (nlist <= cap)
+        collect(list);
+    else if (!collect_range(0, si.nact_words))
+        return 0;
+    return 1;
+}

@@ -1,0 +1,365 @@
+// Program -> AIG -> segments and slices. See lower.hpp.
+#include "lower.hpp"
+
+#include <algorithm>
+#include <cstdio>
+
+namespace ts {
+
+static inline uint64_t pair_key(uint32_t a, uint32_t b) { return (uint64_t)a << 32 | b; }
+
+Aig::Aig()
+{
+    nodes.push_back(Node{N_CONST, 0, 0});
+}
+
+Lit Aig::And(Lit x, Lit y)
+{
+    if (x > y)
+        std::swap(x, y);
+    if (x == 0)
+        return 0;          // false & y
+    if (x == 1)
+        return y;          // true & y
+    if (x == y)
+        return x;
+    if (x == lit_not(y))
+        return 0;
+    uint64_t k = pair_key(x, y);
+    auto it = and_table.find(k);
+    if (it != and_table.end())
+        return it->second << 1;
+    uint32_t id = (uint32_t)nodes.size();
+    nodes.push_back(Node{N_AND, x, y});
+    and_table.emplace(k, id);
+    return id << 1;
+}
+
+Lit Aig::Leaf(uint8_t kind, uint32_t a, uint32_t b)
+{
+    uint64_t k = (uint64_t)kind << 56 ^ (uint64_t)a << 24 ^ b;  // a < 2^32, b < 2^16 here
+    auto it = leaf_table.find(k);
+    if (it != leaf_table.end()) {
+        const Node &n = nodes[it->second];
+        if (n.kind == kind && n.a == a && n.b == b)
+            return it->second << 1;
+    }
+    uint32_t id = (uint32_t)nodes.size();
+    nodes.push_back(Node{kind, a, b});
+    leaf_table[k] = id;
+    return id << 1;
+}
+
+// The value of x in machine k ^ mask.
+Lit Aig::Perm(Lit x, uint32_t mask)
+{
+    if (mask == 0)
+        return x;
+    const Node n = nodes[x >> 1];
+    Lit c = x & 1u;
+    switch (n.kind) {
+    case N_CONST:
+        return x;
+    case N_LOAD:
+        return Load(n.a, n.b ^ mask) ^ c;
+    case N_IDBIT:
+        return IdBit(n.a) ^ c ^ ((mask >> n.a) & 1u);
+    case N_PERM:
+        return Perm(n.a, n.b ^ mask) ^ c;
+    default:
+        return Leaf(N_PERM, x & ~1u, mask) ^ c;
+    }
+}
+
+// ---- lowering ------------------------------------------------------------------------------
+
+namespace {
+
+struct PassBuilder {
+    const ts_program *p;
+    const LowerOptions &opt;
+    Lowered &lw;
+    uint32_t RR;
+    std::vector<Lit> cur;
+    std::vector<uint8_t> valid;          // written in the current segment
+    std::vector<int> last_write;         // segment of the last write so far in this pass
+    std::vector<std::vector<uint32_t>> version_ids;  // per cell: (seg + 1) -> version id
+    std::vector<uint32_t> written_order; // cells written in the current segment, first-write order
+
+    PassBuilder(const ts_program *p_, const LowerOptions &o, Lowered &l)
+        : p(p_), opt(o), lw(l), RR(p_->ncells)
+    {
+        cur.assign(RR + 1, 0);
+        valid.assign(RR + 1, 0);
+        last_write.assign(RR + 1, -1);
+        version_ids.resize(RR + 1);
+        lw.segs.emplace_back();
+    }
+
+    Aig &g() { return lw.segs.back().g; }
+    int seg() const { return (int)lw.segs.size() - 1; }
+
+    uint32_t version(uint32_t cell, int s)
+    {
+        auto &v = version_ids[cell];
+        size_t at = (size_t)(s + 1);
+        if (v.size() <= at)
+            v.resize(at + 1, UINT32_MAX);
+        if (v[at] == UINT32_MAX) {
+            v[at] = (uint32_t)lw.versions.size();
+            lw.versions.push_back(Version{cell, s});
+        }
+        return v[at];
+    }
+
+    Lit value(uint32_t cell)
+    {
+        if (valid[cell])
+            return cur[cell];
+        return g().Load(version(cell, last_write[cell]), 0);
+    }
+
+    void write(uint32_t cell, Lit v)
+    {
+        if (!valid[cell]) {
+            valid[cell] = 1;
+            written_order.push_back(cell);
+        }
+        cur[cell] = v;
+    }
+
+    void close_segment()
+    {
+        Segment &s = lw.segs.back();
+        for (uint32_t c : written_order) {
+            s.outs.push_back(SegOut{c, cur[c], -1});
+            last_write[c] = seg();
+            valid[c] = 0;
+        }
+        written_order.clear();
+    }
+
+    void cut()
+    {
+        close_segment();
+        lw.segs.emplace_back();
+    }
+
+    bool run(std::string &err)
+    {
+        for (size_t i = 0; i < p->len; i++) {
+            const ts_instr in = p->code[i];
+            if (in.addr >= p->ncells) {
+                char b[96];
+                snprintf(b, sizeof b, "instruction %zu reads cell %u of %u", i, in.addr, p->ncells);
+                err = b;
+                return false;
+            }
+            if (in.op == TS_OP_STO) {
+                if (!ts_readonly(in.addr))
+                    write(in.addr, value(RR));
+                continue;
+            }
+            unsigned mask = 0;
+            uint32_t c = resolve(in.addr, &mask);
+            Lit v;
+            if (c == TS_CELL_ZERO)
+                v = 0;
+            else if (c == TS_CELL_ONE)
+                v = 1;
+            else if (p->special && p->special[c].kind == TS_CK_ID)
+                v = g().IdBit(p->special[c].dim) ^ ((mask >> p->special[c].dim) & 1u);
+            else if (mask == 0)
+                v = value(c);
+            else {
+                Lit base = value(c);
+                if ((mask >> lw.wave_dims) && g().derived(base)) {
+                    cut();
+                    base = value(c);
+                }
+                v = g().Perm(base, mask);
+            }
+            Lit r = value(RR);
+            write(RR, lit_not(g().And(r, v)));
+        }
+        close_segment();
+        return true;
+    }
+
+    bool ts_readonly(uint32_t c) const
+    {
+        return c < TS_CELL_FIRST_FREE || (p->special && p->special[c].kind != TS_CK_PLAIN);
+    }
+
+    uint32_t resolve(uint32_t c, unsigned *mask) const
+    {
+        unsigned m = 0;
+        while (p->special && p->special[c].kind == TS_CK_LINK) {
+            m ^= 1u << p->special[c].dim;
+            c = p->special[c].src;
+        }
+        *mask = m;
+        return c;
+    }
+};
+
+// Nodes reachable from the given literals, in index order.
+static void cone(const Aig &g, const std::vector<Lit> &roots, std::vector<uint8_t> &mark)
+{
+    mark.assign(g.nodes.size(), 0);
+    std::vector<uint32_t> stack;
+    for (Lit r : roots)
+        stack.push_back(r >> 1);
+    while (!stack.empty()) {
+        uint32_t n = stack.back();
+        stack.pop_back();
+        if (mark[n])
+            continue;
+        mark[n] = 1;
+        const Node &nd = g.nodes[n];
+        if (nd.kind == N_AND) {
+            stack.push_back(nd.a >> 1);
+            stack.push_back(nd.b >> 1);
+        } else if (nd.kind == N_PERM) {
+            stack.push_back(nd.a >> 1);
+        }
+    }
+}
+
+// Splits a segment's stored outputs into slices of about `target` AIG nodes each, in output order
+// (program order keeps neighbouring cells, which share inputs, together).
+static void make_slices(Segment &s, unsigned target)
+{
+    std::vector<uint32_t> stored;
+    for (uint32_t i = 0; i < s.outs.size(); i++)
+        if (s.outs[i].slot >= 0)
+            stored.push_back(i);
+    s.slices.clear();
+    if (stored.empty())
+        return;
+    if (!target) {
+        s.slices.push_back(stored);
+        return;
+    }
+    // visit order: by the output's node index, so cells computed together stay together
+    std::stable_sort(stored.begin(), stored.end(), [&](uint32_t x, uint32_t y) {
+        return (s.outs[x].v >> 1) < (s.outs[y].v >> 1);
+    });
+    std::vector<uint8_t> mark(s.g.nodes.size(), 0);
+    std::vector<uint32_t> touched, stack;
+    std::vector<uint32_t> slice;
+    size_t size = 0;
+    for (uint32_t oi : stored) {
+        stack.push_back(s.outs[oi].v >> 1);
+        size_t added = 0;
+        while (!stack.empty()) {
+            uint32_t n = stack.back();
+            stack.pop_back();
+            if (mark[n])
+                continue;
+            mark[n] = 1;
+            touched.push_back(n);
+            const Node &nd = s.g.nodes[n];
+            if (nd.kind == N_AND) {
+                added++;
+                stack.push_back(nd.a >> 1);
+                stack.push_back(nd.b >> 1);
+            } else if (nd.kind == N_PERM) {
+                added++;
+                stack.push_back(nd.a >> 1);
+            }
+        }
+        slice.push_back(oi);
+        size += added;
+        if (size >= target) {
+            s.slices.push_back(slice);
+            slice.clear();
+            size = 0;
+            for (uint32_t n : touched)
+                mark[n] = 0;
+            touched.clear();
+        }
+    }
+    if (!slice.empty())
+        s.slices.push_back(slice);
+}
+
+}  // namespace
+
+bool lower_program(const ts_program *p, const LowerOptions &opt, Lowered &lw, std::string &err)
+{
+    lw = Lowered();
+    lw.ncells = p->ncells;
+    unsigned dims = TS_CUBE_DIMS;
+    for (uint32_t c = 0; p->special && c < p->ncells; c++)
+        if (p->special[c].kind != TS_CK_PLAIN && p->special[c].dim + 1u > dims)
+            dims = p->special[c].dim + 1u;
+    lw.dims = dims;
+    unsigned wl = opt.cube_words_log2 < 6 ? opt.cube_words_log2 : 6;
+    lw.wave_dims = 5 + wl;
+
+    PassBuilder pb(p, opt, lw);
+    if (!pb.run(err))
+        return false;
+
+    const uint32_t total = p->ncells + 1;
+    lw.last_seg = pb.last_write;
+
+    // where every version lives
+    lw.ver_where.resize(lw.versions.size());
+    std::vector<std::vector<int32_t>> slot_of(total);  // per cell: (seg) -> slot
+    auto need_slot = [&](uint32_t cell, int s) {
+        auto &v = slot_of[cell];
+        if (v.size() <= (size_t)s)
+            v.resize((size_t)s + 1, -1);
+        if (v[(size_t)s] < 0)
+            v[(size_t)s] = (int32_t)lw.nslots++;
+        return v[(size_t)s];
+    };
+    lw.final_slot.assign(total, -1);
+    lw.init_slot.assign(total, -1);
+    for (uint32_t c = 0; c < total; c++) {
+        bool plain = c == p->ncells ||
+                     (c >= TS_CELL_FIRST_FREE && !(p->special && p->special[c].kind != TS_CK_PLAIN));
+        if (!plain)
+            continue;
+        if (lw.last_seg[c] >= 0)
+            lw.final_slot[c] = need_slot(c, lw.last_seg[c]);
+        else
+            lw.init_slot[c] = (int32_t)lw.ninit++;
+    }
+    lw.ver_slot.resize(lw.versions.size());
+    for (size_t v = 0; v < lw.versions.size(); v++) {
+        const Version &ver = lw.versions[v];
+        if (ver.seg >= 0) {
+            lw.ver_where[v] = WHERE_CUR;
+            lw.ver_slot[v] = need_slot(ver.cell, ver.seg);
+        } else if (lw.last_seg[ver.cell] >= 0) {
+            lw.ver_where[v] = WHERE_PREV;
+            lw.ver_slot[v] = lw.final_slot[ver.cell];
+        } else {
+            lw.ver_where[v] = WHERE_INIT;
+            lw.ver_slot[v] = lw.init_slot[ver.cell];
+            if (lw.ver_slot[v] < 0) {
+                err = "internal: a read of a cell without storage";
+                return false;
+            }
+        }
+    }
+    for (size_t s = 0; s < lw.segs.size(); s++)
+        for (SegOut &o : lw.segs[s].outs) {
+            auto &v = slot_of[o.cell];
+            o.slot = s < v.size() ? v[s] : -1;
+        }
+
+    // resident: one segment and every read of a start-of-pass value inside the wave
+    lw.resident = lw.segs.size() == 1;
+    for (const Node &n : lw.segs[0].g.nodes)
+        if (n.kind == N_LOAD && (n.b >> lw.wave_dims))
+            lw.resident = false;
+    for (Segment &s : lw.segs)
+        make_slices(s, lw.resident ? 0 : opt.slice_nodes);
+    return true;
+}
+
+}  // namespace ts

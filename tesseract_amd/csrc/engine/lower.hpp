@@ -1,0 +1,143 @@
+// Lowering of NAND/STO programs to straight-line gfx950 kernels.
+//
+// A pass of a program is a Boolean function of the state at the start of the pass. Symbolic
+// execution of the instruction list turns it into an And-Inverter Graph (AIG): every NAND is an
+// AND node with complemented edges, a STO only renames, constants fold, and structural hashing
+// shares repeated logic. The graph is then printed as HIP source where every value is one 32-bit
+// word = 32 machines (machine k of a word is bit k), so one VALU op evaluates one gate for 32
+// machines. gfx950's v_bitop3_b32 evaluates any 3-input function, so 3-input cuts of the graph are
+// mapped to one instruction each.
+//
+// Cube programs read cells of other machines (links). Machine m lives in word m >> 5, bit m & 31;
+// a read across dims 0-4 is an in-word bit permutation, across dims 5-10 a cross-lane move inside
+// the 64-lane wave, and across dims >= 11 a read of another wave's memory. Values computed inside a
+// kernel can only be moved within the wave, so a read across dims >= 11 of a cell written earlier
+// in the pass cuts the pass into segments (one launch each); values cross segment boundaries
+// through HBM in per-(cell, segment) slots with two parities, so no launch ever reads a location
+// another wave of the same launch writes (the reference's segment rule, src/jit.c:521-560,
+// generalised from "dims >= 9" to "dims the wave cannot see").
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "tesseract_host.h"
+
+namespace ts {
+
+typedef uint32_t Lit;  // node << 1 | complemented
+inline Lit lit_not(Lit x) { return x ^ 1u; }
+
+enum NodeKind : uint8_t {
+    N_CONST = 0,  // node 0: false
+    N_AND,        // a, b: literals
+    N_LOAD,       // a: version id, b: machine mask (value of that version in machine k ^ mask)
+    N_INBIT,      // a: bit j of the enumeration assignment x
+    N_IDBIT,      // a: bit d of the machine number within its cube
+    N_PERM,       // a: literal (an AND), b: machine mask 1..2047 (dims 0-10)
+};
+
+struct Node {
+    uint8_t  kind;
+    uint32_t a, b;
+};
+
+struct Aig {
+    std::vector<Node> nodes;
+    std::unordered_map<uint64_t, uint32_t> and_table;
+    std::unordered_map<uint64_t, uint32_t> leaf_table;
+
+    Aig();
+    Lit And(Lit x, Lit y);
+    Lit Or(Lit x, Lit y) { return lit_not(And(lit_not(x), lit_not(y))); }
+    Lit Xor(Lit x, Lit y) { return Or(And(x, lit_not(y)), And(lit_not(x), y)); }
+    Lit Leaf(uint8_t kind, uint32_t a, uint32_t b);
+    Lit Load(uint32_t version, uint32_t mask) { return Leaf(N_LOAD, version, mask); }
+    Lit InBit(uint32_t j) { return Leaf(N_INBIT, j, 0); }
+    Lit IdBit(uint32_t d) { return Leaf(N_IDBIT, d, 0); }
+    Lit Perm(Lit x, uint32_t mask);
+    uint8_t kind_of(Lit x) const { return nodes[x >> 1].kind; }
+    bool derived(Lit x) const { uint8_t k = kind_of(x); return k == N_AND || k == N_PERM; }
+};
+
+// A value of a cell at a segment boundary: written in segment `seg` of this pass, or (seg = -1)
+// the value at the start of the pass.
+struct Version {
+    uint32_t cell;
+    int      seg;
+};
+
+struct SegOut {
+    uint32_t cell;  // ncells = RR
+    Lit      v;
+    int32_t  slot;  // storage slot, -1 if nobody reads this version
+};
+
+struct Segment {
+    Aig                 g;
+    std::vector<SegOut> outs;
+    // slices: output index ranges, each computed by its own threads (cones duplicated)
+    std::vector<std::vector<uint32_t>> slices;
+};
+
+// Where a version lives at run time.
+enum { WHERE_CUR = 0, WHERE_PREV = 1, WHERE_INIT = 2 };
+
+struct Lowered {
+    uint32_t ncells = 0;     // RR is cell ncells
+    unsigned dims = 9;       // cube dimensions
+    unsigned wave_dims = 0;  // dims < wave_dims are in-thread or in-wave
+    std::vector<Version> versions;
+    std::vector<uint8_t> ver_where;
+    std::vector<int32_t> ver_slot;
+    std::vector<Segment> segs;
+    std::vector<int>     last_seg;   // per cell incl. RR: last segment writing it, -1 = never
+    std::vector<int32_t> final_slot; // per cell incl. RR: slot of the value after a pass, -1 = init
+    std::vector<int32_t> init_slot;  // per cell incl. RR: slot in the init area, -1 = none
+    uint32_t nslots = 0, ninit = 0;
+    bool     resident = false;       // one segment, every read in-wave: the pass loop runs in-kernel
+};
+
+struct LowerOptions {
+    unsigned cube_words_log2;   // log2(words per cube) = dims - 5
+    unsigned slice_nodes;       // target AIG nodes per slice (0 = no slicing)
+};
+
+// Lowers one pass of p into segments. Returns false with a message for programs it cannot take.
+bool lower_program(const ts_program *p, const LowerOptions &opt, Lowered &out, std::string &err);
+
+// ---- code generation --------------------------------------------------------------------------
+
+struct KernelSource {
+    std::string text;
+    std::vector<std::string> names;  // kernel names in launch order (one per segment)
+    size_t and_nodes = 0, bitop3 = 0, values = 0;
+};
+
+struct CodegenOptions {
+    unsigned block = 256;
+    bool     lut3 = true;  // map 3-input cuts to v_bitop3_b32
+};
+
+// Streamed form: one kernel per segment, state in HBM slots.
+// Resident form (out.resident): one kernel, the pass loop inside, state in registers.
+void codegen_run(const Lowered &lw, const CodegenOptions &opt, KernelSource &ks);
+
+// Exhaustive enumeration kernel for link-free programs (verify/equiv/search).
+struct EnumLowering {
+    int      mode;            // TS_EN_*
+    unsigned n;               // inputs
+    unsigned ticks;
+    // search: target bits; verify: spec lowered per lane; equiv: two programs
+    uint64_t target = 0;
+    int      width = 0;
+    std::vector<uint32_t> out_cells_a;  // search/verify: spec output cells (lowest bit first)
+};
+
+bool codegen_enum(const ts_program *a, const ts_program *b, const int *in_map, const int *out_map,
+                  const EnumLowering &el, const CodegenOptions &opt, KernelSource &ks,
+                  std::string &err, const ts_spec *spec);
+
+}  // namespace ts

@@ -1,0 +1,93 @@
+"""Parity of gpu_run with the reference machine: the reference's check protocol (random state for
+every writable cell and RR of every machine, several passes, every machine compared), on link-free
+programs and on cube programs whose links cross in-word, in-wave and cross-wave dimensions."""
+import numpy as np
+import pytest
+
+import tesseract_amd as t
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def check(engine, prog, groups=None, ticks=8, seed=1, tune=None, runs=(1,)):
+    code = engine.compile(prog, tune)
+    groups = groups or prog.groups
+    mem, rr = O.random_lines(prog, groups, seed)
+    st = code.state(groups)
+    st.put(mem, rr)
+    done = 0
+    for n in runs:  # several gpu_run calls continue the same machines
+        st.run(n)
+        done += n
+    want_mem, want_rr = O.run_lines(prog, mem, rr, done)
+    got_mem, got_rr = st.get()
+    assert np.array_equal(got_rr, want_rr), "RR differs"
+    bad = np.argwhere(got_mem != want_mem)
+    assert bad.size == 0, "cells differ, first (group, cell, word) %s of %d" % (bad[0], len(bad))
+    return code
+
+
+def random_cube_program(cells, length, dims, seed, links=24):
+    """A random program with id and link cells reaching every dimension below dims."""
+    rng = np.random.default_rng(seed)
+    b = t.Builder()
+    plain = [b.cell("c%d" % i) for i in range(cells)]
+    ids = [b.cell("ID%d" % d) for d in range(dims)]
+    lk = []
+    for k in range(links):
+        d = int(rng.integers(0, dims)) if k else dims - 1
+        src = plain[int(rng.integers(0, cells))] if k % 5 else ids[int(rng.integers(0, dims))]
+        lk.append(b.link("l%d" % k, src, d))
+    for k in range(4):  # chains of links add up
+        lk.append(b.link("m%d" % k, lk[int(rng.integers(0, len(lk)))], int(rng.integers(0, dims))))
+    readable = [0, 1] + plain + ids + lk
+    for _ in range(length):
+        if rng.random() < 0.55:
+            b.nand(readable[int(rng.integers(0, len(readable)))])
+        else:
+            b.sto(plain[int(rng.integers(0, cells))])
+    return b.finish()
+
+
+@pytest.mark.parametrize("name", ["adder8", "mul4", "hash16", "rand20", "rand400"])
+def test_linkfree(engine, name):
+    prog = {"adder8": lambda: t.gen_adder(8), "mul4": lambda: t.gen_mul(4),
+            "hash16": lambda: t.gen_hash(16, 5), "rand20": lambda: t.random_program(20, 500, 42),
+            "rand400": lambda: t.random_program(400, 3000, 7)}[name]()
+    print("check", name, flush=True)
+    check(engine, prog, groups=4, ticks=8, runs=(3, 5))
+
+
+def test_linkfree_many_groups(engine):
+    prog = t.random_program(50, 300, 9)
+    print("check rand50 x 64 groups", flush=True)
+    check(engine, prog, groups=64, runs=(4,))
+
+
+@pytest.mark.parametrize("dims", [9, 10, 11, 12])
+def test_random_cube(engine, dims):
+    prog = random_cube_program(24, 400, dims, seed=100 + dims)
+    print("check random cube dims", dims, flush=True)
+    check(engine, prog, runs=(2, 3))
+
+
+@pytest.mark.parametrize("dims", [9, 12])
+def test_rank(engine, dims):
+    print("check rank", dims, flush=True)
+    check(engine, t.gen_rank(dims), runs=(1, 1))
+
+
+def test_perm8_sort8(engine):
+    print("check perm8/sort8", flush=True)
+    check(engine, t.gen_perm8(), runs=(2,))
+    check(engine, t.gen_sort8(), groups=2, runs=(1,))
+
+
+def test_life_cube_small(engine):
+    prog, board = t.gen_life_cube(4, 4, 3, 3)  # 64 machines? no: dims = max(9, 6) -> one group
+    print("check life cube 4x4 on 3+3 dims", flush=True)
+    check(engine, prog, runs=(2,))
+    prog, board = t.gen_life_cube(4, 4, 6, 6)  # dims 12: cross-wave links, segments
+    print("check life cube 4x4 on 6+6 dims", flush=True)
+    check(engine, prog, runs=(1, 2))

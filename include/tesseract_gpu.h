@@ -92,6 +92,16 @@ typedef struct {
 
 int  gpu_enum_run(gpu_engine *e, const gpu_enum *en, ts_enum_result *r, char *err, size_t errlen);
 
+/* gpu_enum_run in two steps, so one compiled enumeration can run many times: gpu_enum_compile
+   validates and builds the kernel (the reference's enum_run does jit_compile first,
+   src/enum.c:276-288); gpu_enum_exec runs the whole enumeration. The gpu_enum_code keeps copies of
+   what it needs from the programs. */
+typedef struct gpu_enum_code gpu_enum_code;
+int  gpu_enum_compile(gpu_engine *e, const gpu_enum *en, gpu_enum_code **out, char *err,
+                      size_t errlen);
+int  gpu_enum_exec(gpu_enum_code *c, ts_enum_result *r, char *err, size_t errlen);
+void gpu_enum_free(gpu_enum_code *c);
+
 /* Listings without a device (the counterpart of `alpha jit`, reference src/main.c:468-496):
    the HIP source gpu_compile / gpu_enum_run would build, and a hipRTC compile of any such source
    for an architecture ("gfx950"), which needs no GPU. */

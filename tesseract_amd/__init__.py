@@ -136,6 +136,9 @@ def lib():
             "gpu_enum_source": (i32, [C.POINTER(GpuEnumStruct), C.POINTER(vp), C.c_char_p, sz]),
             "gpu_rtc_check": (i32, [C.c_char_p, C.c_char_p, C.POINTER(C.c_double),
                                     C.POINTER(sz), C.c_char_p, sz]),
+            "gpu_enum_compile": (i32, [vp, C.POINTER(GpuEnumStruct), C.POINTER(vp), C.c_char_p, sz]),
+            "gpu_enum_exec": (i32, [vp, C.POINTER(EnumResultStruct), C.c_char_p, sz]),
+            "gpu_enum_free": (None, [vp]),
             "free": (None, [vp]),
         }
         for name, (res, args) in sig.items():
@@ -488,6 +491,33 @@ class Engine:
         e = _err()
         if not lib().gpu_enum_run(self._e, C.byref(en), C.byref(r), e, len(e)):
             raise ValueError(e.value.decode())
+        return _result(r)
+
+
+class EnumCode:
+    """gpu_enum_code: a compiled enumeration that can run many times."""
+
+    def __init__(self, engine, mode, a, b=None, spec=None, ticks=1, part=0, part_end=0, parts=1,
+                 tune=None):
+        self.engine = engine
+        self._keep = (a, b)
+        en = _enum_struct(mode, a, b, spec, ticks, part, part_end, parts, tune)
+        h = C.c_void_p()
+        e = _err()
+        if not lib().gpu_enum_compile(engine._e, C.byref(en), C.byref(h), e, len(e)):
+            raise ValueError(e.value.decode())
+        self._c = h
+
+    def __del__(self):
+        if getattr(self, "_c", None) and _lib is not None:
+            _lib.gpu_enum_free(self._c)
+            self._c = None
+
+    def run(self):
+        r = EnumResultStruct()
+        e = _err()
+        if not lib().gpu_enum_exec(self._c, C.byref(r), e, len(e)):
+            raise RuntimeError(e.value.decode())
         return _result(r)
 
 

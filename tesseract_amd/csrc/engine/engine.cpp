@@ -630,41 +630,82 @@ extern "C" int gpu_enum_source(const gpu_enum *en, char **source, char *err, siz
     return 1;
 }
 
-extern "C" int gpu_enum_run(gpu_engine *e, const gpu_enum *en, ts_enum_result *r, char *err,
-                            size_t errlen)
+struct gpu_enum_code {
+    gpu_engine   *e = nullptr;
+    hipModule_t   mod = nullptr;
+    hipFunction_t fn = nullptr;
+    gpu_tune      tune{};
+    int           part = 0, part_end = 0, parts = 1;
+    unsigned      n = 0, ticks = 1;
+    uint64_t      len_total = 0;  // instructions per pass of a (+ b for equiv)
+    uint64_t     *d_out = nullptr, *d_list = nullptr;
+    double        compile_seconds = 0;
+};
+
+static const uint64_t kListCap = 1u << 16;
+
+extern "C" int gpu_enum_compile(gpu_engine *e, const gpu_enum *en, gpu_enum_code **out, char *err,
+                                size_t errlen)
+{
+    *out = nullptr;
+    std::unique_ptr<gpu_enum_code> c(new gpu_enum_code());
+    c->e = e;
+    KernelSource ks;
+    if (!enum_prepare(e, en, c->tune, ks, err, errlen))
+        return 0;
+    if (!build_module(e, ks.text, &c->mod, &c->compile_seconds, err, errlen))
+        return 0;
+    auto fail = [&](void) { hipModuleUnload(c->mod); c->mod = nullptr; };
+    if (hipModuleGetFunction(&c->fn, c->mod, "ts_enum") != hipSuccess) {
+        fail();
+        say(err, errlen, "hipModuleGetFunction failed");
+        return 0;
+    }
+    c->part = en->part;
+    c->part_end = en->part_end;
+    c->parts = en->parts;
+    c->n = (unsigned)en->a->ninputs;
+    c->ticks = en->ticks;
+    c->len_total = en->a->len + (en->mode == TS_EN_EQUIV ? en->b->len : 0);
+    if (hipMalloc(&c->d_out, 2 * sizeof(uint64_t)) != hipSuccess ||
+        hipMalloc(&c->d_list, 2 * kListCap * sizeof(uint64_t)) != hipSuccess) {
+        hipFree(c->d_out);
+        fail();
+        say(err, errlen, "hipMalloc failed for the hit list");
+        return 0;
+    }
+    *out = c.release();
+    return 1;
+}
+
+extern "C" void gpu_enum_free(gpu_enum_code *c)
+{
+    if (!c)
+        return;
+    hipSetDevice(c->e->device);
+    if (c->d_out)
+        hipFree(c->d_out);
+    if (c->d_list)
+        hipFree(c->d_list);
+    if (c->mod)
+        hipModuleUnload(c->mod);
+    delete c;
+}
+
+extern "C" int gpu_enum_exec(gpu_enum_code *c, ts_enum_result *r, char *err, size_t errlen)
 {
     memset(r, 0, sizeof *r);
-    const ts_program *a = en->a, *b = en->b;
-    const size_t n = a->ninputs;
-    gpu_tune tune;
-    KernelSource ks;
-    if (!enum_prepare(e, en, tune, ks, err, errlen))
-        return 0;
-    hipModule_t mod = nullptr;
-    double csecs = 0;
-    if (!build_module(e, ks.text, &mod, &csecs, err, errlen))
-        return 0;
-    struct ModGuard {
-        hipModule_t m;
-        ~ModGuard() { if (m) hipModuleUnload(m); }
-    } guard{mod};
-    hipFunction_t fn;
-    HIPCHK(hipModuleGetFunction(&fn, mod, "ts_enum"));
-
-    ShareInfo si = shares((unsigned)n, en->part, en->part_end, en->parts);
+    gpu_engine *e = c->e;
+    HIPCHK(hipSetDevice(e->device));
+    const gpu_tune tune = c->tune;
+    ShareInfo si = shares(c->n, c->part, c->part_end, c->parts);
     r->space = si.space;
     r->tried = si.tried;
-    r->ops = si.passes_run * en->ticks * (a->len + (en->mode == TS_EN_EQUIV ? b->len : 0)) * 512;
+    r->ops = si.passes_run * c->ticks * c->len_total * 512;
     r->threads = 1;
-
-    const uint64_t cap = 1u << 16;
-    uint64_t *d_out = nullptr, *d_list = nullptr;
-    HIPCHK(hipMalloc(&d_out, 2 * sizeof(uint64_t)));
-    HIPCHK(hipMalloc(&d_list, 2 * cap * sizeof(uint64_t)));
-    struct Free {
-        void *p;
-        ~Free() { if (p) hipFree(p); }
-    } f1{d_out}, f2{d_list};
+    const uint64_t cap = kListCap;
+    uint64_t *d_out = c->d_out, *d_list = c->d_list;
+    hipFunction_t fn = c->fn;
 
     auto run = [&](uint64_t i0, uint64_t i1, uint64_t *hits, uint64_t *nlist,
                    std::vector<uint64_t> &list, bool timed) -> int {
@@ -672,7 +713,7 @@ extern "C" int gpu_enum_run(gpu_engine *e, const gpu_enum *en, ts_enum_result *r
         uint64_t blocks = (i1 - i0 + tune.block - 1) / tune.block;
         unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, tune.grid));
         uint64_t space = si.space, capv = cap;
-        uint32_t parts = (uint32_t)en->parts, part = (uint32_t)en->part, nshare = si.nshare;
+        uint32_t parts = (uint32_t)c->parts, part = (uint32_t)c->part, nshare = si.nshare;
         void *args[] = {&d_out, &d_list, &capv, &i0, &i1, &space, &parts, &part, &nshare};
         if (timed)
             HIPCHK(hipEventRecord(e->ev0, e->stream));
@@ -730,4 +771,16 @@ extern "C" int gpu_enum_run(gpu_engine *e, const gpu_enum *en, ts_enum_result *r
     else if (!collect_range(0, si.nact_words))
         return 0;
     return 1;
+}
+
+extern "C" int gpu_enum_run(gpu_engine *e, const gpu_enum *en, ts_enum_result *r, char *err,
+                            size_t errlen)
+{
+    memset(r, 0, sizeof *r);
+    gpu_enum_code *c = nullptr;
+    if (!gpu_enum_compile(e, en, &c, err, errlen))
+        return 0;
+    int ok = gpu_enum_exec(c, r, err, errlen);
+    gpu_enum_free(c);
+    return ok;
 }

@@ -84,6 +84,7 @@ def main():
     value = ops_per_step * args.steps / wall
     kavg = sum(kernel_ms) / len(kernel_ms)
     achieved = ops_per_step / (kavg * 1e-3)
+    valu_rate = eng.valu_rate()  # measured v_bitop3_b32 lane-ops/s on this device
     line = {
         "metric": "1-bit NAND/STO ops/s on 1 GPU (% int-VALU peak); mul16 2^32 verify sec",
         "value": value,
@@ -104,7 +105,9 @@ def main():
                    "compile_seconds": compile_s, "device": eng.describe()},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_1BIT,
                      "unit": "1-bit op/s", "frac": achieved / PEAK_1BIT, "traffic": None,
-                     "kernel": "ts_enum", "kernel_ms": kavg},
+                     "kernel": "ts_enum", "kernel_ms": kavg,
+                     "measured_valu_lane_ops_per_s": valu_rate,
+                     "frac_of_measured": achieved / (valu_rate * 32)},
         "cpu_baseline": None,
     }
     if not args.no_cpu:

@@ -73,6 +73,9 @@ int  gpu_state_get(gpu_state *s, unsigned group0, unsigned ngroups, ts_line *mem
 /* `passes` whole passes of every machine of the state, cubes in lockstep: the counterpart of
    jit_run (src/jit.h:54) and jit_cube_run (src/jit.h:59). */
 int  gpu_run(gpu_code *c, gpu_state *s, uint64_t passes, char *err, size_t errlen);
+/* The device's measured rate of 32-bit v_bitop3_b32 lane-operations per second (a calibration
+   launch that fills every SIMD), for stating results against the real integer-VALU roofline. */
+int  gpu_valu_rate(gpu_engine *e, double *lane_ops_per_s, char *err, size_t errlen);
 /* Milliseconds of device time the last gpu_run / gpu_enum_run took (HIP events on the engine's
    stream). */
 double gpu_last_ms(const gpu_engine *e);

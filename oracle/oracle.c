@@ -406,3 +406,43 @@ uint64_t oracle_toyhash(uint64_t x, int n, int rounds)
     }
     return x;
 }
+
+/* B3/S23 on a torus of h rows x 64*ww columns, 64 cells per word (cell c of a row is bit c % 64 of
+   word c / 64): the neighbour count is summed bit-parallel with half/full adders into bits
+   (ones, twos, fours); pinned against oracle_life_scalar. */
+void oracle_life_words(uint64_t *b, int ww, int h, int gens)
+{
+    uint64_t *next = malloc((size_t)ww * (size_t)h * sizeof *next);
+    for (int g = 0; g < gens; g++) {
+        for (int r = 0; r < h; r++) {
+            const uint64_t *up = b + (size_t)((r + h - 1) % h) * ww, *mid = b + (size_t)r * ww,
+                           *dn = b + (size_t)((r + 1) % h) * ww;
+            for (int i = 0; i < ww; i++) {
+                int il = (i + ww - 1) % ww, ir = (i + 1) % ww;
+                uint64_t n[8];
+                const uint64_t *rows[3] = {up, mid, dn};
+                int k = 0;
+                for (int j = 0; j < 3; j++) {
+                    uint64_t x = rows[j][i];
+                    uint64_t west = x << 1 | rows[j][il] >> 63;  /* cell c-1 */
+                    uint64_t east = x >> 1 | rows[j][ir] << 63;  /* cell c+1 */
+                    n[k++] = west;
+                    n[k++] = east;
+                    if (j != 1)
+                        n[k++] = x;
+                }
+                uint64_t ones = 0, twos = 0, fours = 0;
+                for (int j = 0; j < 8; j++) {
+                    uint64_t c1 = ones & n[j];
+                    ones ^= n[j];
+                    uint64_t c2 = twos & c1;
+                    twos ^= c1;
+                    fours |= c2;  /* 4 or more: dead either way */
+                }
+                next[(size_t)r * ww + i] = ~fours & twos & (ones | mid[i]);
+            }
+        }
+        memcpy(b, next, (size_t)ww * (size_t)h * sizeof *next);
+    }
+    free(next);
+}

@@ -18,6 +18,8 @@
                            programs (oracle_exact_cycle) and by oracle/_ref's t1_cycles.
      oracle_life_scalar    life_scalar (src/gen.c:686-700): textbook B3/S23 on a torus; pinned by
                            known oscillators (blinker period 2, block still life, glider shift).
+     oracle_life_words     the same rule bit-parallel, 64 cells per word, for full-size boards;
+                           pinned against oracle_life_scalar on random boards of several sizes.
      oracle_benes_apply    benes_apply (src/benes.c:77-86); pinned by routing a permutation and
                            checking every value lands at dest[k].
      oracle_toyhash        toyhash (src/gen.c:107-121); pinned against oracle/_ref's toyhash and the
@@ -68,6 +70,8 @@ uint64_t oracle_exact_cycle(const ts_program *p, const ts_line *mem, const ts_li
                             unsigned lane);
 
 void     oracle_life_scalar(uint8_t *board, int w, int h);
+/* The same rule on 64 cells per word (cell c of a row is bit c % 64 of word c / 64). */
+void     oracle_life_words(uint64_t *board, int words_per_row, int h, int gens);
 void     oracle_benes_apply(const uint8_t *ctrl /* [17][512] */, uint64_t *v /* [512] */);
 uint64_t oracle_toyhash(uint64_t x, int bits, int rounds);
 

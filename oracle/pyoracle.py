@@ -42,6 +42,7 @@ def lib():
         L.oracle_exact_cycle.restype = u64
         L.oracle_life_scalar.argtypes = [vp, C.c_int, C.c_int]
         L.oracle_benes_apply.argtypes = [vp, vp]
+        L.oracle_life_words.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.oracle_toyhash.argtypes = [u64, C.c_int, C.c_int]
         L.oracle_toyhash.restype = u64
         _o = L
@@ -121,3 +122,12 @@ def benes_apply(ctrl, v):
 
 def toyhash(x, bits, rounds):
     return int(lib().oracle_toyhash(x, bits, rounds))
+
+
+def life_words(board_bits, gens):
+    """B3/S23 on a torus, bit-parallel: board_bits is (h, w) of 0/1 with w a multiple of 64."""
+    h, w = board_bits.shape
+    words = np.packbits(np.ascontiguousarray(board_bits, dtype=np.uint8), axis=1,
+                        bitorder="little").view(np.uint64).copy()
+    lib().oracle_life_words(words.ctypes.data, w // 64, h, gens)
+    return np.unpackbits(words.view(np.uint8), axis=1, bitorder="little")[:, :w]

@@ -119,6 +119,7 @@ def lib():
             "gpu_engine_close": (None, [vp]),
             "gpu_engine_describe": (C.c_char_p, [vp]),
             "gpu_last_ms": (C.c_double, [vp]),
+            "gpu_valu_rate": (i32, [vp, C.POINTER(C.c_double), C.c_char_p, sz]),
             "gpu_compile": (i32, [vp, P, C.POINTER(GpuTune), C.POINTER(vp), C.POINTER(C.c_char_p),
                                   C.c_char_p, sz]),
             "gpu_code_free": (None, [vp]),
@@ -477,6 +478,14 @@ class Engine:
 
     def describe(self):
         return lib().gpu_engine_describe(self._e).decode()
+
+    def valu_rate(self):
+        """Measured 32-bit v_bitop3_b32 lane-operations per second."""
+        v = C.c_double()
+        e = _err()
+        if not lib().gpu_valu_rate(self._e, C.byref(v), e, len(e)):
+            raise RuntimeError(e.value.decode())
+        return v.value
 
     @property
     def last_ms(self):

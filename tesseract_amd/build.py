@@ -16,7 +16,7 @@ OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libtesseract.so")
 
 HOST_C = ["program", "builder", "gen", "spec", "tester", "benes", "net"]
-ENGINE_CPP = ["lower", "codegen", "engine", "batch"]
+ENGINE_CPP = ["lower", "codegen", "engine", "calibrate", "batch"]
 ARCH = "gfx950"
 # Packed FP32 stays off in every HIP compile.
 NO_PACKED = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -38,7 +38,9 @@ def build(verbose=False, jobs=8):
                      "-I" + INCLUDE, "-c", src, "-o", obj])
         objs.append(obj)
     for name in ENGINE_CPP:
-        src = os.path.join(CSRC, "engine", name + (".hip" if name == "batch" else ".cpp"))
+        src = os.path.join(CSRC, "engine", name + ".cpp")
+        if not os.path.exists(src):
+            src = os.path.join(CSRC, "engine", name + ".hip")
         if not os.path.exists(src):
             continue
         obj = os.path.join(OBJ, name + ".o")

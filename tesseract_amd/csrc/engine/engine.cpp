@@ -200,6 +200,18 @@ extern "C" void gpu_engine_close(gpu_engine *e)
 }
 
 extern "C" const char *gpu_engine_describe(const gpu_engine *e) { return e->desc.c_str(); }
+
+extern "C" int ts_valu_calibrate(hipStream_t stream, int cus, double *lane_ops_per_s);
+
+extern "C" int gpu_valu_rate(gpu_engine *e, double *lane_ops_per_s, char *err, size_t errlen)
+{
+    HIPCHK(hipSetDevice(e->device));
+    if (!ts_valu_calibrate(e->stream, e->cus, lane_ops_per_s)) {
+        say(err, errlen, "the VALU calibration launch failed");
+        return 0;
+    }
+    return 1;
+}
 extern "C" double gpu_last_ms(const gpu_engine *e) { return e->last_ms; }
 
 // ---- compile --------------------------------------------------------------------------------

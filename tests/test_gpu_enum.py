@@ -103,3 +103,23 @@ def test_toyhash32_published_target(engine):
     for x in r["hit"]:
         assert O.toyhash(_rev(x, 32), 32, 8) == 0x165c2a07
     assert r["hits"] == len(r["hit"]) and r["hits"] >= 1
+
+
+def test_mul16_counts_every_assignment(engine):
+    # a*b + (a & b & 1) differs from the circuit exactly when a and b are both odd: 2^30 of the
+    # 2^32 assignments, so the count proves every assignment was evaluated and compared; a0 is
+    # bit 31 of x and b0 bit 15, so the smallest are 2^31 + 2^15 + 0..15
+    print("mul16 with a planted spec difference over 2^32", flush=True)
+    r = engine.enum_run(t.EN_VERIFY, t.gen_mul(16), spec="p = a * b + (a & b & 1)")
+    assert r["tried"] == 1 << 32 and r["hits"] == 1 << 30
+    assert r["hit"] == [(1 << 31) + (1 << 15) + k for k in range(16)]
+
+
+def test_toyhash32_search_random_target(engine):
+    rng = np.random.default_rng(11)
+    x = int(rng.integers(0, 1 << 32))
+    target = O.toyhash(x, 32, 8)
+    r = engine.enum_run(t.EN_SEARCH, t.gen_hash(32, 8), spec="h = %d" % target)
+    assert _rev(x, 32) in r["hit"] or r["hits"] > 16
+    for h in r["hit"]:
+        assert O.toyhash(_rev(h, 32), 32, 8) == target

@@ -105,6 +105,43 @@ int  gpu_enum_compile(gpu_engine *e, const gpu_enum *en, gpu_enum_code **out, ch
 int  gpu_enum_exec(gpu_enum_code *c, ts_enum_result *r, char *err, size_t errlen);
 void gpu_enum_free(gpu_enum_code *c);
 
+/* Batched cycle search: many link-free programs, each on `groups` x 512 machines started from the
+   reference's t1_start stream of its seed (group 0 exactly t1_start, src/tester.c:83-91; later
+   groups continue the stream), every machine running `max_passes` passes, its cycle length found
+   with the reference's Brent schedule (src/tester.c:96-164). The counterpart of tester 1's worker
+   pool (src/tester.c:182-279): programs are data for a persistent interpreter kernel fed by a device
+   work queue, not compiled one by one. stats[i] is program i's histogram (bucket b: cycle length
+   in (2^(b-1), 2^b], src/tester.c:42-48; bucket 63: none within max_passes). plain = 1 runs the same
+   kernel one launch per program, the baseline a batch is compared with. */
+typedef struct {
+    const ts_program *prog;
+    uint64_t          seed;
+} gpu_batch_item;
+
+typedef struct {
+    uint64_t hist[64];
+    uint64_t lam_sum;   /* sum of the cycle lengths found */
+    uint64_t longest;
+} gpu_batch_stats;
+
+typedef struct {
+    unsigned groups;        /* 512-machine groups per program (default 2048 = 2^20 machines) */
+    unsigned max_passes;    /* default 64 */
+    unsigned plain;
+    unsigned lds_cells;     /* programs with ncells + 1 <= this keep their machines in LDS (default 130) */
+    unsigned waves_per_cu;  /* persistent waves per CU for HBM-resident programs (default 8) */
+} gpu_batch_tune;
+
+typedef struct {
+    double   ms;            /* device time of the batch */
+    uint64_t ops;           /* 1-bit ops: instructions x passes x machines, summed */
+    uint64_t launches;
+    unsigned lds_cells;
+} gpu_batch_info;
+
+int  gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t n, const gpu_batch_tune *tune,
+                   gpu_batch_stats *stats, gpu_batch_info *info, char *err, size_t errlen);
+
 /* Listings without a device (the counterpart of `alpha jit`, reference src/main.c:468-496):
    the HIP source gpu_compile / gpu_enum_run would build, and a hipRTC compile of any such source
    for an architecture ("gfx950"), which needs no GPU. */

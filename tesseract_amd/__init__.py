@@ -55,6 +55,24 @@ class GpuCodeInfo(C.Structure):
                 ("dims", C.c_uint32)]
 
 
+class GpuBatchItem(C.Structure):
+    _fields_ = [("prog", C.POINTER(ProgramStruct)), ("seed", C.c_uint64)]
+
+
+class GpuBatchStats(C.Structure):
+    _fields_ = [("hist", C.c_uint64 * 64), ("lam_sum", C.c_uint64), ("longest", C.c_uint64)]
+
+
+class GpuBatchTune(C.Structure):
+    _fields_ = [("groups", C.c_uint), ("max_passes", C.c_uint), ("plain", C.c_uint),
+                ("lds_cells", C.c_uint), ("waves_per_cu", C.c_uint)]
+
+
+class GpuBatchInfo(C.Structure):
+    _fields_ = [("ms", C.c_double), ("ops", C.c_uint64), ("launches", C.c_uint64),
+                ("lds_cells", C.c_uint)]
+
+
 class GpuEnumStruct(C.Structure):
     _fields_ = [("mode", C.c_int), ("a", C.POINTER(ProgramStruct)), ("b", C.POINTER(ProgramStruct)),
                 ("spec", C.c_char_p), ("ticks", C.c_uint), ("part", C.c_int), ("part_end", C.c_int),
@@ -140,6 +158,8 @@ def lib():
             "gpu_enum_compile": (i32, [vp, C.POINTER(GpuEnumStruct), C.POINTER(vp), C.c_char_p, sz]),
             "gpu_enum_exec": (i32, [vp, C.POINTER(EnumResultStruct), C.c_char_p, sz]),
             "gpu_enum_free": (None, [vp]),
+            "gpu_batch_run": (i32, [vp, C.POINTER(GpuBatchItem), sz, C.POINTER(GpuBatchTune),
+                                    C.POINTER(GpuBatchStats), C.POINTER(GpuBatchInfo), C.c_char_p, sz]),
             "free": (None, [vp]),
         }
         for name, (res, args) in sig.items():
@@ -490,6 +510,23 @@ class Engine:
     @property
     def last_ms(self):
         return float(lib().gpu_last_ms(self._e))
+
+    def batch_run(self, progs, seeds, groups=2048, max_passes=64, plain=False, lds_cells=0,
+                  waves_per_cu=0):
+        """gpu_batch_run: per program a dict(hist, lam_sum, longest); plus the run info."""
+        n = len(progs)
+        items = (GpuBatchItem * n)()
+        for i, (p, sd) in enumerate(zip(progs, seeds)):
+            items[i].prog = p.ptr
+            items[i].seed = sd
+        tune = GpuBatchTune(groups, max_passes, int(plain), lds_cells, waves_per_cu)
+        stats = (GpuBatchStats * n)()
+        info = GpuBatchInfo()
+        e = _err()
+        if not lib().gpu_batch_run(self._e, items, n, C.byref(tune), stats, C.byref(info), e, len(e)):
+            raise RuntimeError("gpu_batch_run: " + e.value.decode())
+        out = [{"hist": list(st.hist), "lam_sum": st.lam_sum, "longest": st.longest} for st in stats]
+        return out, {"ms": info.ms, "ops": info.ops, "launches": info.launches}
 
     def compile(self, prog, tune=None, listing=False):
         return Code(self, prog, tune, listing)

@@ -1,0 +1,442 @@
+// Batched execution: thousands of programs of uneven length and cell count in one persistent launch
+// per residency class, fed by a device work queue (the reference runs random programs one per
+// thread, src/tester.c:182-226, and finds cycle lengths with Brent's algorithm on 512 lanes,
+// src/tester.c:96-164).
+//
+// Programs are data, not compiled code: a hipRTC compile per random program would cost more than
+// running it. Each program is re-encoded on the host into *windows* of 4 instructions in which no
+// NAND reads a cell stored earlier in the same window; a window's NAND operands can then all be
+// loaded before any of its instructions executes, so 4 loads are in flight per chain instead of
+// one. A window is one 16-byte record: per slot the cell index, bit 31 set for NAND; short windows
+// are padded with stores to a sink cell. The 16 NAND/STO shapes of a window are 16 straight-line
+// cases of one switch.
+//
+// A work item is 64 words (2,048 machines) of one program: one wave. The machines' cells live in
+// LDS (64 lanes x cells x 4 bytes, hare and tortoise) for small programs, or in a per-wave region
+// of HBM for large ones (coalesced: 256 contiguous bytes per cell). Start states come from the
+// reference's t1_start stream, generated in registers. Every machine runs exactly max_passes
+// passes; Brent's schedule gives each machine's cycle length, and per-program histograms are
+// reduced per wave before one atomic per bucket.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "engine_internal.hpp"
+#include "tesseract_gpu.h"
+
+namespace {
+
+constexpr unsigned kWin = 4;          // instructions per window record
+constexpr unsigned kLanes = 64;       // words per work item (one wave)
+constexpr unsigned kStats = 66;       // hist[64], lam_sum, longest
+constexpr unsigned kSample = 64;      // cells compared before a full comparison (reference: 64)
+
+struct BatchProg {
+    uint32_t rec_off;    // first window record
+    uint32_t nrec;       // window records per pass
+    uint32_t ncells;     // cells incl. ZERO and ONE (the sink is cell ncells)
+    uint32_t nsample;    // sampled cells (<= 64)
+    uint32_t sample_off; // into the sample list
+    uint32_t first_item; // global index of the program's first work item
+    uint64_t seed;       // t1_start stream seed
+};
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Storage of one wave's machines: element (cell, lane) at base[cell * 64 + lane].
+struct Cells {
+    uint32_t *base;
+    __device__ __forceinline__ uint32_t ld(uint32_t cell, uint32_t lane) const
+    {
+        return base[cell * kLanes + lane];
+    }
+    __device__ __forceinline__ void st(uint32_t cell, uint32_t lane, uint32_t v) const
+    {
+        base[cell * kLanes + lane] = v;
+    }
+};
+
+template <unsigned SHAPE>
+__device__ __forceinline__ void window(const Cells &h, uint32_t lane, uint32_t c0, uint32_t c1,
+                                       uint32_t c2, uint32_t c3, uint32_t &rr)
+{
+    uint32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+    if (SHAPE & 1) v0 = h.ld(c0, lane);
+    if (SHAPE & 2) v1 = h.ld(c1, lane);
+    if (SHAPE & 4) v2 = h.ld(c2, lane);
+    if (SHAPE & 8) v3 = h.ld(c3, lane);
+    if (SHAPE & 1) rr = ~(rr & v0); else h.st(c0, lane, rr);
+    if (SHAPE & 2) rr = ~(rr & v1); else h.st(c1, lane, rr);
+    if (SHAPE & 4) rr = ~(rr & v2); else h.st(c2, lane, rr);
+    if (SHAPE & 8) rr = ~(rr & v3); else h.st(c3, lane, rr);
+}
+
+__device__ __forceinline__ void run_pass(const Cells &h, uint32_t lane, const uint4 *rec,
+                                         uint32_t nrec, uint32_t &rr)
+{
+    for (uint32_t r = 0; r < nrec; r++) {
+        const uint4 w = rec[r];
+        const uint32_t shape = (w.x >> 31) | (w.y >> 31) << 1 | (w.z >> 31) << 2 | (w.w >> 31) << 3;
+        const uint32_t c0 = w.x & 0x7FFFFFFFu, c1 = w.y & 0x7FFFFFFFu, c2 = w.z & 0x7FFFFFFFu,
+                       c3 = w.w & 0x7FFFFFFFu;
+        switch (shape) {
+#define TS_CASE(S) case S: window<S>(h, lane, c0, c1, c2, c3, rr); break;
+            TS_CASE(0) TS_CASE(1) TS_CASE(2) TS_CASE(3) TS_CASE(4) TS_CASE(5) TS_CASE(6)
+            TS_CASE(7) TS_CASE(8) TS_CASE(9) TS_CASE(10) TS_CASE(11) TS_CASE(12) TS_CASE(13)
+            TS_CASE(14) TS_CASE(15)
+#undef TS_CASE
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+    for (int o = 32; o >= 1; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ int bucket_of(uint64_t lam)
+{
+    int b = 0;
+    while (b < 62 && (1ull << b) < lam)
+        b++;
+    return b;
+}
+
+// lanes in `found` reached their cycle, of length lam
+__device__ __forceinline__ void record(uint64_t *st, uint32_t found, uint64_t lam, uint32_t lane)
+{
+    uint32_t n = wave_sum((uint32_t)__builtin_popcount(found));
+    if (n && lane == 0) {
+        atomicAdd((unsigned long long *)&st[bucket_of(lam)], (unsigned long long)n);
+        atomicAdd((unsigned long long *)&st[64], (unsigned long long)(n * lam));
+        atomicMax((unsigned long long *)&st[65], (unsigned long long)lam);
+    }
+}
+
+template <bool LDS>
+__global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ progs,
+                                               const uint32_t *__restrict__ item_prog,
+                                               uint32_t item_base, uint32_t nitems,
+                                               const uint4 *__restrict__ recs,
+                                               const uint32_t *__restrict__ samples,
+                                               uint32_t *__restrict__ queue,
+                                               uint32_t *__restrict__ scratch, uint32_t region_cells,
+                                               uint64_t *__restrict__ stats, uint32_t max_passes)
+{
+    extern __shared__ uint32_t lds[];
+    const uint32_t lane = threadIdx.x;
+    uint32_t *region = LDS ? lds : scratch + (size_t)blockIdx.x * 2 * region_cells * kLanes;
+    for (;;) {
+        uint32_t item = 0;
+        if (lane == 0)
+            item = atomicAdd(queue, 1u);
+        item = __builtin_amdgcn_readfirstlane(__shfl(item, 0, 64));
+        if (item >= nitems)
+            break;
+        item += item_base;
+        const uint32_t pi = __builtin_amdgcn_readfirstlane(item_prog[item]);
+        const BatchProg P = progs[pi];
+        const uint32_t ncells = P.ncells;
+        Cells hare{region}, tort{region + (ncells + 1) * kLanes};
+        const uint4 *rec = recs + P.rec_off;
+        uint64_t *st = stats + (size_t)pi * kStats;
+
+        // start state: group g = word / 16, u32 word j of its line; the stream continues group
+        // after group, cells 2.. then RR, 8 u64 words each (reference src/tester.c:83-91)
+        const uint32_t word = (item - P.first_item) * kLanes + lane;
+        const uint64_t g = word >> 4;
+        const uint32_t w64 = (word & 15) >> 1, hi = word & 1;
+        const uint64_t s0 = P.seed ^ 0x5DEECE66Dull, per_group = (uint64_t)(ncells - 1) * 8;
+        for (uint32_t c = 2; c < ncells; c++) {
+            uint64_t k = g * per_group + (uint64_t)(c - 2) * 8 + w64;
+            uint64_t v = mix64(s0 + (k + 1) * 0x9E3779B97F4A7C15ull);
+            hare.st(c, lane, hi ? (uint32_t)(v >> 32) : (uint32_t)v);
+        }
+        hare.st(0, lane, 0u);
+        hare.st(1, lane, 0xFFFFFFFFu);
+        uint32_t rr;
+        {
+            uint64_t k = g * per_group + (uint64_t)(ncells - 2) * 8 + w64;
+            uint64_t v = mix64(s0 + (k + 1) * 0x9E3779B97F4A7C15ull);
+            rr = hi ? (uint32_t)(v >> 32) : (uint32_t)v;
+        }
+
+        // Brent: the tortoise jumps to the hare at powers of two (reference src/tester.c:96-164)
+        for (uint32_t c = 0; c < ncells; c++)
+            tort.st(c, lane, hare.ld(c, lane));
+        uint32_t trr = rr;
+        run_pass(hare, lane, rec, P.nrec, rr);
+        uint64_t lam = 1, power = 1, steps = 1;
+        uint32_t active = 0xFFFFFFFFu;
+        const uint32_t *smp = samples + P.sample_off;
+        for (;;) {
+            uint32_t d = rr ^ trr;
+            for (uint32_t i = 0; i < P.nsample; i++) {
+                uint32_t c = smp[i];
+                d |= hare.ld(c, lane) ^ tort.ld(c, lane);
+            }
+            if (P.nsample < ncells - 2 && __any(active & ~d)) {
+                d = rr ^ trr;
+                for (uint32_t c = 2; c < ncells; c++)
+                    d |= hare.ld(c, lane) ^ tort.ld(c, lane);
+            }
+            const uint32_t found = active & ~d;
+            if (__any(found))
+                record(st, found, lam, lane);
+            active &= d;
+            if (steps >= max_passes)
+                break;
+            if (power == lam) {
+                for (uint32_t c = 2; c < ncells; c++)
+                    tort.st(c, lane, hare.ld(c, lane));
+                trr = rr;
+                power *= 2;
+                lam = 0;
+            }
+            run_pass(hare, lane, rec, P.nrec, rr);
+            lam++;
+            steps++;
+        }
+        uint32_t left = wave_sum((uint32_t)__builtin_popcount(active));
+        if (left && lane == 0)
+            atomicAdd((unsigned long long *)&st[63], (unsigned long long)left);
+        __syncthreads();  // one wave per block: orders this item's LDS use before the next
+    }
+}
+
+struct Encoded {
+    std::vector<uint32_t> words;  // window records, 4 words each
+    uint32_t nrec = 0;
+};
+
+// Windows of up to 4 instructions in which no NAND reads a cell stored earlier in the window.
+// STO to read-only cells (ignored by the machine) and padding both store to the sink cell.
+Encoded encode(const ts_program *p)
+{
+    Encoded e;
+    const uint32_t sink = p->ncells;
+    std::vector<uint32_t> win;
+    std::vector<uint32_t> stored;
+    auto flush = [&]() {
+        while (win.size() < kWin)
+            win.push_back(sink);  // STO sink
+        e.words.insert(e.words.end(), win.begin(), win.end());
+        e.nrec++;
+        win.clear();
+        stored.clear();
+    };
+    for (size_t i = 0; i < p->len; i++) {
+        const ts_instr in = p->code[i];
+        if (in.op == TS_OP_NAND) {
+            bool hazard = std::find(stored.begin(), stored.end(), in.addr) != stored.end();
+            if (hazard)
+                flush();
+            win.push_back(in.addr | 0x80000000u);
+        } else {
+            bool ro = in.addr < TS_CELL_FIRST_FREE ||
+                      (p->special && p->special[in.addr].kind != TS_CK_PLAIN);
+            uint32_t c = ro ? sink : in.addr;
+            win.push_back(c);
+            stored.push_back(c);
+        }
+        if (win.size() == kWin)
+            flush();
+    }
+    if (!win.empty())
+        flush();
+    return e;
+}
+
+void say(char *err, size_t errlen, const char *fmt, const char *what)
+{
+    if (err && errlen)
+        snprintf(err, errlen, fmt, what);
+}
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    bool alloc(size_t n) { return hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) == hipSuccess; }
+};
+
+}  // namespace
+
+#define BCHK(call)                                                                           \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            say(err, errlen, "%s", hipGetErrorString(e_));                                   \
+            return 0;                                                                        \
+        }                                                                                    \
+    } while (0)
+
+extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t n,
+                             const gpu_batch_tune *tune_in, gpu_batch_stats *stats_out,
+                             gpu_batch_info *info, char *err, size_t errlen)
+{
+    gpu_batch_tune tune{};
+    if (tune_in)
+        tune = *tune_in;
+    if (!tune.groups)
+        tune.groups = 2048;
+    if (!tune.max_passes)
+        tune.max_passes = 64;
+    if (!tune.lds_cells)
+        tune.lds_cells = 130;
+    if (!tune.waves_per_cu)
+        tune.waves_per_cu = 8;
+    if (info)
+        memset(info, 0, sizeof *info);
+    if (!n) {
+        return 1;
+    }
+    for (size_t i = 0; i < n; i++) {
+        const ts_program *p = items[i].prog;
+        if (p->ncells < 3) {
+            say(err, errlen, "%s", "a batched program needs at least one writable cell");
+            return 0;
+        }
+        for (uint32_t c = 0; p->special && c < p->ncells; c++)
+            if (p->special[c].kind == TS_CK_LINK || p->special[c].kind == TS_CK_ID) {
+                say(err, errlen, "%s", "batched programs have no id or link cells");
+                return 0;
+            }
+    }
+    BCHK(hipSetDevice(e->device));
+    const uint32_t items_per_prog = tune.groups * 16 / kLanes;
+
+    // host encoding, by residency class: 0 = LDS (<= 32 cells with the sink), 1 = LDS (<= lds_cells),
+    // 2 = HBM; each class is one persistent launch sized by what a CU can hold
+    const uint32_t limit[3] = {std::min<uint32_t>(32, tune.lds_cells), tune.lds_cells, 0xFFFFFFFFu};
+    auto class_of = [&](const ts_program *p) {
+        uint32_t need = p->ncells + 1;
+        return need <= limit[0] ? 0 : need <= limit[1] ? 1 : 2;
+    };
+    std::vector<BatchProg> progs(n);
+    std::vector<uint32_t> recs, samples;
+    std::vector<uint32_t> item_of[3];
+    uint32_t maxcells[3] = {0, 0, 0};
+    uint64_t ops = 0;
+    for (size_t i = 0; i < n; i++) {
+        const ts_program *p = items[i].prog;
+        Encoded enc = encode(p);
+        BatchProg &bp = progs[i];
+        bp.rec_off = (uint32_t)(recs.size() / kWin);
+        bp.nrec = enc.nrec;
+        recs.insert(recs.end(), enc.words.begin(), enc.words.end());
+        bp.ncells = p->ncells;
+        uint32_t writable = p->ncells - TS_CELL_FIRST_FREE;
+        bp.nsample = writable < kSample ? writable : kSample;
+        bp.sample_off = (uint32_t)samples.size();
+        for (uint32_t k = 0; k < bp.nsample; k++)
+            samples.push_back(TS_CELL_FIRST_FREE + (uint32_t)((uint64_t)k * writable / bp.nsample));
+        bp.seed = items[i].seed;
+        int cls = class_of(p);
+        bp.first_item = (uint32_t)item_of[cls].size();
+        for (uint32_t k = 0; k < items_per_prog; k++)
+            item_of[cls].push_back((uint32_t)i);
+        maxcells[cls] = std::max(maxcells[cls], p->ncells + 1);
+        ops += (uint64_t)p->len * tune.max_passes * tune.groups * 512ull;
+    }
+    DevBuf<BatchProg> d_progs;
+    DevBuf<uint4> d_recs;
+    DevBuf<uint32_t> d_samples, d_items[3], d_queue, d_scratch;
+    DevBuf<uint64_t> d_stats;
+    bool ok = d_progs.alloc(n) && d_recs.alloc(recs.size() / 4) && d_samples.alloc(samples.size()) &&
+              d_queue.alloc(3 + n) && d_stats.alloc(n * kStats);
+    for (int c = 0; c < 3; c++)
+        ok = ok && d_items[c].alloc(item_of[c].size());
+    if (!ok) {
+        say(err, errlen, "%s", "out of device memory for the batch");
+        return 0;
+    }
+    const unsigned waves = (unsigned)e->cus * tune.waves_per_cu;
+    if (maxcells[2] && !d_scratch.alloc((size_t)waves * 2 * maxcells[2] * kLanes)) {
+        say(err, errlen, "%s", "out of device memory for the HBM-resident machines");
+        return 0;
+    }
+    hipStream_t s = e->stream;
+    BCHK(hipMemcpyAsync(d_progs.p, progs.data(), n * sizeof(BatchProg), hipMemcpyHostToDevice, s));
+    BCHK(hipMemcpyAsync(d_recs.p, recs.data(), recs.size() * 4, hipMemcpyHostToDevice, s));
+    BCHK(hipMemcpyAsync(d_samples.p, samples.data(), samples.size() * 4, hipMemcpyHostToDevice, s));
+    for (int c = 0; c < 3; c++)
+        if (!item_of[c].empty())
+            BCHK(hipMemcpyAsync(d_items[c].p, item_of[c].data(), item_of[c].size() * 4,
+                                hipMemcpyHostToDevice, s));
+    BCHK(hipMemsetAsync(d_queue.p, 0, (3 + n) * 4, s));
+    BCHK(hipMemsetAsync(d_stats.p, 0, n * kStats * 8, s));
+
+    uint64_t launches = 0;
+    auto launch_class = [&](int cls, uint32_t *queue, uint32_t first, uint32_t count) -> int {
+        const uint32_t *ip = d_items[cls].p;
+        if (cls < 2) {
+            // LDS limits how many waves a CU holds
+            const size_t lds_bytes = (size_t)2 * maxcells[cls] * kLanes * 4;
+            unsigned per_cu = (unsigned)std::max<size_t>(
+                1, std::min<size_t>(tune.waves_per_cu, (160u << 10) / lds_bytes));
+            unsigned grid = std::min<unsigned>((unsigned)e->cus * per_cu, count);
+            hipLaunchKernelGGL(ts_batch<true>, dim3(grid), dim3(kLanes), lds_bytes, s, d_progs.p,
+                               ip, first, count, d_recs.p, d_samples.p, queue,
+                               (uint32_t *)nullptr, 0u, d_stats.p, tune.max_passes);
+        } else {
+            unsigned grid = std::min<unsigned>(waves, count);
+            hipLaunchKernelGGL(ts_batch<false>, dim3(grid), dim3(kLanes), 0, s, d_progs.p, ip,
+                               first, count, d_recs.p, d_samples.p, queue, d_scratch.p,
+                               maxcells[2], d_stats.p, tune.max_passes);
+        }
+        launches++;
+        return hipGetLastError() == hipSuccess;
+    };
+    BCHK(hipEventRecord(e->ev0, s));
+    if (!tune.plain) {
+        for (int cls = 0; cls < 3; cls++) {
+            if (item_of[cls].empty())
+                continue;
+            if (!launch_class(cls, d_queue.p + cls, 0, (uint32_t)item_of[cls].size())) {
+                say(err, errlen, "%s", "batch launch failed");
+                return 0;
+            }
+        }
+    } else {
+        // one launch per program, in program order (the baseline the batch is compared with)
+        for (size_t i = 0; i < n; i++) {
+            int cls = class_of(items[i].prog);
+            if (!launch_class(cls, d_queue.p + 3 + i, progs[i].first_item, items_per_prog)) {
+                say(err, errlen, "%s", "batch launch failed");
+                return 0;
+            }
+        }
+    }
+    BCHK(hipEventRecord(e->ev1, s));
+    BCHK(hipEventSynchronize(e->ev1));
+    float ms = 0;
+    BCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    e->last_ms = ms;
+    std::vector<uint64_t> st(n * kStats);
+    BCHK(hipMemcpy(st.data(), d_stats.p, st.size() * 8, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) {
+        memcpy(stats_out[i].hist, &st[i * kStats], 64 * 8);
+        stats_out[i].lam_sum = st[i * kStats + 64];
+        stats_out[i].longest = st[i * kStats + 65];
+    }
+    if (info) {
+        info->ms = ms;
+        info->ops = ops;
+        info->launches = launches;
+        info->lds_cells = tune.lds_cells;
+    }
+    return 1;
+}

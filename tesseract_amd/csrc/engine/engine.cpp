@@ -19,14 +19,7 @@
 
 using namespace ts;
 
-struct gpu_engine {
-    int         device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t  ev0 = nullptr, ev1 = nullptr;
-    int         cus = 0;
-    std::string arch, desc;
-    double      last_ms = 0;
-};
+#include "engine_internal.hpp"
 
 struct gpu_code {
     gpu_engine            *e = nullptr;

@@ -137,6 +137,8 @@ typedef struct {
     uint64_t ops;           /* 1-bit ops: instructions x passes x machines, summed */
     uint64_t launches;
     unsigned lds_cells;
+    double   class_ms[3];   /* batched: device time per residency class (LDS small, LDS, HBM) */
+    uint64_t class_ops[3];
 } gpu_batch_info;
 
 int  gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t n, const gpu_batch_tune *tune,

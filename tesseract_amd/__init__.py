@@ -70,7 +70,7 @@ class GpuBatchTune(C.Structure):
 
 class GpuBatchInfo(C.Structure):
     _fields_ = [("ms", C.c_double), ("ops", C.c_uint64), ("launches", C.c_uint64),
-                ("lds_cells", C.c_uint)]
+                ("lds_cells", C.c_uint), ("class_ms", C.c_double * 3), ("class_ops", C.c_uint64 * 3)]
 
 
 class GpuEnumStruct(C.Structure):
@@ -526,7 +526,8 @@ class Engine:
         if not lib().gpu_batch_run(self._e, items, n, C.byref(tune), stats, C.byref(info), e, len(e)):
             raise RuntimeError("gpu_batch_run: " + e.value.decode())
         out = [{"hist": list(st.hist), "lam_sum": st.lam_sum, "longest": st.longest} for st in stats]
-        return out, {"ms": info.ms, "ops": info.ops, "launches": info.launches}
+        return out, {"ms": info.ms, "ops": info.ops, "launches": info.launches,
+                     "class_ms": list(info.class_ms), "class_ops": list(info.class_ops)}
 
     def compile(self, prog, tune=None, listing=False):
         return Code(self, prog, tune, listing)

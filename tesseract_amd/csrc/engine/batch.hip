@@ -4,12 +4,12 @@
 // src/tester.c:96-164).
 //
 // Programs are data, not compiled code: a hipRTC compile per random program would cost more than
-// running it. Each program is re-encoded on the host into *windows* of 4 instructions in which no
-// NAND reads a cell stored earlier in the same window; a window's NAND operands can then all be
-// loaded before any of its instructions executes, so 4 loads are in flight per chain instead of
-// one. A window is one 16-byte record: per slot the cell index, bit 31 set for NAND; short windows
-// are padded with stores to a sink cell. The 16 NAND/STO shapes of a window are 16 straight-line
-// cases of one switch.
+// running it. Each instruction touches one 4-byte word per 32 machines, so the interpreter is
+// bound by how many of those accesses it keeps in flight. Each program is re-encoded on the host
+// into *windows* of 8 instructions in which no NAND reads a cell stored earlier in the same
+// window; a window's NAND operands are then loaded together before any of its instructions
+// executes, 8 accesses in flight per wave instead of one. A window is two 16-byte records: per
+// slot the cell index, bit 31 set for NAND; short windows are padded with stores to a sink cell.
 //
 // A work item is 64 words (2,048 machines) of one program: one wave. The machines' cells live in
 // LDS (64 lanes x cells x 4 bytes, hare and tortoise) for small programs, or in a per-wave region
@@ -31,7 +31,7 @@
 
 namespace {
 
-constexpr unsigned kWin = 4;          // instructions per window record
+constexpr unsigned kWin = 8;          // instructions per window (two 16-byte records)
 constexpr unsigned kLanes = 64;       // words per work item (one wave)
 constexpr unsigned kStats = 66;       // hist[64], lam_sum, longest
 constexpr unsigned kSample = 64;      // cells compared before a full comparison (reference: 64)
@@ -66,36 +66,41 @@ struct Cells {
     }
 };
 
-template <unsigned SHAPE>
-__device__ __forceinline__ void window(const Cells &h, uint32_t lane, uint32_t c0, uint32_t c1,
-                                       uint32_t c2, uint32_t c3, uint32_t &rr)
+// One window of 8 slots (two 16-byte records; bit 31 of a slot = NAND): the NAND operands are
+// loaded first, all in flight together, then the slots execute in order.
+__device__ __forceinline__ void window(const Cells &h, uint32_t lane, const uint4 a, const uint4 b,
+                                       uint32_t &rr)
 {
-    uint32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0;
-    if (SHAPE & 1) v0 = h.ld(c0, lane);
-    if (SHAPE & 2) v1 = h.ld(c1, lane);
-    if (SHAPE & 4) v2 = h.ld(c2, lane);
-    if (SHAPE & 8) v3 = h.ld(c3, lane);
-    if (SHAPE & 1) rr = ~(rr & v0); else h.st(c0, lane, rr);
-    if (SHAPE & 2) rr = ~(rr & v1); else h.st(c1, lane, rr);
-    if (SHAPE & 4) rr = ~(rr & v2); else h.st(c2, lane, rr);
-    if (SHAPE & 8) rr = ~(rr & v3); else h.st(c3, lane, rr);
+    const uint32_t c[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    uint32_t v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        if (c[k] >> 31)
+            v[k] = h.ld(c[k] & 0x7FFFFFFFu, lane);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        if (c[k] >> 31)
+            rr = ~(rr & v[k]);
+        else
+            h.st(c[k], lane, rr);
+    }
 }
 
 __device__ __forceinline__ void run_pass(const Cells &h, uint32_t lane, const uint4 *rec,
                                          uint32_t nrec, uint32_t &rr)
 {
+    // nrec counts 8-slot windows (2 records each); the next window's records are fetched while
+    // this one runs
+    if (!nrec)
+        return;
+    uint4 a = rec[0], b = rec[1];
     for (uint32_t r = 0; r < nrec; r++) {
-        const uint4 w = rec[r];
-        const uint32_t shape = (w.x >> 31) | (w.y >> 31) << 1 | (w.z >> 31) << 2 | (w.w >> 31) << 3;
-        const uint32_t c0 = w.x & 0x7FFFFFFFu, c1 = w.y & 0x7FFFFFFFu, c2 = w.z & 0x7FFFFFFFu,
-                       c3 = w.w & 0x7FFFFFFFu;
-        switch (shape) {
-#define TS_CASE(S) case S: window<S>(h, lane, c0, c1, c2, c3, rr); break;
-            TS_CASE(0) TS_CASE(1) TS_CASE(2) TS_CASE(3) TS_CASE(4) TS_CASE(5) TS_CASE(6)
-            TS_CASE(7) TS_CASE(8) TS_CASE(9) TS_CASE(10) TS_CASE(11) TS_CASE(12) TS_CASE(13)
-            TS_CASE(14) TS_CASE(15)
-#undef TS_CASE
+        const uint4 ca = a, cb = b;
+        if (r + 1 < nrec) {
+            a = rec[2 * r + 2];
+            b = rec[2 * r + 3];
         }
+        window(h, lane, ca, cb, rr);
     }
 }
 
@@ -217,11 +222,11 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
 }
 
 struct Encoded {
-    std::vector<uint32_t> words;  // window records, 4 words each
+    std::vector<uint32_t> words;  // windows, 8 words each
     uint32_t nrec = 0;
 };
 
-// Windows of up to 4 instructions in which no NAND reads a cell stored earlier in the window.
+// Windows of up to 8 instructions in which no NAND reads a cell stored earlier in the window.
 // STO to read-only cells (ignored by the machine) and padding both store to the sink cell.
 Encoded encode(const ts_program *p)
 {
@@ -334,7 +339,7 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
         const ts_program *p = items[i].prog;
         Encoded enc = encode(p);
         BatchProg &bp = progs[i];
-        bp.rec_off = (uint32_t)(recs.size() / kWin);
+        bp.rec_off = (uint32_t)(recs.size() / 4);
         bp.nrec = enc.nrec;
         recs.insert(recs.end(), enc.words.begin(), enc.words.end());
         bp.ncells = p->ncells;
@@ -400,9 +405,17 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
         launches++;
         return hipGetLastError() == hipSuccess;
     };
+    hipEvent_t mark[4];
+    for (int k = 0; k < 4; k++)
+        BCHK(hipEventCreate(&mark[k]));
+    struct Marks {
+        hipEvent_t *m;
+        ~Marks() { for (int k = 0; k < 4; k++) (void)hipEventDestroy(m[k]); }
+    } marks_guard{mark};
     BCHK(hipEventRecord(e->ev0, s));
     if (!tune.plain) {
         for (int cls = 0; cls < 3; cls++) {
+            BCHK(hipEventRecord(mark[cls], s));
             if (item_of[cls].empty())
                 continue;
             if (!launch_class(cls, d_queue.p + cls, 0, (uint32_t)item_of[cls].size())) {
@@ -410,6 +423,7 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
                 return 0;
             }
         }
+        BCHK(hipEventRecord(mark[3], s));
     } else {
         // one launch per program, in program order (the baseline the batch is compared with)
         for (size_t i = 0; i < n; i++) {
@@ -432,7 +446,16 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
         stats_out[i].lam_sum = st[i * kStats + 64];
         stats_out[i].longest = st[i * kStats + 65];
     }
+    if (info && !tune.plain)
+        for (int k = 0; k < 3; k++) {
+            float cms = 0;
+            BCHK(hipEventElapsedTime(&cms, mark[k], mark[k + 1]));
+            info->class_ms[k] = cms;
+        }
     if (info) {
+        for (size_t i = 0; i < n; i++)
+            info->class_ops[class_of(items[i].prog)] +=
+                (uint64_t)items[i].prog->len * tune.max_passes * tune.groups * 512ull;
         info->ms = ms;
         info->ops = ops;
         info->launches = launches;

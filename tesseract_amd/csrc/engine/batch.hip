@@ -4,13 +4,14 @@
 // src/tester.c:96-164).
 //
 // Programs are data, not compiled code: a hipRTC compile per random program would cost more than
-// running it. Each instruction touches one 4-byte word per 32 machines, so the interpreter is
-// bound by how many of those accesses it keeps in flight. Each program is re-encoded on the host
-// into *windows* of 8 instructions in which no NAND reads a cell stored earlier in the same
-// window; a window's NAND operands are then loaded together before any of its instructions
-// executes, 8 accesses in flight per wave instead of one. A window is two 16-byte records: per
-// slot the cell index, bit 31 set for NAND; short windows are padded with stores to a sink cell.
-//
+// running it. Each instruction touches one 4-byte word per 32 machines, so the interpreter must
+// keep many accesses in flight and spend almost nothing on control. Each program is re-encoded on
+// the host into *windows* of 8 instructions in which no NAND reads a cell stored earlier in the
+// same window, so a window's 8 loads issue together before any of its instructions executes. The
+// window runs without a branch: every slot is one load and one store, a NAND slot storing to a
+// per-wave sink cell and a STO slot loading from it, and rr = M ? !(rr & v) : rr is a single
+// v_bitop3_b32 with the slot's wave-uniform mask M. A window is 16 words (one scalar load): per
+// slot the byte offset of the cell loaded (bit 31 = NAND) and of the cell stored.
 // A work item is 64 words (2,048 machines) of one program: one wave. The machines' cells live in
 // LDS (64 lanes x cells x 4 bytes, hare and tortoise) for small programs, or in a per-wave region
 // of HBM for large ones (coalesced: 256 contiguous bytes per cell). Start states come from the
@@ -38,7 +39,7 @@ constexpr unsigned kSample = 64;      // cells compared before a full comparison
 
 struct BatchProg {
     uint32_t rec_off;    // first window record
-    uint32_t nrec;       // window records per pass
+    uint32_t nrec;       // windows per pass (16 words each)
     uint32_t ncells;     // cells incl. ZERO and ONE (the sink is cell ncells)
     uint32_t nsample;    // sampled cells (<= 64)
     uint32_t sample_off; // into the sample list
@@ -53,54 +54,45 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z)
     return z ^ (z >> 31);
 }
 
-// Storage of one wave's machines: element (cell, lane) at base[cell * 64 + lane].
+// Storage of one wave's machines: element (cell, lane) at byte cell * 256 + lane * 4 of the base.
 struct Cells {
-    uint32_t *base;
+    uint8_t *base;
     __device__ __forceinline__ uint32_t ld(uint32_t cell, uint32_t lane) const
     {
-        return base[cell * kLanes + lane];
+        return *(const uint32_t *)(base + cell * (kLanes * 4) + lane * 4);
     }
     __device__ __forceinline__ void st(uint32_t cell, uint32_t lane, uint32_t v) const
     {
-        base[cell * kLanes + lane] = v;
+        *(uint32_t *)(base + cell * (kLanes * 4) + lane * 4) = v;
+    }
+    __device__ __forceinline__ uint32_t ld_at(uint32_t byte_off, uint32_t lane4) const
+    {
+        return *(const uint32_t *)(base + byte_off + lane4);
+    }
+    __device__ __forceinline__ void st_at(uint32_t byte_off, uint32_t lane4, uint32_t v) const
+    {
+        *(uint32_t *)(base + byte_off + lane4) = v;
     }
 };
 
-// One window of 8 slots (two 16-byte records; bit 31 of a slot = NAND): the NAND operands are
-// loaded first, all in flight together, then the slots execute in order.
-__device__ __forceinline__ void window(const Cells &h, uint32_t lane, const uint4 a, const uint4 b,
-                                       uint32_t &rr)
-{
-    const uint32_t c[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-    uint32_t v[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++)
-        if (c[k] >> 31)
-            v[k] = h.ld(c[k] & 0x7FFFFFFFu, lane);
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        if (c[k] >> 31)
-            rr = ~(rr & v[k]);
-        else
-            h.st(c[k], lane, rr);
-    }
-}
-
 __device__ __forceinline__ void run_pass(const Cells &h, uint32_t lane, const uint4 *rec,
-                                         uint32_t nrec, uint32_t &rr)
+                                         uint32_t nwin, uint32_t &rr)
 {
-    // nrec counts 8-slot windows (2 records each); the next window's records are fetched while
-    // this one runs
-    if (!nrec)
-        return;
-    uint4 a = rec[0], b = rec[1];
-    for (uint32_t r = 0; r < nrec; r++) {
-        const uint4 ca = a, cb = b;
-        if (r + 1 < nrec) {
-            a = rec[2 * r + 2];
-            b = rec[2 * r + 3];
+    const uint32_t lane4 = lane * 4;
+    for (uint32_t r = 0; r < nwin; r++) {
+        const uint4 q0 = rec[4 * r], q1 = rec[4 * r + 1], q2 = rec[4 * r + 2], q3 = rec[4 * r + 3];
+        const uint32_t w[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                                q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+        uint32_t v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            v[k] = h.ld_at(w[2 * k] & 0x7FFFFFFFu, lane4);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const uint32_t m = (uint32_t)((int32_t)w[2 * k] >> 31);  // all ones for a NAND
+            rr = __builtin_amdgcn_bitop3_b32(m, rr, v[k], 0x7Cu);   // m ? !(rr & v) : rr
+            h.st_at(w[2 * k + 1], lane4, rr);
         }
-        window(h, lane, ca, cb, rr);
     }
 }
 
@@ -142,7 +134,7 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
 {
     extern __shared__ uint32_t lds[];
     const uint32_t lane = threadIdx.x;
-    uint32_t *region = LDS ? lds : scratch + (size_t)blockIdx.x * 2 * region_cells * kLanes;
+    uint8_t *region = (uint8_t *)(LDS ? lds : scratch + (size_t)blockIdx.x * 2 * region_cells * kLanes);
     for (;;) {
         uint32_t item = 0;
         if (lane == 0)
@@ -154,7 +146,7 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
         const uint32_t pi = __builtin_amdgcn_readfirstlane(item_prog[item]);
         const BatchProg P = progs[pi];
         const uint32_t ncells = P.ncells;
-        Cells hare{region}, tort{region + (ncells + 1) * kLanes};
+        Cells hare{region}, tort{region + (size_t)(ncells + 1) * kLanes * 4};
         const uint4 *rec = recs + P.rec_off;
         uint64_t *st = stats + (size_t)pi * kStats;
 
@@ -222,41 +214,45 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
 }
 
 struct Encoded {
-    std::vector<uint32_t> words;  // windows, 8 words each
-    uint32_t nrec = 0;
+    std::vector<uint32_t> words;  // windows, 16 words each: (load offset | NAND << 31, store offset) x 8
+    uint32_t nwin = 0;
 };
 
 // Windows of up to 8 instructions in which no NAND reads a cell stored earlier in the window.
-// STO to read-only cells (ignored by the machine) and padding both store to the sink cell.
+// Offsets are bytes into a wave's cells (cell * 256). A NAND stores to the sink cell, a STO loads
+// from it; STO to read-only cells (ignored by the machine) and padding store to it too.
 Encoded encode(const ts_program *p)
 {
     Encoded e;
-    const uint32_t sink = p->ncells;
+    const uint32_t sink = p->ncells * kLanes * 4;
     std::vector<uint32_t> win;
     std::vector<uint32_t> stored;
     auto flush = [&]() {
-        while (win.size() < kWin)
-            win.push_back(sink);  // STO sink
+        while (win.size() < 2 * kWin) {
+            win.push_back(sink);
+            win.push_back(sink);
+        }
         e.words.insert(e.words.end(), win.begin(), win.end());
-        e.nrec++;
+        e.nwin++;
         win.clear();
         stored.clear();
     };
     for (size_t i = 0; i < p->len; i++) {
         const ts_instr in = p->code[i];
         if (in.op == TS_OP_NAND) {
-            bool hazard = std::find(stored.begin(), stored.end(), in.addr) != stored.end();
-            if (hazard)
+            if (std::find(stored.begin(), stored.end(), in.addr) != stored.end())
                 flush();
-            win.push_back(in.addr | 0x80000000u);
+            win.push_back(in.addr * kLanes * 4 | 0x80000000u);
+            win.push_back(sink);
         } else {
             bool ro = in.addr < TS_CELL_FIRST_FREE ||
                       (p->special && p->special[in.addr].kind != TS_CK_PLAIN);
-            uint32_t c = ro ? sink : in.addr;
-            win.push_back(c);
-            stored.push_back(c);
+            win.push_back(sink);
+            win.push_back(ro ? sink : in.addr * kLanes * 4);
+            if (!ro)
+                stored.push_back(in.addr);
         }
-        if (win.size() == kWin)
+        if (win.size() == 2 * kWin)
             flush();
     }
     if (!win.empty())
@@ -340,7 +336,7 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
         Encoded enc = encode(p);
         BatchProg &bp = progs[i];
         bp.rec_off = (uint32_t)(recs.size() / 4);
-        bp.nrec = enc.nrec;
+        bp.nrec = enc.nwin;
         recs.insert(recs.end(), enc.words.begin(), enc.words.end());
         bp.ncells = p->ncells;
         uint32_t writable = p->ncells - TS_CELL_FIRST_FREE;

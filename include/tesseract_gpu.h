@@ -129,7 +129,7 @@ typedef struct {
     unsigned max_passes;    /* default 64 */
     unsigned plain;
     unsigned lds_cells;     /* programs with ncells + 1 <= this keep their machines in LDS (default 130) */
-    unsigned waves_per_cu;  /* persistent waves per CU for HBM-resident programs (default 8) */
+    unsigned waves_per_cu;  /* persistent waves per CU per class, LDS permitting (default 16) */
 } gpu_batch_tune;
 
 typedef struct {
@@ -137,7 +137,8 @@ typedef struct {
     uint64_t ops;           /* 1-bit ops: instructions x passes x machines, summed */
     uint64_t launches;
     unsigned lds_cells;
-    double   class_ms[3];   /* batched: device time per residency class (LDS small, LDS, HBM) */
+    double   class_ms[3];   /* batched: from the start to the end of each residency class's launch
+                               (LDS small, LDS, HBM); the three run concurrently */
     uint64_t class_ops[3];
 } gpu_batch_info;
 

@@ -134,7 +134,9 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
 {
     extern __shared__ uint32_t lds[];
     const uint32_t lane = threadIdx.x;
-    uint8_t *region = (uint8_t *)(LDS ? lds : scratch + (size_t)blockIdx.x * 2 * region_cells * kLanes);
+    // the hare (every instruction touches it) in LDS or HBM; the tortoise (Brent's compare and
+    // copy only) always in this wave's HBM region
+    uint8_t *hbm = (uint8_t *)(scratch + (size_t)blockIdx.x * 2 * region_cells * kLanes);
     for (;;) {
         uint32_t item = 0;
         if (lane == 0)
@@ -146,7 +148,7 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
         const uint32_t pi = __builtin_amdgcn_readfirstlane(item_prog[item]);
         const BatchProg P = progs[pi];
         const uint32_t ncells = P.ncells;
-        Cells hare{region}, tort{region + (size_t)(ncells + 1) * kLanes * 4};
+        Cells hare{LDS ? (uint8_t *)lds : hbm}, tort{hbm + (size_t)region_cells * kLanes * 4};
         const uint4 *rec = recs + P.rec_off;
         uint64_t *st = stats + (size_t)pi * kStats;
 
@@ -298,7 +300,7 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     if (!tune.lds_cells)
         tune.lds_cells = 130;
     if (!tune.waves_per_cu)
-        tune.waves_per_cu = 8;
+        tune.waves_per_cu = 16;
     if (info)
         memset(info, 0, sizeof *info);
     if (!n) {
@@ -364,9 +366,18 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
         say(err, errlen, "%s", "out of device memory for the batch");
         return 0;
     }
-    const unsigned waves = (unsigned)e->cus * tune.waves_per_cu;
-    if (maxcells[2] && !d_scratch.alloc((size_t)waves * 2 * maxcells[2] * kLanes)) {
-        say(err, errlen, "%s", "out of device memory for the HBM-resident machines");
+    // per class: persistent waves per CU (LDS bounds the LDS classes) and its HBM regions
+    unsigned per_cu[3];
+    size_t scratch_off[4] = {0, 0, 0, 0};
+    for (int c = 0; c < 3; c++) {
+        per_cu[c] = tune.waves_per_cu;
+        if (c < 2 && maxcells[c])
+            per_cu[c] = (unsigned)std::max<size_t>(
+                1, std::min<size_t>(per_cu[c], (160u << 10) / ((size_t)maxcells[c] * kLanes * 4)));
+        scratch_off[c + 1] = scratch_off[c] + (size_t)e->cus * per_cu[c] * 2 * maxcells[c] * kLanes;
+    }
+    if (!d_scratch.alloc(scratch_off[3])) {
+        say(err, errlen, "%s", "out of device memory for the machines' HBM regions");
         return 0;
     }
     hipStream_t s = e->stream;
@@ -381,50 +392,62 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     BCHK(hipMemsetAsync(d_stats.p, 0, n * kStats * 8, s));
 
     uint64_t launches = 0;
-    auto launch_class = [&](int cls, uint32_t *queue, uint32_t first, uint32_t count) -> int {
+    auto launch_class = [&](int cls, hipStream_t st, uint32_t *queue, uint32_t first,
+                            uint32_t count) -> int {
         const uint32_t *ip = d_items[cls].p;
+        unsigned grid = std::min<unsigned>((unsigned)e->cus * per_cu[cls], count);
+        uint32_t *sc = d_scratch.p + scratch_off[cls];
         if (cls < 2) {
-            // LDS limits how many waves a CU holds
-            const size_t lds_bytes = (size_t)2 * maxcells[cls] * kLanes * 4;
-            unsigned per_cu = (unsigned)std::max<size_t>(
-                1, std::min<size_t>(tune.waves_per_cu, (160u << 10) / lds_bytes));
-            unsigned grid = std::min<unsigned>((unsigned)e->cus * per_cu, count);
-            hipLaunchKernelGGL(ts_batch<true>, dim3(grid), dim3(kLanes), lds_bytes, s, d_progs.p,
-                               ip, first, count, d_recs.p, d_samples.p, queue,
-                               (uint32_t *)nullptr, 0u, d_stats.p, tune.max_passes);
+            const size_t lds_bytes = (size_t)maxcells[cls] * kLanes * 4;
+            hipLaunchKernelGGL(ts_batch<true>, dim3(grid), dim3(kLanes), lds_bytes, st, d_progs.p,
+                               ip, first, count, d_recs.p, d_samples.p, queue, sc, maxcells[cls],
+                               d_stats.p, tune.max_passes);
         } else {
-            unsigned grid = std::min<unsigned>(waves, count);
-            hipLaunchKernelGGL(ts_batch<false>, dim3(grid), dim3(kLanes), 0, s, d_progs.p, ip,
-                               first, count, d_recs.p, d_samples.p, queue, d_scratch.p,
-                               maxcells[2], d_stats.p, tune.max_passes);
+            hipLaunchKernelGGL(ts_batch<false>, dim3(grid), dim3(kLanes), 0, st, d_progs.p, ip,
+                               first, count, d_recs.p, d_samples.p, queue, sc, maxcells[cls],
+                               d_stats.p, tune.max_passes);
         }
         launches++;
         return hipGetLastError() == hipSuccess;
     };
-    hipEvent_t mark[4];
-    for (int k = 0; k < 4; k++)
-        BCHK(hipEventCreate(&mark[k]));
-    struct Marks {
-        hipEvent_t *m;
-        ~Marks() { for (int k = 0; k < 4; k++) (void)hipEventDestroy(m[k]); }
-    } marks_guard{mark};
+    // batched: the three classes run concurrently, LDS-bound next to HBM-bound work, on side
+    // streams that join the engine's stream; plain: one launch per program on the engine's stream
+    hipStream_t side[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t start_ev = nullptr, done[3] = {nullptr, nullptr, nullptr};
+    struct Cleanup {
+        hipStream_t *s;
+        hipEvent_t *st, *d;
+        ~Cleanup()
+        {
+            for (int k = 0; k < 3; k++) {
+                if (s[k]) (void)hipStreamDestroy(s[k]);
+                if (d[k]) (void)hipEventDestroy(d[k]);
+            }
+            if (*st) (void)hipEventDestroy(*st);
+        }
+    } cleanup{side, &start_ev, done};
     BCHK(hipEventRecord(e->ev0, s));
     if (!tune.plain) {
+        BCHK(hipEventCreateWithFlags(&start_ev, hipEventDisableTiming));
+        BCHK(hipEventRecord(start_ev, s));
         for (int cls = 0; cls < 3; cls++) {
-            BCHK(hipEventRecord(mark[cls], s));
             if (item_of[cls].empty())
                 continue;
-            if (!launch_class(cls, d_queue.p + cls, 0, (uint32_t)item_of[cls].size())) {
+            BCHK(hipStreamCreateWithFlags(&side[cls], hipStreamNonBlocking));
+            BCHK(hipEventCreate(&done[cls]));
+            BCHK(hipStreamWaitEvent(side[cls], start_ev, 0));
+            if (!launch_class(cls, side[cls], d_queue.p + cls, 0, (uint32_t)item_of[cls].size())) {
                 say(err, errlen, "%s", "batch launch failed");
                 return 0;
             }
+            BCHK(hipEventRecord(done[cls], side[cls]));
+            BCHK(hipStreamWaitEvent(s, done[cls], 0));
         }
-        BCHK(hipEventRecord(mark[3], s));
     } else {
         // one launch per program, in program order (the baseline the batch is compared with)
         for (size_t i = 0; i < n; i++) {
             int cls = class_of(items[i].prog);
-            if (!launch_class(cls, d_queue.p + 3 + i, progs[i].first_item, items_per_prog)) {
+            if (!launch_class(cls, s, d_queue.p + 3 + i, progs[i].first_item, items_per_prog)) {
                 say(err, errlen, "%s", "batch launch failed");
                 return 0;
             }
@@ -445,8 +468,9 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     if (info && !tune.plain)
         for (int k = 0; k < 3; k++) {
             float cms = 0;
-            BCHK(hipEventElapsedTime(&cms, mark[k], mark[k + 1]));
-            info->class_ms[k] = cms;
+            if (done[k])
+                BCHK(hipEventElapsedTime(&cms, e->ev0, done[k]));
+            info->class_ms[k] = cms;  // from the batch's start to the class's end (they overlap)
         }
     if (info) {
         for (size_t i = 0; i < n; i++)

@@ -70,6 +70,58 @@ def run_batch(engine, n, groups, passes, seed, plain_too=True, lds_cells=0, wave
     return out
 
 
+def run_life(engine, gens, check=True, tune=None):
+    from .life import LifeCube
+    t0 = time.perf_counter()
+    life = LifeCube(engine, tune=tune)  # 8192 x 8192 torus on 65,536 machines, 32 x 32 tiles
+    setup = time.perf_counter() - t0
+    info = life.code.stats()
+    life.run(1)  # warm-up generation
+    ms = life.run(gens)
+    out = {"workload": "life", "board": "8192x8192", "machines": 65536, "generations": gens,
+           "instructions_per_generation": life.prog.len, "segments": info["segments"],
+           "slices": info["slices"], "bitop3_per_generation_word": info["bitop3"],
+           "compile_and_setup_s": setup, "ms": ms, "ms_per_generation": ms / gens,
+           "cell_updates_per_s": 8192.0 * 8192 * gens / (ms * 1e-3),
+           "ops_per_s": life.prog.len * 65536.0 * gens / (ms * 1e-3)}
+    if check:
+        from oracle import pyoracle as O  # the checker only: the run above is finished
+        start = life.start_bits()
+        want = O.life_words(start, gens + 1)
+        out["bit_exact_vs_plain_c"] = bool(np.array_equal(life.board_bits(), want))
+    return out
+
+
+def run_benes(engine, cubes, seed):
+    from .cube import Perm8, Sort8
+    rng = np.random.default_rng(seed)
+    vals = rng.integers(0, 256, (cubes, 512)).astype(np.uint8)
+    dests = np.stack([rng.permutation(512) for _ in range(cubes)])
+    perm, sort = Perm8(engine), Sort8(engine)
+    perm.run(vals[:1], dests[:1])
+    sort.run(vals[:1])
+    got_p, ms_p = perm.run(vals, dests)
+    got_s, ms_s = sort.run(vals)
+    ok_p = all((got_p[i, dests[i]] == vals[i]).all() for i in range(cubes))
+    ok_s = bool((got_s == np.sort(vals, axis=1)).all())
+    return {"workload": "benes", "cubes": cubes, "perm8_instructions": perm.prog.len,
+            "sort8_instructions": sort.prog.len, "perm_ms": ms_p, "sort_ms": ms_s,
+            "permutations_per_s": cubes / (ms_p * 1e-3), "sorts_per_s": cubes / (ms_s * 1e-3),
+            "perm_ops_per_s": perm.prog.len * 512.0 * cubes / (ms_p * 1e-3),
+            "sort_ops_per_s": sort.prog.len * 512.0 * cubes / (ms_s * 1e-3),
+            "perm_bit_exact": ok_p, "sort_bit_exact": ok_s}
+
+
+def run_hash(engine, target=0x165C2A07):
+    prog = t.gen_hash(32, 8)
+    code = t.EnumCode(engine, t.EN_SEARCH, prog, spec="h = %d" % target)
+    code.run()
+    r = code.run()
+    return {"workload": "hash", "program": "gen_hash(32, 8)", "instructions": prog.len,
+            "target": target, "assignments": r["tried"], "hits": r["hits"], "hit": r["hit"],
+            "ms": engine.last_ms, "ops_per_s": r["ops"] / (engine.last_ms * 1e-3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("workload", choices=["batch", "life", "hash", "benes"])
@@ -79,6 +131,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--gens", type=int, default=1000)
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--cubes", type=int, default=4096)
     ap.add_argument("--lds-cells", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
     a = ap.parse_args()
@@ -86,8 +139,12 @@ def main():
     if a.workload == "batch":
         out = run_batch(eng, a.programs, a.groups, a.passes, a.seed, not a.no_plain, a.lds_cells,
                         a.waves_per_cu)
+    elif a.workload == "life":
+        out = run_life(eng, a.gens)
+    elif a.workload == "benes":
+        out = run_benes(eng, a.cubes, a.seed)
     else:
-        raise SystemExit("not built yet: " + a.workload)
+        out = run_hash(eng)
     out["device"] = eng.describe()
     print(json.dumps(out), flush=True)
 

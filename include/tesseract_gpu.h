@@ -34,6 +34,8 @@ typedef struct {
     unsigned slice_nodes;  /* AIG nodes per slice for segmented programs (0 = default) */
     unsigned lut3;         /* 1 = map to v_bitop3_b32 (default), 2 = plain AND/NOT */
     unsigned grid;         /* workgroups for enumeration kernels (0 = default) */
+    unsigned graph_passes; /* segmented programs: passes per captured HIP graph (default 32,
+                              1 = launch every segment directly) */
 } gpu_tune;
 
 typedef struct {

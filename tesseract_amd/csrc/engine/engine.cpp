@@ -40,6 +40,10 @@ struct gpu_state {
     uint32_t  nw = 0;        // words = groups * 16
     uint32_t *slots = nullptr, *init = nullptr;
     int       latest = 1;    // parity holding the newest values of written cells
+    // segmented programs: `graph_passes` passes of segment launches captured once per starting
+    // parity and replayed, instead of one launch per segment per pass
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    unsigned       graph_passes = 0;
 };
 
 namespace {
@@ -76,6 +80,8 @@ gpu_tune defaults(const gpu_engine *e, const gpu_tune *t)
         d.lut3 = 1;
     if (!d.grid)
         d.grid = (unsigned)std::max(1, e->cus) * 8;
+    if (!d.graph_passes)
+        d.graph_passes = 32;
     return d;
 }
 
@@ -361,6 +367,9 @@ extern "C" void gpu_state_free(gpu_state *s)
     if (!s)
         return;
     hipSetDevice(s->c->e->device);
+    for (int k = 0; k < 2; k++)
+        if (s->graph[k])
+            (void)hipGraphExecDestroy(s->graph[k]);
     hipFree(s->slots);
     hipFree(s->init);
     delete s;
@@ -500,9 +509,10 @@ extern "C" int gpu_run(gpu_code *c, gpu_state *s, uint64_t passes, char *err, si
             left -= now;
         }
     } else {
-        for (uint64_t p = 0; p < passes; p++) {
-            uint32_t *prev = s->slots + (size_t)s->latest * lw.nslots * nw;
-            uint32_t *cur = s->slots + (size_t)(1 - s->latest) * lw.nslots * nw;
+        // one pass: every segment's launch, parity flipping after the pass
+        auto one_pass = [&](int latest) -> int {
+            uint32_t *prev = s->slots + (size_t)latest * lw.nslots * nw;
+            uint32_t *cur = s->slots + (size_t)(1 - latest) * lw.nslots * nw;
             for (size_t k = 0; k < c->fns.size(); k++) {
                 if (!c->slices[k])
                     continue;
@@ -512,6 +522,41 @@ extern "C" int gpu_run(gpu_code *c, gpu_state *s, uint64_t passes, char *err, si
                             errlen))
                     return 0;
             }
+            return 1;
+        };
+        const unsigned gp = c->tune.graph_passes;
+        uint64_t p = 0;
+        if (gp > 1 && passes >= gp) {
+            if (!s->graph[s->latest]) {
+                // capture gp passes starting from this parity (kernel launches only)
+                hipGraph_t g;
+                HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
+                int lat = s->latest, ok = 1;
+                for (unsigned q = 0; q < gp && ok; q++, lat = 1 - lat)
+                    ok = one_pass(lat);
+                hipError_t ce = hipStreamEndCapture(e->stream, &g);
+                if (!ok || ce != hipSuccess) {
+                    say(err, errlen, "graph capture failed: %s", hipGetErrorString(ce));
+                    return 0;
+                }
+                hipError_t ie = hipGraphInstantiate(&s->graph[s->latest], g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (ie != hipSuccess) {
+                    say(err, errlen, "graph instantiate failed: %s", hipGetErrorString(ie));
+                    return 0;
+                }
+            }
+            for (; p + gp <= passes; p += gp) {
+                HIPCHK(hipGraphLaunch(s->graph[s->latest], e->stream));
+                if (gp & 1)
+                    s->latest = 1 - s->latest;
+                if (!s->graph[s->latest] && p + 2 * gp <= passes)
+                    break;  // an odd pass count changed the parity: capture that one next call
+            }
+        }
+        for (; p < passes; p++) {
+            if (!one_pass(s->latest))
+                return 0;
             s->latest = 1 - s->latest;
         }
     }

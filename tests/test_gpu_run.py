@@ -91,3 +91,26 @@ def test_life_cube_small(engine):
     prog, board = t.gen_life_cube(4, 4, 6, 6)  # dims 12: cross-wave links, segments
     print("check life cube 4x4 on 6+6 dims", flush=True)
     check(engine, prog, runs=(1, 2))
+
+
+def test_graph_replays_match_eager(engine):
+    # segmented program, passes replayed from captured graphs of 4 passes: several replays with
+    # eager launches in between, each checked against the oracle
+    prog = random_cube_program(20, 300, 12, seed=77)
+    code = engine.compile(prog, {"graph_passes": 4})
+    assert code.stats()["segments"] > 1
+    mem, rr = O.random_lines(prog, prog.groups, 5)
+    st = code.state()
+    st.put(mem, rr)
+    done = 0
+    for n in (9, 8, 3, 4):
+        st.run(n)
+        done += n
+        gm, gr = st.get()
+        wm, wr = O.run_lines(prog, mem, rr, done)
+        assert np.array_equal(gm, wm) and np.array_equal(gr, wr), "after %d passes" % done
+    code1 = engine.compile(prog, {"graph_passes": 1})
+    st1 = code1.state()
+    st1.put(mem, rr)
+    st1.run(done)
+    assert all(np.array_equal(a, b) for a, b in zip(st1.get(), st.get()))

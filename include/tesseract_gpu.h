@@ -147,6 +147,36 @@ typedef struct {
 int  gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t n, const gpu_batch_tune *tune,
                    gpu_batch_stats *stats, gpu_batch_info *info, char *err, size_t errlen);
 
+/* Serving: verify / equiv / search requests submitted over time and run with continuous batching.
+   A request is copied, compiled on a pool of host threads (hipRTC), admitted to the device when its
+   code fits the memory budget and fewer than max_active requests run, and enumerated in chunks of
+   chunk_words words; `streams` device slots each run one chunk at a time, taking the next chunk of
+   the admitted request with the least work left, so requests that arrive while others run start
+   within a chunk and their work overlaps on the device. Each result is exactly what gpu_enum_run
+   returns for the request alone. Compiled modules stay resident for repeated circuits. */
+typedef struct gpu_server gpu_server;
+typedef struct {
+    unsigned streams;          /* device slots (default 4) */
+    unsigned compile_threads;  /* host compile threads (default 8) */
+    uint64_t memory_budget;    /* bytes of resident compiled code (default 1 GiB) */
+    uint64_t chunk_words;      /* words (32 assignments each) per launch (default 2^24) */
+    unsigned max_active;       /* admitted requests at once (default 64) */
+} gpu_serve_tune;
+typedef struct {
+    double   submitted_s, compiled_s, admitted_s, started_s, done_s;  /* steady clock, seconds */
+    uint64_t launches;
+} gpu_request_times;
+/* state: 0 queued, 1 compiling, 2 waiting for admission, 3 running, 4 done, 5 failed */
+int  gpu_server_open(gpu_engine *e, const gpu_serve_tune *tune, gpu_server **out, char *err,
+                     size_t errlen);
+int  gpu_server_submit(gpu_server *s, const gpu_enum *request, uint64_t *ticket, char *err,
+                       size_t errlen);
+int  gpu_server_poll(gpu_server *s, uint64_t ticket, ts_enum_result *r, int *state, char *err,
+                     size_t errlen);
+int  gpu_server_wait(gpu_server *s, uint64_t ticket, ts_enum_result *r, char *err, size_t errlen);
+int  gpu_server_times(gpu_server *s, uint64_t ticket, gpu_request_times *t);
+void gpu_server_close(gpu_server *s);
+
 /* Listings without a device (the counterpart of `alpha jit`, reference src/main.c:468-496):
    the HIP source gpu_compile / gpu_enum_run would build, and a hipRTC compile of any such source
    for an architecture ("gfx950"), which needs no GPU. */

@@ -74,6 +74,16 @@ class GpuBatchInfo(C.Structure):
                 ("lds_cells", C.c_uint), ("class_ms", C.c_double * 3), ("class_ops", C.c_uint64 * 3)]
 
 
+class GpuServeTune(C.Structure):
+    _fields_ = [("streams", C.c_uint), ("compile_threads", C.c_uint), ("memory_budget", C.c_uint64),
+                ("chunk_words", C.c_uint64), ("max_active", C.c_uint)]
+
+
+class GpuRequestTimes(C.Structure):
+    _fields_ = [("submitted_s", C.c_double), ("compiled_s", C.c_double), ("admitted_s", C.c_double),
+                ("started_s", C.c_double), ("done_s", C.c_double), ("launches", C.c_uint64)]
+
+
 class GpuEnumStruct(C.Structure):
     _fields_ = [("mode", C.c_int), ("a", C.POINTER(ProgramStruct)), ("b", C.POINTER(ProgramStruct)),
                 ("spec", C.c_char_p), ("ticks", C.c_uint), ("part", C.c_int), ("part_end", C.c_int),
@@ -161,6 +171,13 @@ def lib():
             "gpu_enum_free": (None, [vp]),
             "gpu_batch_run": (i32, [vp, C.POINTER(GpuBatchItem), sz, C.POINTER(GpuBatchTune),
                                     C.POINTER(GpuBatchStats), C.POINTER(GpuBatchInfo), C.c_char_p, sz]),
+            "gpu_server_open": (i32, [vp, C.POINTER(GpuServeTune), C.POINTER(vp), C.c_char_p, sz]),
+            "gpu_server_submit": (i32, [vp, C.POINTER(GpuEnumStruct), C.POINTER(u64), C.c_char_p, sz]),
+            "gpu_server_poll": (i32, [vp, u64, C.POINTER(EnumResultStruct), C.POINTER(i32),
+                                      C.c_char_p, sz]),
+            "gpu_server_wait": (i32, [vp, u64, C.POINTER(EnumResultStruct), C.c_char_p, sz]),
+            "gpu_server_times": (i32, [vp, u64, C.POINTER(GpuRequestTimes)]),
+            "gpu_server_close": (None, [vp]),
             "free": (None, [vp]),
         }
         for name, (res, args) in sig.items():
@@ -567,6 +584,58 @@ class EnumCode:
         if not lib().gpu_enum_exec(self._c, C.byref(r), e, len(e)):
             raise RuntimeError(e.value.decode())
         return _result(r)
+
+
+class Server:
+    """gpu_server: continuous-batching request loop on one engine."""
+
+    def __init__(self, engine, streams=0, compile_threads=0, memory_budget=0, chunk_words=0,
+                 max_active=0):
+        self.engine = engine
+        tune = GpuServeTune(streams, compile_threads, memory_budget, chunk_words, max_active)
+        h = C.c_void_p()
+        e = _err()
+        if not lib().gpu_server_open(engine._e, C.byref(tune), C.byref(h), e, len(e)):
+            raise RuntimeError("gpu_server_open: " + e.value.decode())
+        self._s = h
+
+    def submit(self, mode, a, b=None, spec=None, ticks=1, part=0, part_end=0, parts=1):
+        en = _enum_struct(mode, a, b, spec, ticks, part, part_end, parts, None)
+        tk = C.c_uint64()
+        e = _err()
+        if not lib().gpu_server_submit(self._s, C.byref(en), C.byref(tk), e, len(e)):
+            raise RuntimeError("gpu_server_submit: " + e.value.decode())
+        return tk.value
+
+    def poll(self, ticket):
+        r = EnumResultStruct()
+        st = C.c_int()
+        e = _err()
+        if not lib().gpu_server_poll(self._s, ticket, C.byref(r), C.byref(st), e, len(e)):
+            raise RuntimeError(e.value.decode())
+        return st.value, (_result(r) if st.value == 4 else None), e.value.decode()
+
+    def wait(self, ticket):
+        r = EnumResultStruct()
+        e = _err()
+        if not lib().gpu_server_wait(self._s, ticket, C.byref(r), e, len(e)):
+            raise ValueError(e.value.decode())
+        return _result(r)
+
+    def times(self, ticket):
+        t = GpuRequestTimes()
+        lib().gpu_server_times(self._s, ticket, C.byref(t))
+        return {f: getattr(t, f) for f, _ in GpuRequestTimes._fields_}
+
+    def close(self):
+        if self._s:
+            lib().gpu_server_close(self._s)
+            self._s = None
+
+    def __del__(self):
+        if getattr(self, "_s", None) and _lib is not None:
+            _lib.gpu_server_close(self._s)
+            self._s = None
 
 
 class Code:

@@ -16,7 +16,7 @@ OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libtesseract.so")
 
 HOST_C = ["program", "builder", "gen", "spec", "tester", "benes", "net"]
-ENGINE_CPP = ["lower", "codegen", "engine", "calibrate", "batch"]
+ENGINE_CPP = ["lower", "codegen", "engine", "calibrate", "batch", "serve"]
 ARCH = "gfx950"
 # Packed FP32 stays off in every HIP compile.
 NO_PACKED = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -44,9 +44,10 @@ def build(verbose=False, jobs=8):
         if not os.path.exists(src):
             continue
         obj = os.path.join(OBJ, name + ".o")
-        cmd = ["hipcc", "-std=c++17", "-O2", "-fPIC", "-I" + INCLUDE, "-Wno-unused-command-line-argument"]
+        cmd = ["hipcc", "-std=c++17", "-O2", "-fPIC", "-I" + INCLUDE, "-Wno-unused-command-line-argument",
+               "--offload-arch=" + ARCH] + NO_PACKED
         if src.endswith(".hip"):
-            cmd += ["--offload-arch=" + ARCH, "-O3"] + NO_PACKED
+            cmd += ["-O3"]
         cmds.append(cmd + ["-c", src, "-o", obj])
         objs.append(obj)
     procs = []

@@ -85,8 +85,10 @@ gpu_tune defaults(const gpu_engine *e, const gpu_tune *t)
     return d;
 }
 
+}  // namespace
+
 // hipRTC: source -> gfx950 code object (needs no device)
-int rtc_compile(const std::string &src, const std::string &arch_name, std::vector<char> &code,
+int ts_internal::rtc_compile(const std::string &src, const std::string &arch_name, std::vector<char> &code,
                 double *seconds, char *err, size_t errlen)
 {
     auto t0 = std::chrono::steady_clock::now();
@@ -119,11 +121,13 @@ int rtc_compile(const std::string &src, const std::string &arch_name, std::vecto
     return 1;
 }
 
+namespace {
+
 int build_module(gpu_engine *e, const std::string &src, hipModule_t *mod, double *seconds, char *err,
                  size_t errlen)
 {
     std::vector<char> code;
-    if (!rtc_compile(src, e->arch, code, seconds, err, errlen))
+    if (!ts_internal::rtc_compile(src, e->arch, code, seconds, err, errlen))
         return 0;
     HIPCHK(hipSetDevice(e->device));
     HIPCHK(hipModuleLoadData(mod, code.data()));
@@ -311,7 +315,7 @@ extern "C" int gpu_rtc_check(const char *source, const char *arch, double *secon
                              size_t *code_bytes, char *err, size_t errlen)
 {
     std::vector<char> code;
-    if (!rtc_compile(source, arch ? arch : "gfx950", code, seconds, err, errlen))
+    if (!ts_internal::rtc_compile(source, arch ? arch : "gfx950", code, seconds, err, errlen))
         return 0;
     if (code_bytes)
         *code_bytes = code.size();
@@ -570,15 +574,11 @@ extern "C" int gpu_run(gpu_code *c, gpu_state *s, uint64_t passes, char *err, si
 
 // ---- enumeration ----------------------------------------------------------------------------
 
-namespace {
-
-struct ShareInfo {
-    uint64_t space, passes, nchunks, tried, passes_run, nact_words;
-    unsigned nshare;
-};
+using ts_internal::ShareInfo;
+using ts_internal::kListCap;
 
 // reference src/enum.c:215-242: passes of 512 in chunks of 64; chunk c belongs to share c % parts
-ShareInfo shares(unsigned n, int part, int part_end, int parts)
+ShareInfo ts_internal::shares(unsigned n, int part, int part_end, int parts)
 {
     ShareInfo si{};
     si.space = 1ull << n;
@@ -605,13 +605,13 @@ ShareInfo shares(unsigned n, int part, int part_end, int parts)
     return si;
 }
 
-}  // namespace
-
 namespace {
 
+}  // namespace
+
 // validation (the reference's messages, src/enum.c:244-268), spec parsing and code generation
-int enum_prepare(const gpu_engine *e, const gpu_enum *en, gpu_tune &tune, KernelSource &ks,
-                 char *err, size_t errlen)
+int ts_internal::enum_prepare(const gpu_engine *e, const gpu_enum *en, gpu_tune &tune,
+                              KernelSource &ks, char *err, size_t errlen)
 {
     const ts_program *a = en->a, *b = en->b;
     if (a->special || (en->mode == TS_EN_EQUIV && b && b->special)) {
@@ -666,15 +666,13 @@ int enum_prepare(const gpu_engine *e, const gpu_enum *en, gpu_tune &tune, Kernel
     return 1;
 }
 
-}  // namespace
-
 extern "C" int gpu_enum_source(const gpu_enum *en, char **source, char *err, size_t errlen)
 {
     gpu_engine none;
     none.cus = 256;
     gpu_tune tune;
     KernelSource ks;
-    if (!enum_prepare(&none, en, tune, ks, err, errlen))
+    if (!ts_internal::enum_prepare(&none, en, tune, ks, err, errlen))
         return 0;
     *source = strdup(ks.text.c_str());
     return 1;
@@ -692,7 +690,6 @@ struct gpu_enum_code {
     double        compile_seconds = 0;
 };
 
-static const uint64_t kListCap = 1u << 16;
 
 extern "C" int gpu_enum_compile(gpu_engine *e, const gpu_enum *en, gpu_enum_code **out, char *err,
                                 size_t errlen)
@@ -701,7 +698,7 @@ extern "C" int gpu_enum_compile(gpu_engine *e, const gpu_enum *en, gpu_enum_code
     std::unique_ptr<gpu_enum_code> c(new gpu_enum_code());
     c->e = e;
     KernelSource ks;
-    if (!enum_prepare(e, en, c->tune, ks, err, errlen))
+    if (!ts_internal::enum_prepare(e, en, c->tune, ks, err, errlen))
         return 0;
     if (!build_module(e, ks.text, &c->mod, &c->compile_seconds, err, errlen))
         return 0;
@@ -748,7 +745,7 @@ extern "C" int gpu_enum_exec(gpu_enum_code *c, ts_enum_result *r, char *err, siz
     gpu_engine *e = c->e;
     HIPCHK(hipSetDevice(e->device));
     const gpu_tune tune = c->tune;
-    ShareInfo si = shares(c->n, c->part, c->part_end, c->parts);
+    ShareInfo si = ts_internal::shares(c->n, c->part, c->part_end, c->parts);
     r->space = si.space;
     r->tried = si.tried;
     r->ops = si.passes_run * c->ticks * c->len_total * 512;

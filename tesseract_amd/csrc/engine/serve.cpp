@@ -1,0 +1,589 @@
+// Serving: verify / equiv / search requests arriving over time, with continuous batching.
+//
+// A submitted request is copied (programs and spec), code-generated and compiled through hipRTC on a
+// pool of host threads, then admitted to the device when its footprint (code object, launch
+// buffers) fits the memory budget and an admission slot is free. Every admitted enumeration is cut
+// into chunks of words; a scheduler thread keeps `streams` device slots busy, each running one
+// chunk at a time, picking the next chunk from the admitted request with the least work left — so
+// a request that arrives while a large one is running starts within a chunk's time, and chunks of
+// different requests run side by side on the device. A chunk's grid takes a 1/streams share of the
+// device's resident waves. Results are exact and independent of the mix: words are enumerated in
+// increasing x, a chunk's hits all precede the next chunk's, so hit counts add and the 16 smallest
+// come from the earliest chunks (a chunk whose hit list overflowed is listed again by halves).
+// Compiled modules stay resident, keyed by their source, for requests that repeat a circuit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdarg>
+#include <cstring>
+#include <deque>
+#include <functional>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "engine_internal.hpp"
+
+using namespace ts_internal;
+
+namespace {
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void put_err(char *err, size_t errlen, const char *fmt, ...)
+{
+    if (!err || !errlen)
+        return;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(err, errlen, fmt, ap);
+    va_end(ap);
+}
+
+enum { ST_QUEUED = 0, ST_COMPILING = 1, ST_WAITING = 2, ST_RUNNING = 3, ST_DONE = 4, ST_FAILED = 5 };
+
+struct Chunk {
+    uint64_t i0, i1;
+    uint64_t hits = 0;
+    std::vector<uint64_t> list;  // (word, mask) pairs
+    bool overflow = false, done = false;
+};
+
+struct Module {
+    hipModule_t mod = nullptr;
+    hipFunction_t fn = nullptr;
+    size_t bytes = 0;
+    int users = 0;
+    double last_use = 0;
+    unsigned blocks_per_cu = 1;
+};
+
+struct Request {
+    uint64_t id = 0;
+    gpu_enum en{};
+    ts_program a{}, b{};
+    bool has_b = false;
+    std::string spec;
+    std::string source, error;
+    std::vector<char> code;
+    gpu_tune tune{};
+    ShareInfo si{};
+    Module *mod = nullptr;
+    std::vector<Chunk> chunks;
+    size_t next_chunk = 0, chunks_done = 0;
+    uint64_t words_left = 0;
+    ts_enum_result result{};
+    int state = ST_QUEUED;
+    gpu_request_times times{};
+    uint64_t len_total = 0;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    uint64_t *d_out = nullptr, *d_list = nullptr;
+    uint64_t *h_out = nullptr;  // pinned
+    Request *req = nullptr;
+    size_t chunk = 0;
+};
+
+}  // namespace
+
+struct gpu_server {
+    gpu_engine *e = nullptr;
+    gpu_serve_tune tune{};
+    std::mutex mu;
+    std::condition_variable cv_compile, cv_sched, cv_done;
+    std::deque<Request *> to_compile, ready;
+    std::vector<Request *> active;
+    std::unordered_map<uint64_t, std::unique_ptr<Request>> all;
+    std::unordered_map<std::string, Module> modules;
+    std::vector<Slot> slots;
+    Slot relist_slot;  // synchronous re-listing of overflowed chunks, off the dispatch slots
+    std::vector<std::thread> compilers;
+    std::thread scheduler;
+    std::atomic<bool> stop{false};
+    uint64_t next_id = 1;
+    size_t module_bytes = 0;
+    std::string fatal;
+};
+
+namespace {
+
+void compile_loop(gpu_server *s)
+{
+    for (;;) {
+        Request *r;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_compile.wait(lk, [&] { return s->stop || !s->to_compile.empty(); });
+            if (s->stop && s->to_compile.empty())
+                return;
+            r = s->to_compile.front();
+            s->to_compile.pop_front();
+            r->state = ST_COMPILING;
+        }
+        char err[1024] = "";
+        ts::KernelSource ks;
+        bool ok = enum_prepare(s->e, &r->en, r->tune, ks, err, sizeof err) != 0;
+        if (ok) {
+            r->source = ks.text;
+            bool cached;
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                cached = s->modules.count(r->source) != 0;
+            }
+            if (!cached)
+                ok = rtc_compile(r->source, s->e->arch, r->code, nullptr, err, sizeof err) != 0;
+        }
+        std::lock_guard<std::mutex> lk(s->mu);
+        r->times.compiled_s = now_s();
+        if (!ok) {
+            r->error = err;
+            r->state = ST_FAILED;
+            r->times.done_s = r->times.compiled_s;
+            s->cv_done.notify_all();
+        } else {
+            r->state = ST_WAITING;
+            s->ready.push_back(r);
+            s->cv_sched.notify_all();
+        }
+    }
+}
+
+// device side of one request: load (or reuse) its module and cut its share into chunks
+bool admit(gpu_server *s, Request *r)
+{
+    Module &m = s->modules[r->source];
+    if (!m.mod) {
+        if (r->code.empty()) {  // its module was evicted after the compile found it resident
+            char err[512];
+            if (!rtc_compile(r->source, s->e->arch, r->code, nullptr, err, sizeof err)) {
+                r->error = err;
+                s->modules.erase(r->source);
+                return false;
+            }
+        }
+        if (hipModuleLoadData(&m.mod, r->code.data()) != hipSuccess ||
+            hipModuleGetFunction(&m.fn, m.mod, "ts_enum") != hipSuccess) {
+            r->error = "module load failed";
+            s->modules.erase(r->source);
+            return false;
+        }
+        m.bytes = r->code.size();
+        s->module_bytes += m.bytes;
+        int blocks = 1;
+        if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, m.fn, r->tune.block, 0) !=
+                hipSuccess ||
+            blocks < 1)
+            blocks = 1;
+        m.blocks_per_cu = (unsigned)blocks;
+    }
+    m.users++;
+    m.last_use = now_s();
+    r->mod = &m;
+    r->code.clear();
+    r->code.shrink_to_fit();
+    r->si = shares((unsigned)r->en.a->ninputs, r->en.part, r->en.part_end, r->en.parts);
+    r->result.space = r->si.space;
+    r->result.tried = r->si.tried;
+    r->result.ops = r->si.passes_run * r->en.ticks * r->len_total * 512;
+    r->result.threads = 1;
+    const uint64_t step = std::max<uint64_t>(1024, s->tune.chunk_words);
+    for (uint64_t i = 0; i < r->si.nact_words; i += step) {
+        Chunk c;
+        c.i0 = i;
+        c.i1 = std::min(r->si.nact_words, i + step);
+        r->chunks.push_back(c);
+    }
+    r->words_left = r->si.nact_words;
+    r->state = ST_RUNNING;
+    r->times.admitted_s = now_s();
+    return true;
+}
+
+// drop modules nobody uses, oldest first, until the budget holds `need` more bytes
+void evict(gpu_server *s, size_t need)
+{
+    while (s->module_bytes + need > s->tune.memory_budget) {
+        auto victim = s->modules.end();
+        for (auto it = s->modules.begin(); it != s->modules.end(); ++it)
+            if (it->second.users == 0 && it->second.mod &&
+                (victim == s->modules.end() || it->second.last_use < victim->second.last_use))
+                victim = it;
+        if (victim == s->modules.end())
+            return;
+        (void)hipModuleUnload(victim->second.mod);
+        s->module_bytes -= victim->second.bytes;
+        s->modules.erase(victim);
+    }
+}
+
+bool launch_chunk(gpu_server *s, Slot &sl, Request *r, size_t ci)
+{
+    Chunk &c = r->chunks[ci];
+    const unsigned block = r->tune.block;
+    uint64_t blocks = (c.i1 - c.i0 + block - 1) / block;
+    uint64_t share = std::max<uint64_t>(1, (uint64_t)s->e->cus * r->mod->blocks_per_cu / s->slots.size());
+    unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, share));
+    uint64_t space = r->si.space, cap = kListCap, i0 = c.i0, i1 = c.i1;
+    uint32_t parts = (uint32_t)r->en.parts, part = (uint32_t)r->en.part, nshare = r->si.nshare;
+    void *args[] = {&sl.d_out, &sl.d_list, &cap, &i0, &i1, &space, &parts, &part, &nshare};
+    if (hipMemsetAsync(sl.d_out, 0, 16, sl.stream) != hipSuccess ||
+        hipModuleLaunchKernel(r->mod->fn, grid, 1, 1, block, 1, 1, 0, sl.stream, args, nullptr) !=
+            hipSuccess ||
+        hipMemcpyAsync(sl.h_out, sl.d_out, 16, hipMemcpyDeviceToHost, sl.stream) != hipSuccess ||
+        hipEventRecord(sl.done, sl.stream) != hipSuccess)
+        return false;
+    sl.req = r;
+    sl.chunk = ci;
+    if (r->times.started_s == 0)
+        r->times.started_s = now_s();
+    r->times.launches++;
+    return true;
+}
+
+// a finished chunk: counts from the pinned copy, then its hit list (a short synchronous copy)
+bool finish_chunk(gpu_server *s, Slot &sl)
+{
+    (void)s;
+    Request *r = sl.req;
+    Chunk &c = r->chunks[sl.chunk];
+    c.hits = sl.h_out[0];
+    uint64_t nlist = sl.h_out[1];
+    if (nlist > kListCap) {
+        c.overflow = true;
+    } else if (nlist) {
+        c.list.resize(2 * nlist);
+        if (hipMemcpyAsync(c.list.data(), sl.d_list, 16 * nlist, hipMemcpyDeviceToHost, sl.stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(sl.stream) != hipSuccess)
+            return false;
+    }
+    c.done = true;
+    r->chunks_done++;
+    r->words_left -= c.i1 - c.i0;
+    sl.req = nullptr;
+    return true;
+}
+
+// the 16 smallest hits of an overflowed range, by halves, on one slot's stream (synchronous)
+bool relist(gpu_server *s, Slot &sl, Request *r, uint64_t i0, uint64_t i1, ts_enum_result &res)
+{
+    if (i0 >= i1 || res.nhit >= TS_EN_SHOW)
+        return true;
+    Chunk tmp;
+    tmp.i0 = i0;
+    tmp.i1 = i1;
+    r->chunks.push_back(tmp);
+    size_t ci = r->chunks.size() - 1;
+    if (!launch_chunk(s, sl, r, ci) || hipEventSynchronize(sl.done) != hipSuccess)
+        return false;
+    Chunk &c = r->chunks[ci];
+    c.hits = sl.h_out[0];
+    uint64_t nlist = sl.h_out[1];
+    sl.req = nullptr;
+    if (nlist > kListCap) {
+        uint64_t mid = i0 + (i1 - i0) / 2;
+        return relist(s, sl, r, i0, mid, res) && relist(s, sl, r, mid, i1, res);
+    }
+    std::vector<uint64_t> list(2 * nlist);
+    if (nlist && (hipMemcpy(list.data(), sl.d_list, 16 * nlist, hipMemcpyDeviceToHost) != hipSuccess))
+        return false;
+    for (size_t k = 0; k + 1 < list.size(); k += 2)
+        for (uint32_t m = (uint32_t)list[k + 1]; m; m &= m - 1)
+            ts_enum_add_hit(&res, list[k] * 32 + (uint64_t)__builtin_ctz(m));
+    return true;
+}
+
+bool finalize(gpu_server *s, Request *r)
+{
+    ts_enum_result &res = r->result;
+    const size_t n = r->chunks.size();
+    for (size_t k = 0; k < n; k++)
+        res.hits += r->chunks[k].hits;
+    for (size_t k = 0; k < n && res.nhit < TS_EN_SHOW; k++) {
+        Chunk &c = r->chunks[k];
+        if (c.overflow) {
+            if (!relist(s, s->relist_slot, r, c.i0, c.i1, res))
+                return false;
+            continue;
+        }
+        for (size_t q = 0; q + 1 < c.list.size(); q += 2)
+            for (uint32_t m = (uint32_t)c.list[q + 1]; m; m &= m - 1)
+                ts_enum_add_hit(&res, c.list[q] * 32 + (uint64_t)__builtin_ctz(m));
+    }
+    r->chunks.clear();
+    r->chunks.shrink_to_fit();
+    return true;
+}
+
+void schedule_loop(gpu_server *s)
+{
+    if (hipSetDevice(s->e->device) != hipSuccess) {
+        s->fatal = "hipSetDevice failed";
+        return;
+    }
+    for (;;) {
+        std::unique_lock<std::mutex> lk(s->mu);
+        bool busy = false;
+        // admission: budget and active-request limit
+        while (!s->ready.empty() && s->active.size() < s->tune.max_active) {
+            Request *r = s->ready.front();
+            size_t need = s->modules.count(r->source) ? 0 : r->code.size();
+            evict(s, need);
+            if (need && s->module_bytes + need > s->tune.memory_budget && !s->active.empty())
+                break;  // wait for memory to free up
+            s->ready.pop_front();
+            if (!admit(s, r)) {
+                r->state = ST_FAILED;
+                r->times.done_s = now_s();
+                s->cv_done.notify_all();
+                continue;
+            }
+            s->active.push_back(r);
+            busy = true;
+        }
+        // completions
+        for (Slot &sl : s->slots) {
+            if (!sl.req)
+                continue;
+            hipError_t q = hipEventQuery(sl.done);
+            if (q == hipErrorNotReady)
+                continue;
+            if (q != hipSuccess || !finish_chunk(s, sl)) {
+                s->fatal = "a chunk failed on the device";
+                s->stop = true;
+                s->cv_done.notify_all();
+                return;
+            }
+            busy = true;
+        }
+        // finished requests
+        for (size_t k = 0; k < s->active.size();) {
+            Request *r = s->active[k];
+            if (r->chunks_done == r->chunks.size() && r->next_chunk == r->chunks.size()) {
+                bool ok = finalize(s, r);
+                r->mod->users--;
+                r->state = ok ? ST_DONE : ST_FAILED;
+                if (!ok)
+                    r->error = "listing hits failed";
+                r->times.done_s = now_s();
+                s->active.erase(s->active.begin() + (long)k);
+                s->cv_done.notify_all();
+                busy = true;
+            } else {
+                k++;
+            }
+        }
+        // dispatch: each idle slot takes the next chunk of the request with the least work left
+        for (Slot &sl : s->slots) {
+            if (sl.req)
+                continue;
+            Request *best = nullptr;
+            for (Request *r : s->active)
+                if (r->next_chunk < r->chunks.size() && (!best || r->words_left < best->words_left))
+                    best = r;
+            if (!best)
+                break;
+            if (!launch_chunk(s, sl, best, best->next_chunk)) {
+                s->fatal = "launch failed";
+                s->stop = true;
+                s->cv_done.notify_all();
+                return;
+            }
+            best->next_chunk++;
+            busy = true;
+        }
+        bool in_flight = false;
+        for (Slot &sl : s->slots)
+            in_flight |= sl.req != nullptr;
+        if (s->stop && !in_flight && s->ready.empty() && s->to_compile.empty() && s->active.empty())
+            return;
+        if (!busy) {
+            if (in_flight)
+                s->cv_sched.wait_for(lk, std::chrono::microseconds(50));
+            else
+                s->cv_sched.wait_for(lk, std::chrono::milliseconds(5));
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" int gpu_server_open(gpu_engine *e, const gpu_serve_tune *tune, gpu_server **out,
+                               char *err, size_t errlen)
+{
+    *out = nullptr;
+    std::unique_ptr<gpu_server> s(new gpu_server());
+    s->e = e;
+    if (tune)
+        s->tune = *tune;
+    if (!s->tune.streams)
+        s->tune.streams = 4;
+    if (!s->tune.compile_threads)
+        s->tune.compile_threads = 8;
+    if (!s->tune.memory_budget)
+        s->tune.memory_budget = 1ull << 30;
+    if (!s->tune.chunk_words)
+        s->tune.chunk_words = 1ull << 24;
+    if (!s->tune.max_active)
+        s->tune.max_active = 64;
+    if (hipSetDevice(e->device) != hipSuccess) {
+        put_err(err, errlen, "hipSetDevice failed");
+        return 0;
+    }
+    s->slots.resize(s->tune.streams);
+    std::vector<Slot *> all_slots;
+    for (Slot &sl : s->slots)
+        all_slots.push_back(&sl);
+    all_slots.push_back(&s->relist_slot);
+    for (Slot *slp : all_slots) {
+        Slot &sl = *slp;
+        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess ||
+            hipMalloc(&sl.d_out, 16) != hipSuccess ||
+            hipMalloc(&sl.d_list, 16 * kListCap) != hipSuccess ||
+            hipHostMalloc(&sl.h_out, 16, hipHostMallocDefault) != hipSuccess) {
+            put_err(err, errlen, "could not allocate the server's device slots");
+            gpu_server_close(s.release());
+            return 0;
+        }
+    }
+    gpu_server *raw = s.get();
+    for (unsigned k = 0; k < raw->tune.compile_threads; k++)
+        raw->compilers.emplace_back(compile_loop, raw);
+    raw->scheduler = std::thread(schedule_loop, raw);
+    *out = s.release();
+    return 1;
+}
+
+extern "C" int gpu_server_submit(gpu_server *s, const gpu_enum *en, uint64_t *ticket, char *err,
+                                 size_t errlen)
+{
+    std::unique_ptr<Request> r(new Request());
+    if (!ts_program_copy(&r->a, en->a) || (en->b && !ts_program_copy(&r->b, en->b))) {
+        put_err(err, errlen, "out of memory copying the request");
+        return 0;
+    }
+    r->has_b = en->b != nullptr;
+    r->en = *en;
+    r->en.a = &r->a;
+    r->en.b = r->has_b ? &r->b : nullptr;
+    if (en->spec) {
+        r->spec = en->spec;
+        r->en.spec = r->spec.c_str();
+    }
+    r->en.tune = nullptr;
+    r->len_total = r->a.len + (r->has_b && en->mode == TS_EN_EQUIV ? r->b.len : 0);
+    r->times.submitted_s = now_s();
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) {
+        put_err(err, errlen, "the server is closed: %s", s->fatal.c_str());
+        return 0;
+    }
+    r->id = s->next_id++;
+    *ticket = r->id;
+    s->to_compile.push_back(r.get());
+    s->all[r->id] = std::move(r);
+    s->cv_compile.notify_one();
+    return 1;
+}
+
+extern "C" int gpu_server_poll(gpu_server *s, uint64_t ticket, ts_enum_result *res, int *state,
+                               char *err, size_t errlen)
+{
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(ticket);
+    if (it == s->all.end()) {
+        put_err(err, errlen, "no request %llu", (unsigned long long)ticket);
+        return 0;
+    }
+    Request *r = it->second.get();
+    *state = r->state;
+    if (r->state == ST_DONE && res)
+        *res = r->result;
+    if (r->state == ST_FAILED)
+        put_err(err, errlen, "%s", r->error.c_str());
+    return 1;
+}
+
+extern "C" int gpu_server_wait(gpu_server *s, uint64_t ticket, ts_enum_result *res, char *err,
+                               size_t errlen)
+{
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->all.find(ticket);
+    if (it == s->all.end()) {
+        put_err(err, errlen, "no request %llu", (unsigned long long)ticket);
+        return 0;
+    }
+    Request *r = it->second.get();
+    s->cv_done.wait(lk, [&] { return r->state == ST_DONE || r->state == ST_FAILED || s->stop; });
+    if (r->state != ST_DONE) {
+        put_err(err, errlen, "%s", r->state == ST_FAILED ? r->error.c_str() : s->fatal.c_str());
+        return 0;
+    }
+    *res = r->result;
+    return 1;
+}
+
+extern "C" int gpu_server_times(gpu_server *s, uint64_t ticket, gpu_request_times *t)
+{
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(ticket);
+    if (it == s->all.end())
+        return 0;
+    *t = it->second->times;
+    return 1;
+}
+
+extern "C" void gpu_server_close(gpu_server *s)
+{
+    if (!s)
+        return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_compile.notify_all();
+    s->cv_sched.notify_all();
+    for (std::thread &t : s->compilers)
+        t.join();
+    if (s->scheduler.joinable())
+        s->scheduler.join();
+    (void)hipSetDevice(s->e->device);
+    s->slots.push_back(s->relist_slot);
+    for (Slot &sl : s->slots) {
+        if (sl.stream)
+            (void)hipStreamSynchronize(sl.stream);
+        if (sl.d_out)
+            (void)hipFree(sl.d_out);
+        if (sl.d_list)
+            (void)hipFree(sl.d_list);
+        if (sl.h_out)
+            (void)hipHostFree(sl.h_out);
+        if (sl.done)
+            (void)hipEventDestroy(sl.done);
+        if (sl.stream)
+            (void)hipStreamDestroy(sl.stream);
+    }
+    for (auto &kv : s->modules)
+        if (kv.second.mod)
+            (void)hipModuleUnload(kv.second.mod);
+    for (auto &kv : s->all) {
+        ts_program_free(&kv.second->a);
+        if (kv.second->has_b)
+            ts_program_free(&kv.second->b);
+    }
+    delete s;
+}

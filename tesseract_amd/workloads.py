@@ -122,9 +122,70 @@ def run_hash(engine, target=0x165C2A07):
             "ms": engine.last_ms, "ops_per_s": r["ops"] / (engine.last_ms * 1e-3)}
 
 
+def run_serve(engine, n, rate, seed, streams=4, chunk_words=1 << 24, check=True):
+    """A seeded Poisson stream of equiv requests (netlists.request) submitted at their arrival
+    times; latency = done - arrival. Offered load is the stream's; served load is what finished."""
+    import threading
+    from . import netlists as N
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    reqs = []
+    for k in range(n):
+        c, r, n_in, faulty = N.request(seed, k)
+        reqs.append((t.net_parse(c.bench()), t.net_parse(r.bench()), c, r, n_in, faulty))
+    srv = t.Server(engine, streams=streams, chunk_words=chunk_words)
+    tickets = [None] * n
+    t0 = time.perf_counter() + 0.05
+
+    def feeder():
+        for k in range(n):
+            delay = t0 + arrivals[k] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets[k] = srv.submit(t.EN_EQUIV, reqs[k][0], reqs[k][1])
+
+    th = threading.Thread(target=feeder)
+    th.start()
+    th.join()
+    results = [srv.wait(tk) for tk in tickets]
+    times = [srv.times(tk) for tk in tickets]
+    end = max(tm["done_s"] for tm in times)
+    first = min(tm["submitted_s"] for tm in times)
+    srv.close()
+    lat = np.array([tm["done_s"] - tm["submitted_s"] for tm in times])
+    n_in = np.array([q[4] for q in reqs])
+    out = {"workload": "serve", "requests": n, "offered_rate_per_s": rate,
+           "served_rate_per_s": n / (end - first), "wall_s": end - first,
+           "ops": int(sum(r["ops"] for r in results)),
+           "ops_per_s": sum(r["ops"] for r in results) / (end - first),
+           "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+           "compile_p50_ms": float(np.median([tm["compiled_s"] - tm["submitted_s"] for tm in times]) * 1e3),
+           "faulty": int(sum(q[5] for q in reqs)), "mismatching": int(sum(r["hits"] > 0 for r in results))}
+    classes = {}
+    for lo, hi in ((8, 16), (16, 24), (24, 30), (30, 37)):
+        m = (n_in >= lo) & (n_in < hi)
+        if m.any():
+            classes["2^%d-2^%d" % (lo, hi - 1)] = {
+                "n": int(m.sum()), "p50_ms": float(np.percentile(lat[m], 50) * 1e3),
+                "p99_ms": float(np.percentile(lat[m], 99) * 1e3)}
+    out["latency_by_input_space"] = classes
+    if check:
+        bad = 0
+        for (a, b, c, r, ni, faulty), res in zip(reqs, results):
+            if not faulty and res["hits"]:
+                bad += 1
+            for x in res["hit"]:
+                bad += c.evaluate(x) == r.evaluate(x)
+        out["verdicts_checked"] = bad == 0
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("workload", choices=["batch", "life", "hash", "benes"])
+    ap.add_argument("workload", choices=["batch", "life", "hash", "benes", "serve"])
+    ap.add_argument("--requests", type=int, default=200)
+    ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--streams", type=int, default=4)
     ap.add_argument("--programs", type=int, default=10000)
     ap.add_argument("--groups", type=int, default=2048)
     ap.add_argument("--passes", type=int, default=64)
@@ -141,6 +202,8 @@ def main():
                         a.waves_per_cu)
     elif a.workload == "life":
         out = run_life(eng, a.gens)
+    elif a.workload == "serve":
+        out = run_serve(eng, a.requests, a.rate, a.seed, a.streams)
     elif a.workload == "benes":
         out = run_benes(eng, a.cubes, a.seed)
     else:

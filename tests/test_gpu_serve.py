@@ -25,33 +25,45 @@ def small_request(seed, k, n_in, gates):
 
 
 def test_server_mix_equals_alone(engine):
+    # a large request runs first; small ones submitted while it runs must finish before it does
+    # (continuous batching, not a queue), and every result must equal the request run alone
+    rng = np.random.default_rng(11)
+    big_c = N.circuit(36, 8, 20000, rng)
+    big_r = N.rewrite(big_c, rng)
+    big = (t.net_parse(big_c.bench()), t.net_parse(big_r.bench()))
     reqs = []
     for k in range(18):
-        n_in = [10, 14, 22, 27][k % 4]
+        n_in = [10, 14, 18, 22][k % 4]
         gates = [60, 300, 1500][k % 3]
         c, r, faulty = small_request(7, k, n_in, gates)
         reqs.append((t.net_parse(c.bench()), t.net_parse(r.bench()), c, r, faulty))
     adder = t.gen_adder(8)
-    srv = t.Server(engine, streams=4, chunk_words=1 << 14)
+    srv = t.Server(engine, streams=4, chunk_words=1 << 20)
+    big_tk = srv.submit(t.EN_EQUIV, *big)
+    while srv.poll(big_tk)[0] < 3:  # running
+        time.sleep(0.01)
     tickets = []
     for k, (a, b, c, r, faulty) in enumerate(reqs):
         tickets.append(srv.submit(t.EN_EQUIV, a, b))
         if k == 5:
             tickets.append(srv.submit(t.EN_VERIFY, adder, spec="c:s = a + b"))
-        time.sleep(0.01)
     results = [srv.wait(tk) for tk in tickets]
+    big_res = srv.wait(big_tk)
     times = [srv.times(tk) for tk in tickets]
+    big_times = srv.times(big_tk)
     srv.close()
-    # requests overlapped on the device: one started before another finished
-    assert any(times[i]["started_s"] < times[j]["done_s"] and times[j]["started_s"] < times[i]["done_s"]
-               for i in range(len(times)) for j in range(i + 1, len(times)))
+    print("big request: %.2f s running" % (big_times["done_s"] - big_times["started_s"]), flush=True)
+    assert big_res["hits"] == 0 and big_res["tried"] == 1 << 36
+    # requests ready (compiled) well before the big one ends are not queued behind it
+    inside = [tm for tm in times if tm["compiled_s"] < big_times["done_s"] - 0.05]
+    assert len(inside) >= 3
+    assert all(tm["done_s"] <= big_times["done_s"] for tm in inside)
     adder_res = results.pop(6)
     assert adder_res["hits"] == 65536 and adder_res["hit"] == [2 * h + 1 for h in range(16)]
+    key = lambda r: (r["hits"], r["hit"], r["tried"], r["ops"], r["space"])
     for (a, b, c, r, faulty), got in zip(reqs, results):
         alone = engine.enum_run(t.EN_EQUIV, a, b)
-        assert got == {**alone, "seconds": got["seconds"]} or (
-            (got["hits"], got["hit"], got["tried"], got["ops"]) ==
-            (alone["hits"], alone["hit"], alone["tried"], alone["ops"]))
+        assert key(got) == key(alone)
         if not faulty:
             assert got["hits"] == 0
         for x in got["hit"]:

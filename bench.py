@@ -46,6 +46,20 @@ def cpu_baseline(threads, parts):
             "seconds_full_2p32_extrapolated": r["seconds"] * (1 << 32) / r["tried"]}
 
 
+def profiled_traffic():
+    """HBM bytes per ts_enum launch from the committed counter summary (FETCH_SIZE + WRITE_SIZE
+    runs, scaled by the copy-kernel factor), or None if it is absent."""
+    path = os.path.join(ROOT, "profiles", "ts_enum_mul16_verify_counters.txt")
+    try:
+        with open(path) as f:
+            for ln in f:
+                if ln.startswith("traffic_bytes_per_launch:"):
+                    return float(ln.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,6 +68,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-parts", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="afterwards copy a MIB-MiB device buffer (copy_probe): a kernel of known "
+                         "bytes whose FETCH_SIZE/WRITE_SIZE readings scale the kernel's")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each enumeration share "
@@ -110,9 +127,18 @@ def main():
                      "frac_of_measured": achieved / (valu_rate * 32)},
         "cpu_baseline": None,
     }
+    traffic = profiled_traffic()
+    if traffic is not None:
+        line["roofline"]["traffic"] = traffic
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_threads, args.cpu_parts)
     print(json.dumps(line), flush=True)
+    if args.copy_calibration:
+        n = args.copy_calibration << 20
+        rate = eng.hbm_rate(n)  # copy_probe: 4 launches, each reads n bytes and writes n bytes
+        print(json.dumps({"copy_probe_bytes_read_per_launch": n,
+                          "copy_probe_bytes_written_per_launch": n, "copy_bytes_per_s": rate}),
+              flush=True)
 
 
 if __name__ == "__main__":

@@ -78,6 +78,10 @@ int  gpu_run(gpu_code *c, gpu_state *s, uint64_t passes, char *err, size_t errle
 /* The device's measured rate of 32-bit v_bitop3_b32 lane-operations per second (a calibration
    launch that fills every SIMD), for stating results against the real integer-VALU roofline. */
 int  gpu_valu_rate(gpu_engine *e, double *lane_ops_per_s, char *err, size_t errlen);
+/* Bytes per second read + written by a plain device-to-device copy of `bytes` (a multiple of 16),
+   best of three launches. No counterpart in the reference: the HBM side of the engine's roofline,
+   and a copy of known size whose HBM counter readings scale those of the other kernels. */
+int  gpu_hbm_rate(gpu_engine *e, size_t bytes, double *bytes_per_s, char *err, size_t errlen);
 /* Milliseconds of device time the last gpu_run / gpu_enum_run took (HIP events on the engine's
    stream). */
 double gpu_last_ms(const gpu_engine *e);

@@ -149,6 +149,7 @@ def lib():
             "gpu_engine_describe": (C.c_char_p, [vp]),
             "gpu_last_ms": (C.c_double, [vp]),
             "gpu_valu_rate": (i32, [vp, C.POINTER(C.c_double), C.c_char_p, sz]),
+            "gpu_hbm_rate": (i32, [vp, sz, C.POINTER(C.c_double), C.c_char_p, sz]),
             "gpu_compile": (i32, [vp, P, C.POINTER(GpuTune), C.POINTER(vp), C.POINTER(C.c_char_p),
                                   C.c_char_p, sz]),
             "gpu_code_free": (None, [vp]),
@@ -522,6 +523,14 @@ class Engine:
         v = C.c_double()
         e = _err()
         if not lib().gpu_valu_rate(self._e, C.byref(v), e, len(e)):
+            raise RuntimeError(e.value.decode())
+        return v.value
+
+    def hbm_rate(self, nbytes=1 << 30):
+        """Measured bytes/s read + written by a plain copy of nbytes between device buffers."""
+        v = C.c_double()
+        e = _err()
+        if not lib().gpu_hbm_rate(self._e, nbytes, C.byref(v), e, len(e)):
             raise RuntimeError(e.value.decode())
         return v.value
 

@@ -1,6 +1,9 @@
-// Measured 32-bit integer VALU rate of the device: many waves, each lane running 16 independent
-// chains of v_bitop3_b32. The engine reports 1-bit ops against this rate as well as against the
-// published 256 CU x 4 SIMD x 16 lane x 2.4 GHz figure.
+// Measured rates of the device.
+// VALU: many waves, each lane running 16 independent chains of v_bitop3_b32; the engine reports
+// 1-bit ops against this rate as well as against the published 256 CU x 4 SIMD x 16 lane x 2.4 GHz
+// figure.
+// HBM: a plain 16-byte-per-lane copy between two device buffers; its byte count is known exactly,
+// so its FETCH_SIZE / WRITE_SIZE counter readings also scale those of the engine's kernels.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -24,6 +27,13 @@ __global__ void __launch_bounds__(256) valu_probe(uint32_t *out, uint32_t seed, 
     for (int k = 0; k < 16; k++)
         r ^= x[k];
     out[gid] = r;
+}
+
+__global__ void __launch_bounds__(256) copy_probe(const uint4 *__restrict__ src,
+                                                   uint4 *__restrict__ dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (size_t)gridDim.x * 256u)
+        dst[i] = src[i];
 }
 
 }  // namespace
@@ -52,5 +62,46 @@ extern "C" int ts_valu_calibrate(hipStream_t stream, int cus, double *lane_ops_p
     if (e != hipSuccess || ms <= 0)
         return 0;
     *lane_ops_per_s = (double)blocks * 256.0 * iters * 16.0 / (ms * 1e-3);
+    return 1;
+}
+
+// bytes per second read + written by copy_probe over `bytes` (a multiple of 16), best of 3 launches
+extern "C" int ts_hbm_calibrate(hipStream_t stream, int cus, size_t bytes, double *bytes_per_s)
+{
+    if (bytes < 16 || bytes % 16)
+        return 0;
+    uint4 *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, bytes) != hipSuccess)
+        return 0;
+    if (hipMalloc(&b, bytes) != hipSuccess) {
+        hipFree(a);
+        return 0;
+    }
+    const size_t n = bytes / 16;
+    const unsigned blocks = (unsigned)cus * 8u;
+    hipEvent_t ev[2];
+    hipEventCreate(&ev[0]);
+    hipEventCreate(&ev[1]);
+    hipMemsetAsync(a, 0x5A, bytes, stream);
+    copy_probe<<<blocks, 256, 0, stream>>>(a, b, n);  // warm-up
+    float best = 0;
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 3 && e == hipSuccess; k++) {
+        hipEventRecord(ev[0], stream);
+        copy_probe<<<blocks, 256, 0, stream>>>(k & 1 ? b : a, k & 1 ? a : b, n);
+        hipEventRecord(ev[1], stream);
+        e = hipEventSynchronize(ev[1]);
+        float ms = 0;
+        hipEventElapsedTime(&ms, ev[0], ev[1]);
+        if (ms > 0 && (best == 0 || ms < best))
+            best = ms;
+    }
+    hipEventDestroy(ev[0]);
+    hipEventDestroy(ev[1]);
+    hipFree(a);
+    hipFree(b);
+    if (e != hipSuccess || best <= 0)
+        return 0;
+    *bytes_per_s = 2.0 * (double)bytes / (best * 1e-3);
     return 1;
 }

@@ -215,6 +215,20 @@ extern "C" int gpu_valu_rate(gpu_engine *e, double *lane_ops_per_s, char *err, s
     }
     return 1;
 }
+
+extern "C" int ts_hbm_calibrate(hipStream_t stream, int cus, size_t bytes, double *bytes_per_s);
+
+extern "C" int gpu_hbm_rate(gpu_engine *e, size_t bytes, double *bytes_per_s, char *err,
+                            size_t errlen)
+{
+    HIPCHK(hipSetDevice(e->device));
+    if (!ts_hbm_calibrate(e->stream, e->cus, bytes, bytes_per_s)) {
+        say(err, errlen, "the HBM copy calibration failed (bytes must be a nonzero multiple of 16 "
+                         "and two buffers of that size must fit)");
+        return 0;
+    }
+    return 1;
+}
 extern "C" double gpu_last_ms(const gpu_engine *e) { return e->last_ms; }
 
 // ---- compile --------------------------------------------------------------------------------

@@ -31,7 +31,7 @@ const char *gpu_engine_describe(const gpu_engine *e);
 /* Tuning knobs of one compiled program; zero fields take the defaults. */
 typedef struct {
     unsigned block;        /* threads per workgroup (multiple of 64) */
-    unsigned slice_nodes;  /* AIG nodes per slice for segmented programs (0 = default) */
+    unsigned slice_nodes;  /* AIG nodes per slice for segmented programs (0 = by cube size) */
     unsigned lut3;         /* 1 = map to v_bitop3_b32 (default), 2 = plain AND/NOT */
     unsigned grid;         /* workgroups for enumeration kernels (0 = default) */
     unsigned graph_passes; /* segmented programs: passes per captured HIP graph (default 32,

@@ -38,28 +38,63 @@ __global__ void __launch_bounds__(256) copy_probe(const uint4 *__restrict__ src,
 
 }  // namespace
 
+namespace {
+
+// device buffers and a pair of events, released on every exit path; `ok` collects the HIP status
+// of every call made through it
+struct Probe {
+    void *buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool ok = true;
+
+    bool check(hipError_t e)
+    {
+        ok = ok && e == hipSuccess;
+        return ok;
+    }
+    bool setup(size_t bytes0, size_t bytes1)
+    {
+        return check(hipMalloc(&buf[0], bytes0)) && (!bytes1 || check(hipMalloc(&buf[1], bytes1))) &&
+               check(hipEventCreate(&ev[0])) && check(hipEventCreate(&ev[1]));
+    }
+    // milliseconds between the two events once the second has completed (0 on failure)
+    float elapsed()
+    {
+        float ms = 0;
+        if (check(hipEventSynchronize(ev[1])) && check(hipEventElapsedTime(&ms, ev[0], ev[1])))
+            return ms;
+        return 0;
+    }
+    ~Probe()
+    {
+        for (hipEvent_t e : ev)
+            if (e)
+                (void)hipEventDestroy(e);
+        for (void *b : buf)
+            if (b)
+                (void)hipFree(b);
+    }
+};
+
+}  // namespace
+
 // lane-ops per second of v_bitop3_b32 over a launch that fills every SIMD several times
 extern "C" int ts_valu_calibrate(hipStream_t stream, int cus, double *lane_ops_per_s)
 {
     const unsigned blocks = (unsigned)cus * 16u;
     const int iters = 8192;
-    uint32_t *out = nullptr;
-    if (hipMalloc(&out, (size_t)blocks * 256 * sizeof(uint32_t)) != hipSuccess)
+    Probe p;
+    if (!p.setup((size_t)blocks * 256 * sizeof(uint32_t), 0))
         return 0;
-    hipEvent_t a, b;
-    hipEventCreate(&a);
-    hipEventCreate(&b);
+    uint32_t *out = (uint32_t *)p.buf[0];
     valu_probe<<<blocks, 256, 0, stream>>>(out, 1u, iters);  // warm-up
-    hipEventRecord(a, stream);
+    p.check(hipGetLastError());
+    p.check(hipEventRecord(p.ev[0], stream));
     valu_probe<<<blocks, 256, 0, stream>>>(out, 2u, iters);
-    hipEventRecord(b, stream);
-    hipError_t e = hipEventSynchronize(b);
-    float ms = 0;
-    hipEventElapsedTime(&ms, a, b);
-    hipEventDestroy(a);
-    hipEventDestroy(b);
-    hipFree(out);
-    if (e != hipSuccess || ms <= 0)
+    p.check(hipGetLastError());
+    p.check(hipEventRecord(p.ev[1], stream));
+    const float ms = p.ok ? p.elapsed() : 0;
+    if (!p.ok || ms <= 0)
         return 0;
     *lane_ops_per_s = (double)blocks * 256.0 * iters * 16.0 / (ms * 1e-3);
     return 1;
@@ -70,37 +105,26 @@ extern "C" int ts_hbm_calibrate(hipStream_t stream, int cus, size_t bytes, doubl
 {
     if (bytes < 16 || bytes % 16)
         return 0;
-    uint4 *a = nullptr, *b = nullptr;
-    if (hipMalloc(&a, bytes) != hipSuccess)
+    Probe p;
+    if (!p.setup(bytes, bytes))
         return 0;
-    if (hipMalloc(&b, bytes) != hipSuccess) {
-        hipFree(a);
-        return 0;
-    }
+    uint4 *a = (uint4 *)p.buf[0], *b = (uint4 *)p.buf[1];
     const size_t n = bytes / 16;
     const unsigned blocks = (unsigned)cus * 8u;
-    hipEvent_t ev[2];
-    hipEventCreate(&ev[0]);
-    hipEventCreate(&ev[1]);
-    hipMemsetAsync(a, 0x5A, bytes, stream);
+    p.check(hipMemsetAsync(a, 0x5A, bytes, stream));
     copy_probe<<<blocks, 256, 0, stream>>>(a, b, n);  // warm-up
+    p.check(hipGetLastError());
     float best = 0;
-    hipError_t e = hipSuccess;
-    for (int k = 0; k < 3 && e == hipSuccess; k++) {
-        hipEventRecord(ev[0], stream);
+    for (int k = 0; k < 3 && p.ok; k++) {
+        p.check(hipEventRecord(p.ev[0], stream));
         copy_probe<<<blocks, 256, 0, stream>>>(k & 1 ? b : a, k & 1 ? a : b, n);
-        hipEventRecord(ev[1], stream);
-        e = hipEventSynchronize(ev[1]);
-        float ms = 0;
-        hipEventElapsedTime(&ms, ev[0], ev[1]);
+        p.check(hipGetLastError());
+        p.check(hipEventRecord(p.ev[1], stream));
+        const float ms = p.ok ? p.elapsed() : 0;
         if (ms > 0 && (best == 0 || ms < best))
             best = ms;
     }
-    hipEventDestroy(ev[0]);
-    hipEventDestroy(ev[1]);
-    hipFree(a);
-    hipFree(b);
-    if (e != hipSuccess || best <= 0)
+    if (!p.ok || best <= 0)
         return 0;
     *bytes_per_s = 2.0 * (double)bytes / (best * 1e-3);
     return 1;

@@ -74,8 +74,6 @@ gpu_tune defaults(const gpu_engine *e, const gpu_tune *t)
         d = *t;
     if (!d.block || d.block % 64 || d.block > 1024)
         d.block = 256;
-    if (!d.slice_nodes)
-        d.slice_nodes = 1500;
     if (!d.lut3)
         d.lut3 = 1;
     if (!d.grid)
@@ -192,13 +190,13 @@ extern "C" void gpu_engine_close(gpu_engine *e)
 {
     if (!e)
         return;
-    hipSetDevice(e->device);
+    (void)hipSetDevice(e->device);
     if (e->ev0)
-        hipEventDestroy(e->ev0);
+        (void)hipEventDestroy(e->ev0);
     if (e->ev1)
-        hipEventDestroy(e->ev1);
+        (void)hipEventDestroy(e->ev1);
     if (e->stream)
-        hipStreamDestroy(e->stream);
+        (void)hipStreamDestroy(e->stream);
     delete e;
 }
 
@@ -341,8 +339,8 @@ extern "C" void gpu_code_free(gpu_code *c)
     if (!c)
         return;
     if (c->mod) {
-        hipSetDevice(c->e->device);
-        hipModuleUnload(c->mod);
+        (void)hipSetDevice(c->e->device);
+        (void)hipModuleUnload(c->mod);
     }
     delete c;
 }
@@ -384,12 +382,12 @@ extern "C" void gpu_state_free(gpu_state *s)
 {
     if (!s)
         return;
-    hipSetDevice(s->c->e->device);
+    (void)hipSetDevice(s->c->e->device);
     for (int k = 0; k < 2; k++)
         if (s->graph[k])
             (void)hipGraphExecDestroy(s->graph[k]);
-    hipFree(s->slots);
-    hipFree(s->init);
+    (void)hipFree(s->slots);
+    (void)hipFree(s->init);
     delete s;
 }
 
@@ -716,7 +714,7 @@ extern "C" int gpu_enum_compile(gpu_engine *e, const gpu_enum *en, gpu_enum_code
         return 0;
     if (!build_module(e, ks.text, &c->mod, &c->compile_seconds, err, errlen))
         return 0;
-    auto fail = [&](void) { hipModuleUnload(c->mod); c->mod = nullptr; };
+    auto fail = [&](void) { (void)hipModuleUnload(c->mod); c->mod = nullptr; };
     if (hipModuleGetFunction(&c->fn, c->mod, "ts_enum") != hipSuccess) {
         fail();
         say(err, errlen, "hipModuleGetFunction failed");
@@ -730,7 +728,7 @@ extern "C" int gpu_enum_compile(gpu_engine *e, const gpu_enum *en, gpu_enum_code
     c->len_total = en->a->len + (en->mode == TS_EN_EQUIV ? en->b->len : 0);
     if (hipMalloc(&c->d_out, 2 * sizeof(uint64_t)) != hipSuccess ||
         hipMalloc(&c->d_list, 2 * kListCap * sizeof(uint64_t)) != hipSuccess) {
-        hipFree(c->d_out);
+        (void)hipFree(c->d_out);
         fail();
         say(err, errlen, "hipMalloc failed for the hit list");
         return 0;
@@ -743,13 +741,13 @@ extern "C" void gpu_enum_free(gpu_enum_code *c)
 {
     if (!c)
         return;
-    hipSetDevice(c->e->device);
+    (void)hipSetDevice(c->e->device);
     if (c->d_out)
-        hipFree(c->d_out);
+        (void)hipFree(c->d_out);
     if (c->d_list)
-        hipFree(c->d_list);
+        (void)hipFree(c->d_list);
     if (c->mod)
-        hipModuleUnload(c->mod);
+        (void)hipModuleUnload(c->mod);
     delete c;
 }
 

@@ -102,7 +102,7 @@ struct Lowered {
 
 struct LowerOptions {
     unsigned cube_words_log2;   // log2(words per cube) = dims - 5
-    unsigned slice_nodes;       // target AIG nodes per slice (0 = no slicing)
+    unsigned slice_nodes;       // target AIG nodes per slice (0 = chosen per segment)
 };
 
 // Lowers one pass of p into segments. Returns false with a message for programs it cannot take.

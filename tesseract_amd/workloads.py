@@ -76,7 +76,7 @@ def run_life(engine, gens, check=True, tune=None):
     life = LifeCube(engine, tune=tune)  # 8192 x 8192 torus on 65,536 machines, 32 x 32 tiles
     setup = time.perf_counter() - t0
     info = life.code.stats()
-    life.run(1)  # warm-up generation
+    life.run(64)  # warm-up: builds the captured graphs of both parities
     ms = life.run(gens)
     out = {"workload": "life", "board": "8192x8192", "machines": 65536, "generations": gens,
            "instructions_per_generation": life.prog.len, "segments": info["segments"],
@@ -87,7 +87,7 @@ def run_life(engine, gens, check=True, tune=None):
     if check:
         from oracle import pyoracle as O  # the checker only: the run above is finished
         start = life.start_bits()
-        want = O.life_words(start, gens + 1)
+        want = O.life_words(start, gens + 64)
         out["bit_exact_vs_plain_c"] = bool(np.array_equal(life.board_bits(), want))
     return out
 

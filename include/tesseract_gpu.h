@@ -36,6 +36,8 @@ typedef struct {
     unsigned grid;         /* workgroups for enumeration kernels (0 = default) */
     unsigned graph_passes; /* segmented programs: passes per captured HIP graph (default 32,
                               1 = launch every segment directly) */
+    unsigned tape;         /* enumerations: 1 = run the precompiled tape interpreter instead of
+                              compiling a kernel (equiv and search with one tick; no hipRTC) */
 } gpu_tune;
 
 typedef struct {
@@ -157,7 +159,9 @@ int  gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t n, const g
    chunk_words words; `streams` device slots each run one chunk at a time, taking the next chunk of
    the admitted request with the least work left, so requests that arrive while others run start
    within a chunk and their work overlaps on the device. Each result is exactly what gpu_enum_run
-   returns for the request alone. Compiled modules stay resident for repeated circuits. */
+   returns for the request alone. Compiled modules stay resident for repeated circuits. Small
+   equiv/search requests skip the compile: their tape (gpu_enum_tape) runs on the precompiled
+   interpreter kernel. */
 typedef struct gpu_server gpu_server;
 typedef struct {
     unsigned streams;          /* device slots (default 4) */
@@ -165,10 +169,14 @@ typedef struct {
     uint64_t memory_budget;    /* bytes of resident compiled code (default 1 GiB) */
     uint64_t chunk_words;      /* words (32 assignments each) per launch (default 2^24) */
     unsigned max_active;       /* admitted requests at once (default 64) */
+    double   tape_max_ms;      /* equiv/search requests whose estimated device time on the tape
+                                  interpreter is at most this run there with no compile
+                                  (default 0 = 50 ms; negative = always compile) */
 } gpu_serve_tune;
 typedef struct {
     double   submitted_s, compiled_s, admitted_s, started_s, done_s;  /* steady clock, seconds */
     uint64_t launches;
+    uint64_t tier;             /* 0 = compiled kernel, 1 = tape interpreter */
 } gpu_request_times;
 /* state: 0 queued, 1 compiling, 2 waiting for admission, 3 running, 4 done, 5 failed */
 int  gpu_server_open(gpu_engine *e, const gpu_serve_tune *tune, gpu_server **out, char *err,
@@ -187,6 +195,19 @@ void gpu_server_close(gpu_server *s);
 int  gpu_source(const ts_program *p, const gpu_tune *tune, char **source, gpu_code_info *info,
                 char *err, size_t errlen);
 int  gpu_enum_source(const gpu_enum *en, char **source, char *err, size_t errlen);
+/* The tape the interpreter tier (gpu_tune.tape) runs for an equiv or search enumeration, built on
+   the host: groups of 16 entries of independent AND nodes, 3 words per entry (a | ca << 31,
+   b | cb << 31, d: byte offsets of slots of 256 bytes, c = complemented), `count` real nodes per
+   group (the rest read slot 0 and write the last slot), input leaves (slot offset | input bit
+   j << 24), the hit word's slot (| complemented << 31). Free with gpu_tape_free. max_slots 0 = 640
+   (160 KB of LDS). */
+typedef struct {
+    uint32_t *ops, *count, *leaves;
+    uint32_t  ngroups, nleaves, root, slots;
+    uint64_t  and_nodes;
+} gpu_tape;
+int  gpu_enum_tape(const gpu_enum *en, unsigned max_slots, gpu_tape *out, char *err, size_t errlen);
+void gpu_tape_free(gpu_tape *t);
 int  gpu_rtc_check(const char *source, const char *arch, double *seconds, size_t *code_bytes,
                    char *err, size_t errlen);
 

@@ -47,7 +47,7 @@ class EnumResultStruct(C.Structure):
 
 class GpuTune(C.Structure):
     _fields_ = [("block", C.c_uint), ("slice_nodes", C.c_uint), ("lut3", C.c_uint), ("grid", C.c_uint),
-                ("graph_passes", C.c_uint)]
+                ("graph_passes", C.c_uint), ("tape", C.c_uint)]
 
 
 class GpuCodeInfo(C.Structure):
@@ -76,12 +76,19 @@ class GpuBatchInfo(C.Structure):
 
 class GpuServeTune(C.Structure):
     _fields_ = [("streams", C.c_uint), ("compile_threads", C.c_uint), ("memory_budget", C.c_uint64),
-                ("chunk_words", C.c_uint64), ("max_active", C.c_uint)]
+                ("chunk_words", C.c_uint64), ("max_active", C.c_uint), ("tape_max_ms", C.c_double)]
 
 
 class GpuRequestTimes(C.Structure):
     _fields_ = [("submitted_s", C.c_double), ("compiled_s", C.c_double), ("admitted_s", C.c_double),
-                ("started_s", C.c_double), ("done_s", C.c_double), ("launches", C.c_uint64)]
+                ("started_s", C.c_double), ("done_s", C.c_double), ("launches", C.c_uint64),
+                ("tier", C.c_uint64)]
+
+
+class GpuTapeStruct(C.Structure):
+    _fields_ = [("ops", C.POINTER(C.c_uint32)), ("count", C.POINTER(C.c_uint32)),
+                ("leaves", C.POINTER(C.c_uint32)), ("ngroups", C.c_uint32), ("nleaves", C.c_uint32),
+                ("root", C.c_uint32), ("slots", C.c_uint32), ("and_nodes", C.c_uint64)]
 
 
 class GpuEnumStruct(C.Structure):
@@ -165,6 +172,9 @@ def lib():
             "gpu_source": (i32, [P, C.POINTER(GpuTune), C.POINTER(vp), C.POINTER(GpuCodeInfo),
                                  C.c_char_p, sz]),
             "gpu_enum_source": (i32, [C.POINTER(GpuEnumStruct), C.POINTER(vp), C.c_char_p, sz]),
+            "gpu_enum_tape": (i32, [C.POINTER(GpuEnumStruct), C.c_uint, C.POINTER(GpuTapeStruct),
+                                    C.c_char_p, sz]),
+            "gpu_tape_free": (None, [C.POINTER(GpuTapeStruct)]),
             "gpu_rtc_check": (i32, [C.c_char_p, C.c_char_p, C.POINTER(C.c_double),
                                     C.POINTER(sz), C.c_char_p, sz]),
             "gpu_enum_compile": (i32, [vp, C.POINTER(GpuEnumStruct), C.POINTER(vp), C.c_char_p, sz]),
@@ -469,6 +479,28 @@ def enum_source(mode, a, b=None, spec=None, ticks=1, part=0, part_end=0, parts=1
     return _take_string(out)
 
 
+def enum_tape(mode, a, b=None, spec=None, max_slots=0):
+    """The interpreter tier's tape of an equiv/search enumeration (no device): a dict of numpy
+    arrays ops (groups x 16 x 3), count, leaves and root, slots, and_nodes."""
+    import numpy as np
+    en = _enum_struct(mode, a, b, spec, 1, 0, 0, 1, None)
+    t = GpuTapeStruct()
+    e = _err()
+    if not lib().gpu_enum_tape(C.byref(en), max_slots, C.byref(t), e, len(e)):
+        raise ValueError(e.value.decode())
+    try:
+        g = t.ngroups
+        out = {"ops": np.ctypeslib.as_array(t.ops, (g * 48,)).reshape(g, 16, 3).copy() if g else
+               np.zeros((0, 16, 3), np.uint32),
+               "count": np.ctypeslib.as_array(t.count, (g,)).copy() if g else np.zeros(0, np.uint32),
+               "leaves": np.ctypeslib.as_array(t.leaves, (t.nleaves,)).copy() if t.nleaves else
+               np.zeros(0, np.uint32),
+               "root": t.root, "slots": t.slots, "and_nodes": t.and_nodes}
+    finally:
+        lib().gpu_tape_free(C.byref(t))
+    return out
+
+
 def rtc_check(text, arch="gfx950"):
     """hipRTC-compiles HIP source for arch (no device needed); returns (seconds, code bytes)."""
     secs = C.c_double()
@@ -599,9 +631,10 @@ class Server:
     """gpu_server: continuous-batching request loop on one engine."""
 
     def __init__(self, engine, streams=0, compile_threads=0, memory_budget=0, chunk_words=0,
-                 max_active=0):
+                 max_active=0, tape_max_ms=0.0):
         self.engine = engine
-        tune = GpuServeTune(streams, compile_threads, memory_budget, chunk_words, max_active)
+        tune = GpuServeTune(streams, compile_threads, memory_budget, chunk_words, max_active,
+                            tape_max_ms)
         h = C.c_void_p()
         e = _err()
         if not lib().gpu_server_open(engine._e, C.byref(tune), C.byref(h), e, len(e)):

@@ -644,6 +644,42 @@ void emit_transpose(std::ostringstream &o, const char *e)
 
 }  // namespace
 
+void enum_outputs(const ts_program *a, const ts_program *b, const int *in_map, const int *out_map,
+                  const EnumLowering &el, Aig &g, std::vector<Lit> &oa, std::vector<Lit> &ob,
+                  uint32_t *a_end, uint32_t *b_end)
+{
+    const unsigned n = el.n;
+    auto run = [&](const ts_program *p, const std::vector<uint32_t> &in_cells) {
+        std::vector<Lit> cur(p->ncells + 1, 0);
+        for (unsigned k = 0; k < n; k++)
+            cur[in_cells[k]] = g.InBit(n - 1 - k);  // input k = bit n-1-k of x (src/enum.c:113-133)
+        cur[TS_CELL_ZERO] = 0;
+        cur[TS_CELL_ONE] = 1;
+        execute_linkfree(p, g, cur, nullptr);
+        return cur;
+    };
+    oa.clear();
+    ob.clear();
+    std::vector<Lit> ra = run(a, std::vector<uint32_t>(a->inputs, a->inputs + n));
+    if (a_end)
+        *a_end = (uint32_t)g.nodes.size();
+    if (el.mode == TS_EN_EQUIV) {
+        std::vector<uint32_t> b_in;
+        for (unsigned k = 0; k < n; k++)
+            b_in.push_back(b->inputs[in_map[k]]);
+        std::vector<Lit> rb = run(b, b_in);
+        for (size_t o = 0; o < a->noutputs; o++) {
+            oa.push_back(ra[a->outputs[o]]);
+            ob.push_back(rb[b->outputs[out_map[o]]]);
+        }
+    } else {
+        for (uint32_t c : el.out_cells_a)
+            oa.push_back(ra[c]);
+    }
+    if (b_end)
+        *b_end = (uint32_t)g.nodes.size();
+}
+
 bool codegen_enum(const ts_program *a, const ts_program *b, const int *in_map, const int *out_map,
                   const EnumLowering &el, const CodegenOptions &opt, KernelSource &ks,
                   std::string &err, const ts_spec *spec)
@@ -689,38 +725,37 @@ bool codegen_enum(const ts_program *a, const ts_program *b, const int *in_map, c
         return "(0u - (u32)((gw >> " + std::to_string(j - 5) + ") & 1ull))";
     };
 
-    std::vector<Lit> ra(a->ncells + 1, 0), rb;
+    std::vector<Lit> ra, rb;
     std::vector<uint8_t> wa(a->ncells + 1, 0), wb;
-    auto init_cells = [&](const ts_program *p, std::vector<Lit> &cur, const uint32_t *in_cells,
-                          bool symbolic, uint32_t base) {
-        cur.assign(p->ncells + 1, 0);
-        for (uint32_t c = 0; c <= p->ncells; c++)
-            cur[c] = symbolic ? g.Load(base + c, 0) : 0;
-        if (!symbolic)
-            for (unsigned k = 0; k < n; k++)
-                cur[in_cells[k]] = g.InBit(n - 1 - k);
-        cur[TS_CELL_ZERO] = 0;
-        cur[TS_CELL_ONE] = 1;
-    };
     std::vector<uint32_t> b_in;
     if (el.mode == TS_EN_EQUIV)
         for (unsigned k = 0; k < n; k++)
             b_in.push_back(b->inputs[in_map[k]]);
-    init_cells(a, ra, a->inputs, loop, 0);
-    execute_linkfree(a, g, ra, &wa);
-    if (el.mode == TS_EN_EQUIV) {
-        wb.assign(b->ncells + 1, 0);
-        init_cells(b, rb, b_in.data(), loop, a->ncells + 1);
-        execute_linkfree(b, g, rb, &wb);
+    std::vector<Lit> out_a, out_b;  // ticks == 1: the compared outputs
+    if (!loop) {
+        enum_outputs(a, b, in_map, out_map, el, g, out_a, out_b, nullptr, nullptr);
+    } else {
+        auto init_cells = [&](const ts_program *p, std::vector<Lit> &cur, uint32_t base) {
+            cur.assign(p->ncells + 1, 0);
+            for (uint32_t c = 0; c <= p->ncells; c++)
+                cur[c] = g.Load(base + c, 0);
+            cur[TS_CELL_ZERO] = 0;
+            cur[TS_CELL_ONE] = 1;
+        };
+        init_cells(a, ra, 0);
+        execute_linkfree(a, g, ra, &wa);
+        if (el.mode == TS_EN_EQUIV) {
+            wb.assign(b->ncells + 1, 0);
+            init_cells(b, rb, a->ncells + 1);
+            execute_linkfree(b, g, rb, &wb);
+        }
     }
 
     // roots: ticks == 1 the compared outputs; ticks > 1 every written cell (state update)
     std::vector<Lit> roots;
     if (!loop) {
-        for (uint32_t c : cells_a)
-            roots.push_back(ra[c]);
-        for (uint32_t c : cells_b)
-            roots.push_back(rb[c]);
+        roots = out_a;
+        roots.insert(roots.end(), out_b.begin(), out_b.end());
     } else {
         for (uint32_t c = 0; c <= a->ncells; c++)
             if (wa[c])
@@ -744,10 +779,10 @@ bool codegen_enum(const ts_program *a, const ts_program *b, const int *in_map, c
         Emitter e(g, m, o, "v");
         e.emit(leaf);
         ks.bitop3 += e.luts;
-        for (uint32_t c : cells_a)
-            oa.push_back(e.val(ra[c]));
-        for (uint32_t c : cells_b)
-            ob.push_back(e.val(rb[c]));
+        for (Lit v : out_a)
+            oa.push_back(e.val(v));
+        for (Lit v : out_b)
+            ob.push_back(e.val(v));
     } else {
         // state s<c> (a) and s<ncells_a + 1 + c> (b): power-on with the inputs set
         auto init_state = [&](const ts_program *p, const uint32_t *in_cells, uint32_t base) {

@@ -171,6 +171,9 @@ extern "C" int gpu_engine_open(int device, gpu_engine **out, char *err, size_t e
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device));
     e->cus = prop.multiProcessorCount;
+    int lds = 0;
+    HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    e->lds_per_block = (size_t)lds;
     e->arch = prop.gcnArchName;
     size_t colon = e->arch.find(':');
     if (colon != std::string::npos)
@@ -622,8 +625,18 @@ namespace {
 }  // namespace
 
 // validation (the reference's messages, src/enum.c:244-268), spec parsing and code generation
-int ts_internal::enum_prepare(const gpu_engine *e, const gpu_enum *en, gpu_tune &tune,
-                              KernelSource &ks, char *err, size_t errlen)
+namespace {
+
+// Validation with the reference's messages (src/enum.c:244-262), name matching for equiv and spec
+// parsing: everything both tiers need before they build code or a tape.
+struct EnumPlan {
+    ts::EnumLowering el;
+    std::vector<int> in_map, out_map;
+    ts_spec         *spec = nullptr;
+    ~EnumPlan() { ts_spec_free(spec); }
+};
+
+int enum_plan(const gpu_enum *en, EnumPlan &plan, char *err, size_t errlen)
 {
     const ts_program *a = en->a, *b = en->b;
     if (a->special || (en->mode == TS_EN_EQUIV && b && b->special)) {
@@ -645,33 +658,59 @@ int ts_internal::enum_prepare(const gpu_engine *e, const gpu_enum *en, gpu_tune 
         say(err, errlen, "equiv needs two programs");
         return 0;
     }
-    std::vector<int> in_map(n + 1), out_map(a->noutputs + 1);
+    plan.in_map.assign(n + 1, 0);
+    plan.out_map.assign(a->noutputs + 1, 0);
     if (en->mode == TS_EN_EQUIV &&
-        !ts_enum_match(a, b, in_map.data(), out_map.data(), err, errlen))
+        !ts_enum_match(a, b, plan.in_map.data(), plan.out_map.data(), err, errlen))
         return 0;
-    ts_spec *spec = nullptr;
-    EnumLowering el;
+    ts::EnumLowering &el = plan.el;
     el.mode = en->mode;
     el.n = (unsigned)n;
     el.ticks = en->ticks;
     if (en->mode != TS_EN_EQUIV) {
-        spec = ts_spec_parse(en->spec ? en->spec : "", a, en->mode == TS_EN_VERIFY, err, errlen);
-        if (!spec)
+        plan.spec = ts_spec_parse(en->spec ? en->spec : "", a, en->mode == TS_EN_VERIFY, err, errlen);
+        if (!plan.spec)
             return 0;
-        el.width = ts_spec_width(spec);
+        el.width = ts_spec_width(plan.spec);
         for (int bit = 0; bit < el.width; bit++)
-            el.out_cells_a.push_back(ts_spec_out_cell(spec, bit));
+            el.out_cells_a.push_back(ts_spec_out_cell(plan.spec, bit));
         if (en->mode == TS_EN_SEARCH)
-            el.target = ts_spec_value(spec, 0);
+            el.target = ts_spec_value(plan.spec, 0);
     }
+    return 1;
+}
+
+}  // namespace
+
+int ts_internal::enum_prepare(const gpu_engine *e, const gpu_enum *en, gpu_tune &tune,
+                              KernelSource &ks, char *err, size_t errlen)
+{
+    EnumPlan plan;
+    if (!enum_plan(en, plan, err, errlen))
+        return 0;
     tune = defaults(e, en->tune);
     CodegenOptions co;
     co.block = tune.block;
     co.lut3 = tune.lut3 == 1;
     std::string msg;
-    bool ok = codegen_enum(a, b, in_map.data(), out_map.data(), el, co, ks, msg, spec);
-    ts_spec_free(spec);
-    if (!ok) {
+    if (!codegen_enum(en->a, en->b, plan.in_map.data(), plan.out_map.data(), plan.el, co, ks, msg,
+                      plan.spec)) {
+        say(err, errlen, "%s", msg.c_str());
+        return 0;
+    }
+    return 1;
+}
+
+int ts_internal::enum_tape(const gpu_engine *e, const gpu_enum *en, gpu_tune &tune,
+                           unsigned max_slots, Tape &t, char *err, size_t errlen)
+{
+    EnumPlan plan;
+    if (!enum_plan(en, plan, err, errlen))
+        return 0;
+    tune = defaults(e, en->tune);
+    std::string msg;
+    if (!build_tape(en->a, en->b, plan.in_map.data(), plan.out_map.data(), plan.el, max_slots, t,
+                    msg)) {
         say(err, errlen, "%s", msg.c_str());
         return 0;
     }
@@ -690,10 +729,53 @@ extern "C" int gpu_enum_source(const gpu_enum *en, char **source, char *err, siz
     return 1;
 }
 
+extern "C" int gpu_enum_tape(const gpu_enum *en, unsigned max_slots, gpu_tape *out, char *err,
+                             size_t errlen)
+{
+    memset(out, 0, sizeof *out);
+    gpu_engine none;
+    none.cus = 256;
+    gpu_tune tune;
+    Tape t;
+    if (!ts_internal::enum_tape(&none, en, tune, max_slots ? max_slots : 640, t, err, errlen))
+        return 0;
+    auto dup = [](const std::vector<uint32_t> &v) {
+        uint32_t *p = (uint32_t *)malloc(std::max<size_t>(1, v.size()) * sizeof(uint32_t));
+        if (p && !v.empty())
+            memcpy(p, v.data(), v.size() * sizeof(uint32_t));
+        return p;
+    };
+    out->ops = dup(t.ops);
+    out->count = dup(t.count);
+    out->leaves = dup(t.leaves);
+    if (!out->ops || !out->count || !out->leaves) {
+        gpu_tape_free(out);
+        say(err, errlen, "out of memory");
+        return 0;
+    }
+    out->ngroups = (uint32_t)t.count.size();
+    out->nleaves = (uint32_t)t.leaves.size();
+    out->root = t.root;
+    out->slots = t.slots;
+    out->and_nodes = t.and_nodes;
+    return 1;
+}
+
+extern "C" void gpu_tape_free(gpu_tape *t)
+{
+    free(t->ops);
+    free(t->count);
+    free(t->leaves);
+    memset(t, 0, sizeof *t);
+}
+
 struct gpu_enum_code {
     gpu_engine   *e = nullptr;
     hipModule_t   mod = nullptr;
     hipFunction_t fn = nullptr;
+    bool          tape = false;  // the interpreter kernel runs `td` instead of a compiled kernel
+    ts_internal::TapeDev td{};
+    unsigned      tape_blocks_per_cu = 1;
     gpu_tune      tune{};
     int           part = 0, part_end = 0, parts = 1;
     unsigned      n = 0, ticks = 1;
@@ -709,16 +791,39 @@ extern "C" int gpu_enum_compile(gpu_engine *e, const gpu_enum *en, gpu_enum_code
     *out = nullptr;
     std::unique_ptr<gpu_enum_code> c(new gpu_enum_code());
     c->e = e;
-    KernelSource ks;
-    if (!ts_internal::enum_prepare(e, en, c->tune, ks, err, errlen))
-        return 0;
-    if (!build_module(e, ks.text, &c->mod, &c->compile_seconds, err, errlen))
-        return 0;
-    auto fail = [&](void) { (void)hipModuleUnload(c->mod); c->mod = nullptr; };
-    if (hipModuleGetFunction(&c->fn, c->mod, "ts_enum") != hipSuccess) {
-        fail();
-        say(err, errlen, "hipModuleGetFunction failed");
-        return 0;
+    auto fail = [&](void) {
+        if (c->mod)
+            (void)hipModuleUnload(c->mod);
+        c->mod = nullptr;
+        ts_internal::tape_free(c->td);
+    };
+    if (en->tune && en->tune->tape) {
+        auto t0 = std::chrono::steady_clock::now();
+        Tape t;
+        if (!ts_internal::enum_tape(e, en, c->tune, ts_internal::tape_max_slots(e), t, err,
+                                    errlen))
+            return 0;
+        HIPCHK(hipSetDevice(e->device));
+        if (!ts_internal::tape_upload(t, c->td) ||
+            !ts_internal::tape_prepare(e->device, c->td, &c->tape_blocks_per_cu)) {
+            fail();
+            say(err, errlen, "could not place the tape on the device");
+            return 0;
+        }
+        c->tape = true;
+        c->compile_seconds =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    } else {
+        KernelSource ks;
+        if (!ts_internal::enum_prepare(e, en, c->tune, ks, err, errlen))
+            return 0;
+        if (!build_module(e, ks.text, &c->mod, &c->compile_seconds, err, errlen))
+            return 0;
+        if (hipModuleGetFunction(&c->fn, c->mod, "ts_enum") != hipSuccess) {
+            fail();
+            say(err, errlen, "hipModuleGetFunction failed");
+            return 0;
+        }
     }
     c->part = en->part;
     c->part_end = en->part_end;
@@ -748,6 +853,7 @@ extern "C" void gpu_enum_free(gpu_enum_code *c)
         (void)hipFree(c->d_list);
     if (c->mod)
         (void)hipModuleUnload(c->mod);
+    ts_internal::tape_free(c->td);
     delete c;
 }
 
@@ -769,15 +875,27 @@ extern "C" int gpu_enum_exec(gpu_enum_code *c, ts_enum_result *r, char *err, siz
     auto run = [&](uint64_t i0, uint64_t i1, uint64_t *hits, uint64_t *nlist,
                    std::vector<uint64_t> &list, bool timed) -> int {
         HIPCHK(hipMemsetAsync(d_out, 0, 2 * sizeof(uint64_t), e->stream));
-        uint64_t blocks = (i1 - i0 + tune.block - 1) / tune.block;
-        unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, tune.grid));
         uint64_t space = si.space, capv = cap;
         uint32_t parts = (uint32_t)c->parts, part = (uint32_t)c->part, nshare = si.nshare;
-        void *args[] = {&d_out, &d_list, &capv, &i0, &i1, &space, &parts, &part, &nshare};
         if (timed)
             HIPCHK(hipEventRecord(e->ev0, e->stream));
-        if (!launch(e, fn, grid, tune.block, args, err, errlen))
-            return 0;
+        if (c->tape) {
+            // one wave per workgroup, four rounds of the resident workgroups
+            uint64_t blocks = (i1 - i0 + 63) / 64;
+            uint64_t most = (uint64_t)e->cus * c->tape_blocks_per_cu * 4;
+            unsigned grid = (unsigned)std::max<uint64_t>(1, std::min(blocks, most));
+            if (!ts_internal::tape_launch(c->td, e->stream, grid, d_out, d_list, capv, i0, i1,
+                                          space, parts, part, nshare)) {
+                say(err, errlen, "the tape launch failed");
+                return 0;
+            }
+        } else {
+            uint64_t blocks = (i1 - i0 + tune.block - 1) / tune.block;
+            unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, tune.grid));
+            void *args[] = {&d_out, &d_list, &capv, &i0, &i1, &space, &parts, &part, &nshare};
+            if (!launch(e, fn, grid, tune.block, args, err, errlen))
+                return 0;
+        }
         if (timed)
             HIPCHK(hipEventRecord(e->ev1, e->stream));
         uint64_t host[2];

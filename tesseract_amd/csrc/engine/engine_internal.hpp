@@ -10,6 +10,7 @@ struct gpu_engine {
     hipStream_t stream = nullptr;
     hipEvent_t  ev0 = nullptr, ev1 = nullptr;
     int         cus = 0;
+    size_t      lds_per_block = 160 * 1024;  // bytes of LDS one workgroup may allocate
     std::string arch, desc;
     double      last_ms = 0;
 };
@@ -40,5 +41,35 @@ int rtc_compile(const std::string &src, const std::string &arch, std::vector<cha
 // validation with the reference's messages, spec parsing and code generation of an enumeration
 int enum_prepare(const gpu_engine *e, const gpu_enum *en, gpu_tune &tune, ts::KernelSource &ks,
                  char *err, size_t errlen);
+
+// the same validation, then the enumeration's tape for the interpreter kernel (equiv and search
+// with one tick); max_slots bounds the per-lane LDS file
+int enum_tape(const gpu_engine *e, const gpu_enum *en, gpu_tune &tune, unsigned max_slots,
+              ts::Tape &t, char *err, size_t errlen);
+
+// slots of the tape interpreter's file that fit one workgroup's LDS
+inline unsigned tape_max_slots(const gpu_engine *e)
+{
+    return (unsigned)(e->lds_per_block / ts::Tape::kSlotBytes);
+}
+
+// A tape on the device (one allocation) and the interpreter kernel's launch (tape_kernel.hip).
+struct TapeDev {
+    uint32_t *base = nullptr, *ops = nullptr, *leaves = nullptr;
+    uint32_t  ngroups = 0, nleaves = 0, root = 0, slots = 0;
+    size_t    bytes = 0;
+};
+// device bytes of a tape; upload into `into` (at least tape_bytes, kept by the caller) or into a
+// fresh allocation that tape_free releases
+size_t tape_bytes(const ts::Tape &h);
+bool   tape_upload(const ts::Tape &h, TapeDev &d, uint32_t *into = nullptr);
+void   tape_free(TapeDev &d);
+size_t tape_lds_bytes(const TapeDev &t);
+// sets the kernel's LDS limit for this tape and reports resident workgroups per CU
+bool   tape_prepare(int device, const TapeDev &t, unsigned *blocks_per_cu);
+// one launch over words [i0, i1) with the compiled kernel's arguments and results
+bool   tape_launch(const TapeDev &t, hipStream_t stream, unsigned grid, uint64_t *out,
+                   uint64_t *list, uint64_t cap, uint64_t i0, uint64_t i1, uint64_t space,
+                   uint32_t parts, uint32_t part, uint32_t nshare);
 
 }  // namespace ts_internal

@@ -140,4 +140,36 @@ bool codegen_enum(const ts_program *a, const ts_program *b, const int *in_map, c
                   const EnumLowering &el, const CodegenOptions &opt, KernelSource &ks,
                   std::string &err, const ts_spec *spec);
 
+// One pass (ticks == 1) from power-on with the inputs set, as AIG literals: oa[k] = cell
+// el.out_cells_a[k] of a (search/verify) or a's k-th output (equiv), ob[k] = b's matching output.
+// Nodes [1, a_end) were created while executing a, [a_end, b_end) while executing b.
+void enum_outputs(const ts_program *a, const ts_program *b, const int *in_map, const int *out_map,
+                  const EnumLowering &el, Aig &g, std::vector<Lit> &oa, std::vector<Lit> &ob,
+                  uint32_t *a_end, uint32_t *b_end);
+
+// ---- tape: an enumeration as data for the precompiled interpreter kernel ----------------------
+//
+// The hit word of an equiv or search pass (ticks == 1) as a list of AIG AND nodes in groups of up
+// to kGroup mutually independent nodes, every value in a slot of a per-lane file in LDS. Slots are
+// reused once their last reader's group has read them; all reads of a group happen before its
+// writes, so a group may overwrite the slots it frees. Node order interleaves the two programs of
+// an equiv by relative position, so the live values stay those of a window of each program.
+// Every group has kGroup entries: a short group is padded with entries that read slot 0 and write
+// the last slot (the sink, read by nobody), so the kernel runs each group without branches.
+struct Tape {
+    static const unsigned kGroup = 16;
+    static const unsigned kSlotBytes = 256;    // one 32-bit word for each of a wave's 64 lanes
+    std::vector<uint32_t> ops;     // kGroup x 3 words per group: a | ca << 31, b | cb << 31, d
+    std::vector<uint32_t> count;   // nodes in each group, 1..kGroup
+    std::vector<uint32_t> leaves;  // input leaves: slot byte offset | input bit j << 24
+    uint32_t root = 0;             // hit word: slot byte offset | complemented << 31
+    uint32_t slots = 0;            // slot 0 holds 0 (the constant)
+    size_t   and_nodes = 0;
+};
+
+// Builds the tape of an equiv or search enumeration; false (with a message) if it needs more than
+// max_slots slots or the mode has no tape form (verify, ticks > 1).
+bool build_tape(const ts_program *a, const ts_program *b, const int *in_map, const int *out_map,
+                const EnumLowering &el, unsigned max_slots, Tape &t, std::string &err);
+
 }  // namespace ts

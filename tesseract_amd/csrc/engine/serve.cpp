@@ -11,6 +11,12 @@
 // increasing x, a chunk's hits all precede the next chunk's, so hit counts add and the 16 smallest
 // come from the earliest chunks (a chunk whose hit list overflowed is listed again by halves).
 // Compiled modules stay resident, keyed by their source, for requests that repeat a circuit.
+//
+// Two tiers: hipRTC takes 0.1-1.5 s for a kernel of 500-35,000 AND nodes, while the tape
+// interpreter (tape_kernel.hip) starts at once but runs about 100 times slower than a compiled
+// kernel. An equiv/search request whose estimated tape time is at most tape_max_ms runs on the
+// tape; the rest are compiled. Tape buffers are recycled through a pool, never freed while the
+// server runs (hipFree would wait for the whole device).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -68,8 +74,23 @@ struct Module {
     unsigned blocks_per_cu = 1;
 };
 
+// AND nodes per second of the tape interpreter with a file of about 230 slots (two workgroups per
+// CU): measured 2.4e13-2.5e13 on MI355X for netlist pairs of 3,000-35,000 nodes; smaller files
+// run faster, so the estimate errs on the slow side.
+constexpr double kTapeAndPerSecond = 2.0e13;
+
+struct TapeBuffer {
+    uint32_t *ptr = nullptr;
+    size_t bytes = 0;
+};
+
 struct Request {
     uint64_t id = 0;
+    bool tape = false;      // tier: tape interpreter instead of a compiled module
+    ts::Tape host_tape;     // until admission
+    TapeDev td{};
+    TapeBuffer tbuf{};
+    unsigned tape_bpc = 1;  // resident tape workgroups per CU
     gpu_enum en{};
     ts_program a{}, b{};
     bool has_b = false;
@@ -116,6 +137,8 @@ struct gpu_server {
     uint64_t next_id = 1;
     size_t module_bytes = 0;
     std::string fatal;
+    std::vector<TapeBuffer> tape_pool;  // free device buffers for tapes
+    std::vector<TapeBuffer> tape_all;   // every tape buffer, freed at close
 };
 
 namespace {
@@ -134,9 +157,26 @@ void compile_loop(gpu_server *s)
             r->state = ST_COMPILING;
         }
         char err[1024] = "";
+        // tier: the tape when its estimated device time is small (a refusal, e.g. too many
+        // slots, leaves the request to the compiler, which reports validation errors alike)
+        if (s->tune.tape_max_ms >= 0 && r->en.mode != TS_EN_VERIFY && r->en.ticks == 1) {
+            ts::Tape tp;
+            gpu_tune tn{};
+            if (enum_tape(s->e, &r->en, tn, tape_max_slots(s->e), tp, err, sizeof err)) {
+                ShareInfo si = shares((unsigned)r->en.a->ninputs, r->en.part, r->en.part_end,
+                                      r->en.parts);
+                double ms = (double)si.tried * (double)tp.and_nodes / kTapeAndPerSecond * 1e3;
+                if (ms <= s->tune.tape_max_ms) {
+                    r->host_tape = std::move(tp);
+                    r->tune = tn;
+                    r->tape = true;
+                }
+            }
+            err[0] = 0;
+        }
         ts::KernelSource ks;
-        bool ok = enum_prepare(s->e, &r->en, r->tune, ks, err, sizeof err) != 0;
-        if (ok) {
+        bool ok = r->tape || enum_prepare(s->e, &r->en, r->tune, ks, err, sizeof err) != 0;
+        if (ok && !r->tape) {
             r->source = ks.text;
             bool cached;
             {
@@ -148,6 +188,7 @@ void compile_loop(gpu_server *s)
         }
         std::lock_guard<std::mutex> lk(s->mu);
         r->times.compiled_s = now_s();
+        r->times.tier = r->tape ? 1 : 0;
         if (!ok) {
             r->error = err;
             r->state = ST_FAILED;
@@ -161,8 +202,49 @@ void compile_loop(gpu_server *s)
     }
 }
 
-// device side of one request: load (or reuse) its module and cut its share into chunks
-bool admit(gpu_server *s, Request *r)
+// a pooled device buffer of at least `bytes` (sizes are powers of two from 64 KiB)
+bool take_tape_buffer(gpu_server *s, size_t bytes, TapeBuffer &out)
+{
+    size_t best = SIZE_MAX;
+    for (size_t k = 0; k < s->tape_pool.size(); k++)
+        if (s->tape_pool[k].bytes >= bytes &&
+            (best == SIZE_MAX || s->tape_pool[k].bytes < s->tape_pool[best].bytes))
+            best = k;
+    if (best != SIZE_MAX) {
+        out = s->tape_pool[best];
+        s->tape_pool.erase(s->tape_pool.begin() + (long)best);
+        return true;
+    }
+    size_t cap = 64 * 1024;
+    while (cap < bytes)
+        cap *= 2;
+    if (hipMalloc(&out.ptr, cap) != hipSuccess)
+        return false;
+    out.bytes = cap;
+    s->tape_all.push_back(out);
+    return true;
+}
+
+// device side of a tape request: its tape in a pooled buffer
+bool admit_tape(gpu_server *s, Request *r)
+{
+    if (!take_tape_buffer(s, tape_bytes(r->host_tape), r->tbuf)) {
+        r->error = "no device memory for the tape";
+        return false;
+    }
+    if (!tape_upload(r->host_tape, r->td, r->tbuf.ptr) ||
+        !tape_prepare(s->e->device, r->td, &r->tape_bpc)) {
+        s->tape_pool.push_back(r->tbuf);
+        r->tbuf = TapeBuffer();
+        r->error = "could not place the tape on the device";
+        return false;
+    }
+    r->host_tape = ts::Tape();
+    return true;
+}
+
+// device side of a compiled request: load (or reuse) its module
+bool admit_module(gpu_server *s, Request *r)
 {
     Module &m = s->modules[r->source];
     if (!m.mod) {
@@ -194,6 +276,14 @@ bool admit(gpu_server *s, Request *r)
     r->mod = &m;
     r->code.clear();
     r->code.shrink_to_fit();
+    return true;
+}
+
+// device side of one request, then its share cut into chunks
+bool admit(gpu_server *s, Request *r)
+{
+    if (!(r->tape ? admit_tape(s, r) : admit_module(s, r)))
+        return false;
     r->si = shares((unsigned)r->en.a->ninputs, r->en.part, r->en.part_end, r->en.parts);
     r->result.space = r->si.space;
     r->result.tried = r->si.tried;
@@ -232,17 +322,25 @@ void evict(gpu_server *s, size_t need)
 bool launch_chunk(gpu_server *s, Slot &sl, Request *r, size_t ci)
 {
     Chunk &c = r->chunks[ci];
-    const unsigned block = r->tune.block;
+    const unsigned block = r->tape ? 64 : r->tune.block;
+    const unsigned per_cu = r->tape ? r->tape_bpc : r->mod->blocks_per_cu;
     uint64_t blocks = (c.i1 - c.i0 + block - 1) / block;
-    uint64_t share = std::max<uint64_t>(1, (uint64_t)s->e->cus * r->mod->blocks_per_cu / s->slots.size());
+    uint64_t share = std::max<uint64_t>(1, (uint64_t)s->e->cus * per_cu / s->slots.size());
     unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, share));
     uint64_t space = r->si.space, cap = kListCap, i0 = c.i0, i1 = c.i1;
     uint32_t parts = (uint32_t)r->en.parts, part = (uint32_t)r->en.part, nshare = r->si.nshare;
     void *args[] = {&sl.d_out, &sl.d_list, &cap, &i0, &i1, &space, &parts, &part, &nshare};
-    if (hipMemsetAsync(sl.d_out, 0, 16, sl.stream) != hipSuccess ||
-        hipModuleLaunchKernel(r->mod->fn, grid, 1, 1, block, 1, 1, 0, sl.stream, args, nullptr) !=
-            hipSuccess ||
-        hipMemcpyAsync(sl.h_out, sl.d_out, 16, hipMemcpyDeviceToHost, sl.stream) != hipSuccess ||
+    if (hipMemsetAsync(sl.d_out, 0, 16, sl.stream) != hipSuccess)
+        return false;
+    if (r->tape) {
+        if (!tape_launch(r->td, sl.stream, grid, sl.d_out, sl.d_list, cap, i0, i1, space, parts,
+                         part, nshare))
+            return false;
+    } else if (hipModuleLaunchKernel(r->mod->fn, grid, 1, 1, block, 1, 1, 0, sl.stream, args,
+                                     nullptr) != hipSuccess) {
+        return false;
+    }
+    if (hipMemcpyAsync(sl.h_out, sl.d_out, 16, hipMemcpyDeviceToHost, sl.stream) != hipSuccess ||
         hipEventRecord(sl.done, sl.stream) != hipSuccess)
         return false;
     sl.req = r;
@@ -340,7 +438,7 @@ void schedule_loop(gpu_server *s)
         // admission: budget and active-request limit
         while (!s->ready.empty() && s->active.size() < s->tune.max_active) {
             Request *r = s->ready.front();
-            size_t need = s->modules.count(r->source) ? 0 : r->code.size();
+            size_t need = r->tape || s->modules.count(r->source) ? 0 : r->code.size();
             evict(s, need);
             if (need && s->module_bytes + need > s->tune.memory_budget && !s->active.empty())
                 break;  // wait for memory to free up
@@ -374,7 +472,13 @@ void schedule_loop(gpu_server *s)
             Request *r = s->active[k];
             if (r->chunks_done == r->chunks.size() && r->next_chunk == r->chunks.size()) {
                 bool ok = finalize(s, r);
-                r->mod->users--;
+                if (r->mod)
+                    r->mod->users--;
+                if (r->tbuf.ptr) {  // every chunk and re-listing of this tape has completed
+                    s->tape_pool.push_back(r->tbuf);
+                    r->tbuf = TapeBuffer();
+                    r->td = TapeDev();
+                }
                 r->state = ok ? ST_DONE : ST_FAILED;
                 if (!ok)
                     r->error = "listing hits failed";
@@ -439,6 +543,8 @@ extern "C" int gpu_server_open(gpu_engine *e, const gpu_serve_tune *tune, gpu_se
         s->tune.chunk_words = 1ull << 24;
     if (!s->tune.max_active)
         s->tune.max_active = 64;
+    if (s->tune.tape_max_ms == 0)
+        s->tune.tape_max_ms = 50;
     if (hipSetDevice(e->device) != hipSuccess) {
         put_err(err, errlen, "hipSetDevice failed");
         return 0;
@@ -580,6 +686,8 @@ extern "C" void gpu_server_close(gpu_server *s)
     for (auto &kv : s->modules)
         if (kv.second.mod)
             (void)hipModuleUnload(kv.second.mod);
+    for (TapeBuffer &b : s->tape_all)
+        (void)hipFree(b.ptr);
     for (auto &kv : s->all) {
         ts_program_free(&kv.second->a);
         if (kv.second->has_b)

@@ -1,0 +1,165 @@
+// Tape building: an equiv/search enumeration pass as data for the precompiled interpreter kernel
+// (tape_kernel.hip). No hipRTC compile is needed, so a request whose input space is small starts within
+// milliseconds instead of the seconds LLVM spends on a large straight-line kernel.
+#include <algorithm>
+#include <queue>
+
+#include "lower.hpp"
+
+namespace ts {
+
+bool build_tape(const ts_program *a, const ts_program *b, const int *in_map, const int *out_map,
+                const EnumLowering &el, unsigned max_slots, Tape &t, std::string &err)
+{
+    if (el.mode == TS_EN_VERIFY || el.ticks != 1) {
+        err = "only equiv and search with one tick have a tape form";
+        return false;
+    }
+    Aig g;
+    std::vector<Lit> oa, ob;
+    uint32_t a_end = 0, b_end = 0;
+    enum_outputs(a, b, in_map, out_map, el, g, oa, ob, &a_end, &b_end);
+
+    // the hit word: equiv = OR_k (a_k ^ b_k), search = AND_bit (bit == target bit)
+    // (src/enum.c:150-213)
+    Lit hit;
+    if (el.mode == TS_EN_EQUIV) {
+        hit = 0;
+        for (size_t k = 0; k < oa.size(); k++)
+            hit = g.Or(hit, g.Xor(oa[k], ob[k]));
+    } else {
+        hit = 1;
+        for (int bit = 0; bit < el.width; bit++)
+            hit = g.And(hit, (el.target >> bit & 1) ? oa[bit] : lit_not(oa[bit]));
+    }
+    const uint32_t nn = (uint32_t)g.nodes.size();
+
+    // the cone of the hit word, with each node's readers inside it
+    std::vector<uint8_t> in_cone(nn, 0);
+    std::vector<uint32_t> stack{hit >> 1}, uses(nn, 0);
+    while (!stack.empty()) {
+        uint32_t v = stack.back();
+        stack.pop_back();
+        if (in_cone[v])
+            continue;
+        in_cone[v] = 1;
+        const Node &nd = g.nodes[v];
+        if (nd.kind == N_AND) {
+            stack.push_back(nd.a >> 1);
+            stack.push_back(nd.b >> 1);
+        }
+    }
+    std::vector<std::vector<uint32_t>> readers(nn);
+    size_t and_nodes = 0;
+    for (uint32_t v = 0; v < nn; v++) {
+        if (!in_cone[v] || g.nodes[v].kind != N_AND)
+            continue;
+        and_nodes++;
+        const Node &nd = g.nodes[v];
+        uses[nd.a >> 1]++;
+        uses[nd.b >> 1]++;
+        readers[nd.a >> 1].push_back(v);
+        if ((nd.b >> 1) != (nd.a >> 1))
+            readers[nd.b >> 1].push_back(v);
+    }
+    uses[hit >> 1]++;  // the final read
+
+    // relative position of a node in the execution that created it: interleaves a and b
+    auto when = [&](uint32_t v) -> double {
+        if (v < a_end)
+            return (double)v / std::max<uint32_t>(1, a_end);
+        if (v < b_end)
+            return (double)(v - a_end) / std::max<uint32_t>(1, b_end - a_end);
+        return 1.0 + (double)(v - b_end) / std::max<uint32_t>(1, nn - b_end);
+    };
+
+    t = Tape();
+    t.and_nodes = and_nodes;
+    std::vector<int32_t> slot(nn, -1);
+    std::vector<uint32_t> free_slots;
+    uint32_t next_slot = 1;  // slot 0 = constant 0
+    auto alloc = [&]() -> uint32_t {
+        if (!free_slots.empty()) {
+            uint32_t s = free_slots.back();
+            free_slots.pop_back();
+            return s;
+        }
+        return next_slot++;
+    };
+    slot[0] = 0;
+    std::vector<uint8_t> done(nn, 0);
+    done[0] = 1;
+    for (uint32_t v = 1; v < nn; v++)
+        if (in_cone[v] && g.nodes[v].kind == N_INBIT) {
+            slot[v] = (int32_t)alloc();
+            done[v] = 1;
+            t.leaves.push_back((uint32_t)slot[v] * Tape::kSlotBytes | g.nodes[v].a << 24);
+        }
+
+    typedef std::pair<double, uint32_t> Key;
+    std::priority_queue<Key, std::vector<Key>, std::greater<Key>> ready;
+    std::vector<uint8_t> waiting(nn, 0);  // operands not yet computed
+    for (uint32_t v = 0; v < nn; v++) {
+        if (!in_cone[v] || g.nodes[v].kind != N_AND)
+            continue;
+        const Node &nd = g.nodes[v];
+        waiting[v] = (uint8_t)(!done[nd.a >> 1] + ((nd.b >> 1) != (nd.a >> 1) && !done[nd.b >> 1]));
+        if (!waiting[v])
+            ready.push(Key(when(v), v));
+    }
+    std::vector<uint32_t> group;
+    std::vector<size_t> padding;  // ops words that name the sink slot
+    size_t scheduled = 0;
+    while (!ready.empty()) {
+        group.clear();
+        while (!ready.empty() && group.size() < Tape::kGroup) {
+            group.push_back(ready.top().second);
+            ready.pop();
+        }
+        // reads first: operands read for the last time free their slots for this group's writes
+        for (uint32_t v : group) {
+            const Node &nd = g.nodes[v];
+            for (uint32_t u : {nd.a >> 1, nd.b >> 1})
+                if (--uses[u] == 0 && u != 0)
+                    free_slots.push_back((uint32_t)slot[u]);
+        }
+        for (uint32_t v : group) {
+            const Node &nd = g.nodes[v];
+            slot[v] = (int32_t)alloc();
+            t.ops.push_back((uint32_t)slot[nd.a >> 1] * Tape::kSlotBytes | (nd.a & 1u) << 31);
+            t.ops.push_back((uint32_t)slot[nd.b >> 1] * Tape::kSlotBytes | (nd.b & 1u) << 31);
+            t.ops.push_back((uint32_t)slot[v] * Tape::kSlotBytes);
+        }
+        for (size_t k = group.size(); k < Tape::kGroup; k++) {
+            // padding: reads slot 0, writes the sink slot (numbered once the file size is known)
+            t.ops.push_back(0);
+            t.ops.push_back(0);
+            padding.push_back(t.ops.size());
+            t.ops.push_back(0);
+        }
+        t.count.push_back((uint32_t)group.size());
+        scheduled += group.size();
+        for (uint32_t v : group) {
+            done[v] = 1;
+            for (uint32_t r : readers[v])
+                if (--waiting[r] == 0)
+                    ready.push(Key(when(r), r));
+        }
+        if (next_slot + 1 > max_slots) {
+            err = "the tape needs more than " + std::to_string(max_slots) + " slots";
+            return false;
+        }
+    }
+    if (scheduled != and_nodes) {
+        err = "tape scheduling left nodes behind";
+        return false;
+    }
+    t.root = (uint32_t)slot[hit >> 1] * Tape::kSlotBytes | (hit & 1u) << 31;
+    const uint32_t sink = next_slot;
+    for (size_t w : padding)
+        t.ops[w] = sink * Tape::kSlotBytes;
+    t.slots = sink + 1;
+    return true;
+}
+
+}  // namespace ts

@@ -1,0 +1,160 @@
+// The tape interpreter: one precompiled kernel that runs any equiv/search enumeration given as a
+// tape (groups of independent AND nodes over a per-lane file of slots in LDS). It is the no-compile
+// tier of the engine: a request starts as soon as its tape is built on the host.
+//
+// Layout: a workgroup is one wave of 64 lanes; lane l owns the 32-bit word at byte
+// slot * 256 + 4 l of the workgroup's LDS file, so every slot access of a wave is one conflict-free
+// 256-byte row. Each lane enumerates one word (32 assignments) at a time exactly like the compiled
+// ts_enum kernel — same word order, same share rule, same valid mask, same hit list of
+// (word, mask) pairs — so the host's collection of hits is shared by both tiers.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "engine_internal.hpp"
+
+namespace {
+
+constexpr unsigned kGroup = ts::Tape::kGroup;
+
+// bit j < 5 of x = 32 w + k for the 32 lanes k of a word
+__device__ inline uint32_t lane_pattern(uint32_t j)
+{
+    return j == 0u ? 0xAAAAAAAAu
+         : j == 1u ? 0xCCCCCCCCu
+         : j == 2u ? 0xF0F0F0F0u
+         : j == 3u ? 0xFF00FF00u
+                   : 0xFFFF0000u;
+}
+
+__global__ void __launch_bounds__(64)
+    ts_enum_tape(const uint32_t *__restrict__ ops, uint32_t ngroups, const uint32_t *__restrict__ leaves, uint32_t nleaves,
+                 uint32_t root, uint64_t *__restrict__ out, uint64_t *__restrict__ list,
+                 uint64_t cap, uint64_t i0, uint64_t i1, uint64_t space, uint32_t parts,
+                 uint32_t part, uint32_t nshare)
+{
+    extern __shared__ uint32_t file[];
+    char *const my = (char *)file + threadIdx.x * 4u;
+    *(uint32_t *)my = 0u;  // slot 0: the constant
+    for (uint64_t i = i0 + (uint64_t)blockIdx.x * 64u + threadIdx.x; i < i1;
+         i += (uint64_t)gridDim.x * 64u) {
+        uint64_t gw = i;
+        if (parts > 1u) {  // active chunk (64 passes of 512 = 1024 words) -> chunk index
+            const uint64_t ci = i >> 10, q = ci / nshare, r = part + ci % nshare;
+            gw = ((q * parts + r) << 10) | (i & 1023u);
+        }
+        const uint64_t xb = gw << 5;
+        const uint32_t valid = xb >= space               ? 0u
+                               : space - xb >= 32ull     ? 0xffffffffu
+                                                         : ((1u << (uint32_t)(space - xb)) - 1u);
+        for (uint32_t k = 0; k < nleaves; k++) {
+            const uint32_t lf = leaves[k], j = lf >> 24;
+            *(uint32_t *)(my + (lf & 0xffffffu)) =
+                j < 5u ? lane_pattern(j) : 0u - (uint32_t)((gw >> (j - 5u)) & 1ull);
+        }
+        // every group has kGroup entries (short ones padded onto the sink slot): all reads of
+        // the group are issued before the first write, with no branch between them
+        for (uint32_t g = 0; g < ngroups; g++) {
+            const uint32_t *op = ops + (size_t)g * (3 * kGroup);
+            uint32_t v[kGroup];
+#pragma unroll
+            for (unsigned j = 0; j < kGroup; j++) {
+                const uint32_t a = op[3 * j], b = op[3 * j + 1];
+                const uint32_t x = *(const uint32_t *)(my + (a & 0x7fffffffu)) ^ (0u - (a >> 31));
+                const uint32_t y = *(const uint32_t *)(my + (b & 0x7fffffffu)) ^ (0u - (b >> 31));
+                v[j] = x & y;
+            }
+#pragma unroll
+            for (unsigned j = 0; j < kGroup; j++)
+                *(uint32_t *)(my + op[3 * j + 2]) = v[j];
+        }
+        const uint32_t hit =
+            (*(const uint32_t *)(my + (root & 0x7fffffffu)) ^ (0u - (root >> 31))) & valid;
+        if (hit) {
+            atomicAdd((unsigned long long *)&out[0], (unsigned long long)__builtin_popcount(hit));
+            const uint64_t k = atomicAdd((unsigned long long *)&out[1], 1ull);
+            if (k < cap) {
+                list[2 * k] = gw;
+                list[2 * k + 1] = hit;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+namespace ts_internal {
+
+size_t tape_lds_bytes(const TapeDev &t) { return (size_t)t.slots * ts::Tape::kSlotBytes; }
+
+bool tape_prepare(int device, const TapeDev &t, unsigned *blocks_per_cu)
+{
+    const size_t lds = tape_lds_bytes(t);
+    int max_lds = 0;
+    if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) !=
+            hipSuccess ||
+        lds > (size_t)max_lds)
+        return false;
+    // the kernel's limit is the device's (the same value from every thread), so launches of
+    // tapes of any size may run concurrently
+    if (hipFuncSetAttribute((const void *)ts_enum_tape, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            max_lds) != hipSuccess)
+        return false;
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, ts_enum_tape, 64, lds) !=
+            hipSuccess ||
+        blocks < 1)
+        return false;
+    *blocks_per_cu = (unsigned)blocks;
+    return true;
+}
+
+bool tape_launch(const TapeDev &t, hipStream_t stream, unsigned grid, uint64_t *out,
+                 uint64_t *list, uint64_t cap, uint64_t i0, uint64_t i1, uint64_t space,
+                 uint32_t parts, uint32_t part, uint32_t nshare)
+{
+    ts_enum_tape<<<grid, 64, tape_lds_bytes(t), stream>>>(t.ops, t.ngroups, t.leaves,
+                                                           t.nleaves, t.root, out, list, cap, i0,
+                                                           i1, space, parts, part, nshare);
+    return hipGetLastError() == hipSuccess;
+}
+
+size_t tape_bytes(const ts::Tape &h) { return (h.ops.size() + h.leaves.size() + 1) * 4; }
+
+bool tape_upload(const ts::Tape &h, TapeDev &d, uint32_t *into)
+{
+    d = TapeDev();
+    const size_t nops = h.ops.size(), nleaves = h.leaves.size();
+    const size_t words = nops + nleaves + 1;
+    uint32_t *buf = into;
+    if (!buf && hipMalloc(&buf, words * sizeof(uint32_t)) != hipSuccess)
+        return false;
+    std::vector<uint32_t> all;
+    all.reserve(words);
+    all.insert(all.end(), h.ops.begin(), h.ops.end());
+    all.insert(all.end(), h.leaves.begin(), h.leaves.end());
+    all.push_back(0);
+    if (hipMemcpy(buf, all.data(), words * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
+        if (!into)
+            (void)hipFree(buf);
+        return false;
+    }
+    d.base = into ? nullptr : buf;  // a caller's buffer stays the caller's
+    d.ops = buf;
+    d.leaves = buf + nops;
+    d.ngroups = (uint32_t)h.count.size();
+    d.nleaves = (uint32_t)nleaves;
+    d.root = h.root;
+    d.slots = h.slots;
+    d.bytes = words * sizeof(uint32_t);
+    return true;
+}
+
+void tape_free(TapeDev &d)
+{
+    if (d.base)
+        (void)hipFree(d.base);
+    d = TapeDev();
+}
+
+}  // namespace ts_internal

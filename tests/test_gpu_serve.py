@@ -60,6 +60,9 @@ def test_server_mix_equals_alone(engine):
     assert all(tm["done_s"] <= big_times["done_s"] for tm in inside)
     adder_res = results.pop(6)
     assert adder_res["hits"] == 65536 and adder_res["hit"] == [2 * h + 1 for h in range(16)]
+    # both tiers served: small requests on the tape interpreter, the big one and verify compiled
+    assert big_times["tier"] == 0 and times[6]["tier"] == 0
+    assert sum(tm["tier"] for tm in times) >= 10
     key = lambda r: (r["hits"], r["hit"], r["tried"], r["ops"], r["space"])
     for (a, b, c, r, faulty), got in zip(reqs, results):
         alone = engine.enum_run(t.EN_EQUIV, a, b)
@@ -80,3 +83,21 @@ def test_server_reports_errors(engine):
     with pytest.raises(ValueError, match="id and link cells"):
         srv.wait(tk)
     srv.close()
+
+
+def test_server_tiers_agree(engine):
+    # the same requests through the tape tier (generous limit) and compiled only (limit < 0)
+    reqs = []
+    for k in range(8):
+        c, r, faulty = small_request(9, k, [12, 20, 26, 16][k % 4], [80, 700, 2500][k % 3])
+        reqs.append((t.net_parse(c.bench()), t.net_parse(r.bench())))
+    out = {}
+    for limit in (1e6, -1.0):
+        srv = t.Server(engine, streams=3, chunk_words=1 << 16, tape_max_ms=limit)
+        tks = [srv.submit(t.EN_EQUIV, a, b) for a, b in reqs]
+        out[limit] = [srv.wait(tk) for tk in tks]
+        tiers = [srv.times(tk)["tier"] for tk in tks]
+        srv.close()
+        assert tiers == [1 if limit > 0 else 0] * len(reqs)
+    key = lambda r: (r["hits"], r["hit"], r["tried"], r["ops"], r["space"])
+    assert [key(r) for r in out[1e6]] == [key(r) for r in out[-1.0]]

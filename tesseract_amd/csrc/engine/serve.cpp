@@ -124,14 +124,15 @@ struct gpu_server {
     gpu_engine *e = nullptr;
     gpu_serve_tune tune{};
     std::mutex mu;
-    std::condition_variable cv_compile, cv_sched, cv_done;
-    std::deque<Request *> to_compile, ready;
+    std::condition_variable cv_plan, cv_compile, cv_sched, cv_done;
+    std::deque<Request *> to_plan, to_compile, ready;
+    unsigned preparing = 0;  // requests taken by a planner and not yet ready or failed
     std::vector<Request *> active;
     std::unordered_map<uint64_t, std::unique_ptr<Request>> all;
     std::unordered_map<std::string, Module> modules;
     std::vector<Slot> slots;
     Slot relist_slot;  // synchronous re-listing of overflowed chunks, off the dispatch slots
-    std::vector<std::thread> compilers;
+    std::vector<std::thread> planners, compilers;
     std::thread scheduler;
     std::atomic<bool> stop{false};
     uint64_t next_id = 1;
@@ -143,23 +144,45 @@ struct gpu_server {
 
 namespace {
 
-void compile_loop(gpu_server *s)
+// the end of a request's preparation, under the lock: on to admission, or failed
+void prepared(gpu_server *s, Request *r, bool ok, const char *err)
+{
+    s->preparing--;
+    s->cv_compile.notify_all();  // compile threads leave once nothing is left to prepare
+    r->times.compiled_s = now_s();
+    r->times.tier = r->tape ? 1 : 0;
+    if (!ok) {
+        r->error = err;
+        r->state = ST_FAILED;
+        r->times.done_s = r->times.compiled_s;
+        s->cv_done.notify_all();
+    } else {
+        r->state = ST_WAITING;
+        s->ready.push_back(r);
+        s->cv_sched.notify_all();
+    }
+}
+
+// Planner threads pick each request's tier as it arrives: the tape when its estimated device
+// time is small (building it takes milliseconds, so these requests never wait behind a compile),
+// otherwise the compile queue. A refusal (e.g. too many slots) leaves the request to the compiler,
+// which reports validation errors with the same messages.
+void plan_loop(gpu_server *s)
 {
     for (;;) {
         Request *r;
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            s->cv_compile.wait(lk, [&] { return s->stop || !s->to_compile.empty(); });
-            if (s->stop && s->to_compile.empty())
+            s->cv_plan.wait(lk, [&] { return s->stop || !s->to_plan.empty(); });
+            if (s->stop && s->to_plan.empty())
                 return;
-            r = s->to_compile.front();
-            s->to_compile.pop_front();
+            r = s->to_plan.front();
+            s->to_plan.pop_front();
             r->state = ST_COMPILING;
+            s->preparing++;
         }
-        char err[1024] = "";
-        // tier: the tape when its estimated device time is small (a refusal, e.g. too many
-        // slots, leaves the request to the compiler, which reports validation errors alike)
         if (s->tune.tape_max_ms >= 0 && r->en.mode != TS_EN_VERIFY && r->en.ticks == 1) {
+            char err[1024] = "";
             ts::Tape tp;
             gpu_tune tn{};
             if (enum_tape(s->e, &r->en, tn, tape_max_slots(s->e), tp, err, sizeof err)) {
@@ -172,11 +195,36 @@ void compile_loop(gpu_server *s)
                     r->tape = true;
                 }
             }
-            err[0] = 0;
         }
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (r->tape) {
+            prepared(s, r, true, "");
+        } else {
+            s->to_compile.push_back(r);
+            s->cv_compile.notify_one();
+        }
+    }
+}
+
+void compile_loop(gpu_server *s)
+{
+    for (;;) {
+        Request *r;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_compile.wait(lk, [&] {
+                return !s->to_compile.empty() ||
+                       (s->stop && s->to_plan.empty() && s->preparing == 0);
+            });
+            if (s->to_compile.empty())
+                return;
+            r = s->to_compile.front();
+            s->to_compile.pop_front();
+        }
+        char err[1024] = "";
         ts::KernelSource ks;
-        bool ok = r->tape || enum_prepare(s->e, &r->en, r->tune, ks, err, sizeof err) != 0;
-        if (ok && !r->tape) {
+        bool ok = enum_prepare(s->e, &r->en, r->tune, ks, err, sizeof err) != 0;
+        if (ok) {
             r->source = ks.text;
             bool cached;
             {
@@ -187,18 +235,7 @@ void compile_loop(gpu_server *s)
                 ok = rtc_compile(r->source, s->e->arch, r->code, nullptr, err, sizeof err) != 0;
         }
         std::lock_guard<std::mutex> lk(s->mu);
-        r->times.compiled_s = now_s();
-        r->times.tier = r->tape ? 1 : 0;
-        if (!ok) {
-            r->error = err;
-            r->state = ST_FAILED;
-            r->times.done_s = r->times.compiled_s;
-            s->cv_done.notify_all();
-        } else {
-            r->state = ST_WAITING;
-            s->ready.push_back(r);
-            s->cv_sched.notify_all();
-        }
+        prepared(s, r, ok, err);
     }
 }
 
@@ -512,7 +549,8 @@ void schedule_loop(gpu_server *s)
         bool in_flight = false;
         for (Slot &sl : s->slots)
             in_flight |= sl.req != nullptr;
-        if (s->stop && !in_flight && s->ready.empty() && s->to_compile.empty() && s->active.empty())
+        if (s->stop && !in_flight && s->ready.empty() && s->to_plan.empty() && s->preparing == 0 &&
+            s->active.empty())
             return;
         if (!busy) {
             if (in_flight)
@@ -567,6 +605,8 @@ extern "C" int gpu_server_open(gpu_engine *e, const gpu_serve_tune *tune, gpu_se
         }
     }
     gpu_server *raw = s.get();
+    for (unsigned k = 0; k < 2; k++)
+        raw->planners.emplace_back(plan_loop, raw);
     for (unsigned k = 0; k < raw->tune.compile_threads; k++)
         raw->compilers.emplace_back(compile_loop, raw);
     raw->scheduler = std::thread(schedule_loop, raw);
@@ -600,9 +640,9 @@ extern "C" int gpu_server_submit(gpu_server *s, const gpu_enum *en, uint64_t *ti
     }
     r->id = s->next_id++;
     *ticket = r->id;
-    s->to_compile.push_back(r.get());
+    s->to_plan.push_back(r.get());
     s->all[r->id] = std::move(r);
-    s->cv_compile.notify_one();
+    s->cv_plan.notify_one();
     return 1;
 }
 
@@ -661,8 +701,11 @@ extern "C" void gpu_server_close(gpu_server *s)
         std::lock_guard<std::mutex> lk(s->mu);
         s->stop = true;
     }
+    s->cv_plan.notify_all();
     s->cv_compile.notify_all();
     s->cv_sched.notify_all();
+    for (std::thread &t : s->planners)
+        t.join();
     for (std::thread &t : s->compilers)
         t.join();
     if (s->scheduler.joinable())

@@ -32,6 +32,8 @@ typedef struct {
     int         ng, capg;
     int        *inputs, *outputs;
     int         nin, nout, capin, capout;
+    int        *slot;    /* open-addressing table of signal numbers by name, -1 = empty */
+    int         nslot;   /* a power of two, at least twice n */
     char       *err;
     size_t      errlen;
     const char *file;
@@ -70,11 +72,40 @@ static int failed(netlist *net, int line, const char *fmt, ...)
     return 0;
 }
 
+static unsigned name_hash(const char *s)
+{
+    unsigned h = 2166136261u;  /* FNV-1a */
+    for (; *s; s++)
+        h = (h ^ (unsigned char)*s) * 16777619u;
+    return h;
+}
+
+static void rehash(netlist *net, int nslot)
+{
+    free(net->slot);
+    net->slot = malloc((size_t)nslot * sizeof *net->slot);
+    if (!net->slot)
+        abort();
+    net->nslot = nslot;
+    for (int k = 0; k < nslot; k++)
+        net->slot[k] = -1;
+    for (int i = 0; i < net->n; i++) {
+        unsigned k = name_hash(net->name[i]) & (unsigned)(nslot - 1);
+        while (net->slot[k] >= 0)
+            k = (k + 1) & (unsigned)(nslot - 1);
+        net->slot[k] = i;
+    }
+}
+
 static int signal_of(netlist *net, const char *name)
 {
-    for (int i = 0; i < net->n; i++)
-        if (!strcmp(net->name[i], name))
-            return i;
+    if (2 * (net->n + 1) > net->nslot)
+        rehash(net, net->nslot ? 2 * net->nslot : 1024);
+    unsigned k = name_hash(name) & (unsigned)(net->nslot - 1);
+    for (; net->slot[k] >= 0; k = (k + 1) & (unsigned)(net->nslot - 1))
+        if (!strcmp(net->name[net->slot[k]], name))
+            return net->slot[k];
+    net->slot[k] = net->n;
     int c1 = net->cap;
     net->name = xgrow(net->name, &c1, net->n + 1, sizeof *net->name);
     int c2 = net->cap;
@@ -307,5 +338,6 @@ int ts_net_parse(const char *text, const char *name, ts_program *out, char *err,
     free(net.g);
     free(net.inputs);
     free(net.outputs);
+    free(net.slot);
     return ok;
 }

@@ -16,6 +16,7 @@ import argparse
 import ctypes as C
 import json
 import math
+import sys
 import time
 
 import numpy as np
@@ -130,9 +131,13 @@ def run_serve(engine, n, rate, seed, streams=4, chunk_words=1 << 24, check=True)
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
     reqs = []
+    t_gen = time.perf_counter()
     for k in range(n):
         c, r, n_in, faulty = N.request(seed, k)
         reqs.append((t.net_parse(c.bench()), t.net_parse(r.bench()), c, r, n_in, faulty))
+        if (k + 1) % 50 == 0:
+            print("serve: %d requests generated (%.1f s)" % (k + 1, time.perf_counter() - t_gen),
+                  file=sys.stderr, flush=True)
     srv = t.Server(engine, streams=streams, chunk_words=chunk_words)
     tickets = [None] * n
     t0 = time.perf_counter() + 0.05
@@ -160,7 +165,8 @@ def run_serve(engine, n, rate, seed, streams=4, chunk_words=1 << 24, check=True)
            "ops_per_s": sum(r["ops"] for r in results) / (end - first),
            "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
            "compile_p50_ms": float(np.median([tm["compiled_s"] - tm["submitted_s"] for tm in times]) * 1e3),
-           "faulty": int(sum(q[5] for q in reqs)), "mismatching": int(sum(r["hits"] > 0 for r in results))}
+           "faulty": int(sum(q[5] for q in reqs)), "mismatching": int(sum(r["hits"] > 0 for r in results)),
+           "tape_tier_requests": int(sum(tm["tier"] == 1 for tm in times))}
     classes = {}
     for lo, hi in ((8, 16), (16, 24), (24, 30), (30, 37)):
         m = (n_in >= lo) & (n_in < hi)

@@ -196,11 +196,11 @@ int  gpu_source(const ts_program *p, const gpu_tune *tune, char **source, gpu_co
                 char *err, size_t errlen);
 int  gpu_enum_source(const gpu_enum *en, char **source, char *err, size_t errlen);
 /* The tape the interpreter tier (gpu_tune.tape) runs for an equiv or search enumeration, built on
-   the host: groups of 16 entries of independent AND nodes, 3 words per entry (a | ca << 31,
-   b | cb << 31, d: byte offsets of slots of 256 bytes, c = complemented), `count` real nodes per
-   group (the rest read slot 0 and write the last slot), input leaves (slot offset | input bit
-   j << 24), the hit word's slot (| complemented << 31). Free with gpu_tape_free. max_slots 0 = 640
-   (160 KB of LDS). */
+   the host: groups of 16 entries of independent AND nodes, 80 words per group, field-major:
+   a[16], b[16], d[16] (byte offsets of slots of 256 bytes), ma[16], mb[16] (0 or ~0), entry j
+   computing d = (a ^ ma) & (b ^ mb); `count` real nodes per group (the rest read slot 0 and write
+   the last slot), input leaves (slot offset | input bit j << 24), the hit word's slot
+   (| complemented << 31). Free with gpu_tape_free. max_slots 0 = 640 (160 KB of LDS). */
 typedef struct {
     uint32_t *ops, *count, *leaves;
     uint32_t  ngroups, nleaves, root, slots;

@@ -630,16 +630,31 @@ bool lower_spec(const ts_spec *sp, SpecCode &sc, std::string &err)
     return true;
 }
 
+// 32 x 32 bit transpose of e0..e31 in place: stage j swaps the j-bit blocks of word pairs
+// (k, k + j). The 16- and 8-bit stages are byte shuffles, one v_perm_b32 per word (a byte
+// selector picks from {S0 = high word, S1 = low word}: 0-3 = bytes of S1, 4-7 = bytes of S0);
+// the 4-, 2- and 1-bit stages a shift and a bitwise select (v_bitop3_b32) per word.
 void emit_transpose(std::ostringstream &o, const char *e)
 {
     static const uint32_t masks[5] = {0x0000FFFFu, 0x00FF00FFu, 0x0F0F0F0Fu, 0x33333333u, 0x55555555u};
+    // j = 16: a' = [a0 a1 b0 b1], b' = [a2 a3 b2 b3]; j = 8: a' = [a0 b0 a2 b2], b' = [a1 b1 a3 b3]
+    static const uint32_t sel_lo[2] = {0x05040100u, 0x06020400u}, sel_hi[2] = {0x07060302u, 0x07030501u};
     int j = 16;
     for (int st = 0; st < 5; st++, j >>= 1)
-        for (int k = 0; k < 32; k++)
-            if (!(k & j))
-                o << "    { u32 t = ((" << e << k << " >> " << j << ") ^ " << e << k + j << ") & "
-                  << hex32(masks[st]) << "; " << e << k << " ^= t << " << j << "; " << e << k + j
-                  << " ^= t; }\n";
+        for (int k = 0; k < 32; k++) {
+            if (k & j)
+                continue;
+            const std::string a = e + std::to_string(k), b = e + std::to_string(k + j);
+            if (st < 2)
+                o << "    { const u32 a = __builtin_amdgcn_perm(" << b << ", " << a << ", "
+                  << hex32(sel_lo[st]) << "), b = __builtin_amdgcn_perm(" << b << ", " << a << ", "
+                  << hex32(sel_hi[st]) << "); " << a << " = a; " << b << " = b; }\n";
+            else  // bitop3 0xD8 = S2 ? S1 : S0 (a bitwise select)
+                o << "    { const u32 a = __builtin_amdgcn_bitop3_b32(" << a << ", " << b << " << " << j
+                  << ", " << hex32(masks[st] << j) << ", 0xD8), b = __builtin_amdgcn_bitop3_b32(" << b
+                  << ", " << a << " >> " << j << ", " << hex32(masks[st]) << ", 0xD8); " << a
+                  << " = a; " << b << " = b; }\n";
+        }
 }
 
 }  // namespace

@@ -158,8 +158,11 @@ void enum_outputs(const ts_program *a, const ts_program *b, const int *in_map, c
 // the last slot (the sink, read by nobody), so the kernel runs each group without branches.
 struct Tape {
     static const unsigned kGroup = 16;
-    static const unsigned kSlotBytes = 256;    // one 32-bit word for each of a wave's 64 lanes
-    std::vector<uint32_t> ops;     // kGroup x 3 words per group: a | ca << 31, b | cb << 31, d
+    static const unsigned kWords = 5 * kGroup;  // words per group
+    static const unsigned kSlotBytes = 256;     // one 32-bit word for each of a wave's 64 lanes
+    // per group, field-major: a[16], b[16], d[16] (slot byte offsets), ma[16], mb[16] (0 or ~0:
+    // complement masks of a and b); entry j computes d = (a ^ ma) & (b ^ mb)
+    std::vector<uint32_t> ops;
     std::vector<uint32_t> count;   // nodes in each group, 1..kGroup
     std::vector<uint32_t> leaves;  // input leaves: slot byte offset | input bit j << 24
     uint32_t root = 0;             // hit word: slot byte offset | complemented << 31

@@ -123,19 +123,23 @@ bool build_tape(const ts_program *a, const ts_program *b, const int *in_map, con
                 if (--uses[u] == 0 && u != 0)
                     free_slots.push_back((uint32_t)slot[u]);
         }
-        for (uint32_t v : group) {
-            const Node &nd = g.nodes[v];
-            slot[v] = (int32_t)alloc();
-            t.ops.push_back((uint32_t)slot[nd.a >> 1] * Tape::kSlotBytes | (nd.a & 1u) << 31);
-            t.ops.push_back((uint32_t)slot[nd.b >> 1] * Tape::kSlotBytes | (nd.b & 1u) << 31);
-            t.ops.push_back((uint32_t)slot[v] * Tape::kSlotBytes);
-        }
-        for (size_t k = group.size(); k < Tape::kGroup; k++) {
-            // padding: reads slot 0, writes the sink slot (numbered once the file size is known)
-            t.ops.push_back(0);
-            t.ops.push_back(0);
-            padding.push_back(t.ops.size());
-            t.ops.push_back(0);
+        const size_t base = t.ops.size();
+        t.ops.resize(base + Tape::kWords, 0);
+        for (size_t j = 0; j < Tape::kGroup; j++) {
+            uint32_t *e = &t.ops[base + j];
+            if (j < group.size()) {
+                const uint32_t v = group[j];
+                const Node &nd = g.nodes[v];
+                slot[v] = (int32_t)alloc();
+                e[0] = (uint32_t)slot[nd.a >> 1] * Tape::kSlotBytes;
+                e[Tape::kGroup] = (uint32_t)slot[nd.b >> 1] * Tape::kSlotBytes;
+                e[2 * Tape::kGroup] = (uint32_t)slot[v] * Tape::kSlotBytes;
+                e[3 * Tape::kGroup] = 0u - (nd.a & 1u);
+                e[4 * Tape::kGroup] = 0u - (nd.b & 1u);
+            } else {
+                // padding: reads slot 0, writes the sink slot (numbered once the file size is known)
+                padding.push_back(base + 2 * Tape::kGroup + j);
+            }
         }
         t.count.push_back((uint32_t)group.size());
         scheduled += group.size();

@@ -4,7 +4,8 @@
 //
 // Layout: a workgroup is one wave of 64 lanes; lane l owns the 32-bit word at byte
 // slot * 256 + 4 l of the workgroup's LDS file, so every slot access of a wave is one conflict-free
-// 256-byte row. Each lane enumerates one word (32 assignments) at a time exactly like the compiled
+// 256-byte row. A group's 16 entries are described by 80 words (the tape format in lower.hpp),
+// loaded into VGPRs one group ahead. Each lane enumerates one word (32 assignments) at a time exactly like the compiled
 // ts_enum kernel — same word order, same share rule, same valid mask, same hit list of
 // (word, mask) pairs — so the host's collection of hits is shared by both tiers.
 #include <hip/hip_runtime.h>
@@ -27,14 +28,49 @@ __device__ inline uint32_t lane_pattern(uint32_t j)
                    : 0xFFFF0000u;
 }
 
+// The group's 80 descriptor words as 20 16-byte vector loads. `zero` is 0 at run time; adding
+// (lane & zero) makes the address per-lane for the compiler, so the words arrive through vector
+// memory (counted by vmcnt) rather than scalar loads, whose counter is shared with LDS reads and
+// would make every LDS wait also wait for the next group's descriptors.
+__device__ inline void load_group(uint4 (&d)[20], const uint4 *__restrict__ ops, uint32_t g,
+                                  uint32_t lane_zero)
+{
+    const uint4 *p = ops + (size_t)g * 20 + lane_zero;
+#pragma unroll
+    for (int k = 0; k < 20; k++)
+        d[k] = p[k];
+}
+
+__device__ inline uint32_t word(const uint4 (&d)[20], unsigned w)
+{
+    const uint4 &q = d[w >> 2];
+    return (w & 3) == 0 ? q.x : (w & 3) == 1 ? q.y : (w & 3) == 2 ? q.z : q.w;
+}
+
+// one group: every read, then every write (entries of a group are independent)
+__device__ inline void run_group(const uint4 (&d)[20], char *my)
+{
+    uint32_t v[kGroup];
+#pragma unroll
+    for (unsigned j = 0; j < kGroup; j++) {
+        const uint32_t x = *(const uint32_t *)(my + word(d, j)) ^ word(d, 3 * kGroup + j);
+        const uint32_t y = *(const uint32_t *)(my + word(d, kGroup + j)) ^ word(d, 4 * kGroup + j);
+        v[j] = x & y;
+    }
+#pragma unroll
+    for (unsigned j = 0; j < kGroup; j++)
+        *(uint32_t *)(my + word(d, 2 * kGroup + j)) = v[j];
+}
+
 __global__ void __launch_bounds__(64)
-    ts_enum_tape(const uint32_t *__restrict__ ops, uint32_t ngroups, const uint32_t *__restrict__ leaves, uint32_t nleaves,
-                 uint32_t root, uint64_t *__restrict__ out, uint64_t *__restrict__ list,
-                 uint64_t cap, uint64_t i0, uint64_t i1, uint64_t space, uint32_t parts,
-                 uint32_t part, uint32_t nshare)
+    ts_enum_tape(const uint4 *__restrict__ ops, uint32_t ngroups, const uint32_t *__restrict__ leaves,
+                 uint32_t nleaves, uint32_t root, uint32_t zero, uint64_t *__restrict__ out,
+                 uint64_t *__restrict__ list, uint64_t cap, uint64_t i0, uint64_t i1, uint64_t space,
+                 uint32_t parts, uint32_t part, uint32_t nshare)
 {
     extern __shared__ uint32_t file[];
     char *const my = (char *)file + threadIdx.x * 4u;
+    const uint32_t lane_zero = threadIdx.x & zero;
     *(uint32_t *)my = 0u;  // slot 0: the constant
     for (uint64_t i = i0 + (uint64_t)blockIdx.x * 64u + threadIdx.x; i < i1;
          i += (uint64_t)gridDim.x * 64u) {
@@ -52,21 +88,19 @@ __global__ void __launch_bounds__(64)
             *(uint32_t *)(my + (lf & 0xffffffu)) =
                 j < 5u ? lane_pattern(j) : 0u - (uint32_t)((gw >> (j - 5u)) & 1ull);
         }
-        // every group has kGroup entries (short ones padded onto the sink slot): all reads of
-        // the group are issued before the first write, with no branch between them
-        for (uint32_t g = 0; g < ngroups; g++) {
-            const uint32_t *op = ops + (size_t)g * (3 * kGroup);
-            uint32_t v[kGroup];
-#pragma unroll
-            for (unsigned j = 0; j < kGroup; j++) {
-                const uint32_t a = op[3 * j], b = op[3 * j + 1];
-                const uint32_t x = *(const uint32_t *)(my + (a & 0x7fffffffu)) ^ (0u - (a >> 31));
-                const uint32_t y = *(const uint32_t *)(my + (b & 0x7fffffffu)) ^ (0u - (b >> 31));
-                v[j] = x & y;
+        // two descriptor buffers: the next group's loads are in flight while this one runs
+        uint4 da[20], db[20];
+        if (ngroups)
+            load_group(da, ops, 0, lane_zero);
+        for (uint32_t g = 0; g < ngroups; g += 2) {
+            if (g + 1 < ngroups)
+                load_group(db, ops, g + 1, lane_zero);
+            run_group(da, my);
+            if (g + 1 < ngroups) {
+                if (g + 2 < ngroups)
+                    load_group(da, ops, g + 2, lane_zero);
+                run_group(db, my);
             }
-#pragma unroll
-            for (unsigned j = 0; j < kGroup; j++)
-                *(uint32_t *)(my + op[3 * j + 2]) = v[j];
         }
         const uint32_t hit =
             (*(const uint32_t *)(my + (root & 0x7fffffffu)) ^ (0u - (root >> 31))) & valid;
@@ -113,9 +147,10 @@ bool tape_launch(const TapeDev &t, hipStream_t stream, unsigned grid, uint64_t *
                  uint64_t *list, uint64_t cap, uint64_t i0, uint64_t i1, uint64_t space,
                  uint32_t parts, uint32_t part, uint32_t nshare)
 {
-    ts_enum_tape<<<grid, 64, tape_lds_bytes(t), stream>>>(t.ops, t.ngroups, t.leaves,
-                                                           t.nleaves, t.root, out, list, cap, i0,
-                                                           i1, space, parts, part, nshare);
+    ts_enum_tape<<<grid, 64, tape_lds_bytes(t), stream>>>((const uint4 *)t.ops, t.ngroups,
+                                                           t.leaves, t.nleaves, t.root, 0u, out,
+                                                           list, cap, i0, i1, space, parts, part,
+                                                           nshare);
     return hipGetLastError() == hipSuccess;
 }
 

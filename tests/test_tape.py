@@ -23,14 +23,18 @@ def run_tape(tp, n):
         v = f[(int(ref) & 0x7FFFFFFF) // 256]
         return ~v if int(ref) >> 31 else v
 
+    def entry(off, mask):
+        v = f[int(off) // 256]
+        return ~v if int(mask) else v
+
     for lf in tp["leaves"]:
         slot, j = (int(lf) & 0xFFFFFF) // 256, int(lf) >> 24
         f[slot] = PATTERNS[j] if j < 5 else (np.uint64(0) - ((gw >> np.uint64(j - 5)) & np.uint64(1))).astype(np.uint32)
     for g in range(len(tp["count"])):
-        ops = tp["ops"][g, :tp["count"][g]]
-        vals = [read(a) & read(b) for a, b, _ in ops]   # every read of the group first
-        for (_, _, d), v in zip(ops, vals):
-            f[int(d) // 256] = v
+        a, b, d, ma, mb = tp["ops"][g]  # all 16 entries, padding included
+        vals = [entry(a[j], ma[j]) & entry(b[j], mb[j]) for j in range(16)]  # every read first
+        for j in range(16):
+            f[int(d[j]) // 256] = vals[j]
     hit = read(tp["root"]).copy()
     rem = space - 32 * np.arange(nwords, dtype=np.int64)
     valid = np.where(rem >= 32, 0xFFFFFFFF, (1 << np.minimum(rem, 31)) - 1).astype(np.uint32)
@@ -57,6 +61,11 @@ def test_tape_equiv_matches_oracle():
         tp = t.enum_tape(t.EN_EQUIV, a, b)
         assert tp["slots"] <= 640
         assert all(1 <= c_ <= 16 for c_ in tp["count"])
+        # every offset names a slot of the file; padding writes only the last slot (the sink)
+        assert int(tp["ops"][:, :3].max(initial=0)) < 256 * tp["slots"]
+        for g, c_ in enumerate(tp["count"]):
+            assert (tp["ops"][g, 2, c_:] == 256 * (tp["slots"] - 1)).all()
+            assert (tp["ops"][g, 2, :c_] < 256 * (tp["slots"] - 1)).all()
         got = run_tape(tp, n_in)
         o = O.enum(t.EN_EQUIV, a, b, b_in=b.inputs, b_out=b.outputs)
         assert got == (o["hits"], o["hit"]), (k, n_in)

@@ -165,6 +165,63 @@ def test_netlist_errors_and_dff():
     assert p.len > 0
 
 
+# ISCAS-85 c17: 5 inputs, 2 outputs, 6 NAND gates (the public benchmark circuit)
+C17 = """INPUT(1)
+INPUT(2)
+INPUT(3)
+INPUT(6)
+INPUT(7)
+OUTPUT(22)
+OUTPUT(23)
+10 = NAND(1, 3)
+11 = NAND(3, 6)
+16 = NAND(2, 11)
+19 = NAND(11, 7)
+22 = NAND(10, 16)
+23 = NAND(16, 19)
+"""
+
+
+def _ref_net(ref, text):
+    p = ref.Prog()
+    e = C.create_string_buffer(512)
+    assert ref.lib().net_parse(text.encode(), b"net", p.ptr, e, len(e)), e.value
+    return p
+
+
+def test_c17_program():
+    # 6 NANDs of 5 instructions each (ROADMAP.md:299: c17.bench = 30 instructions)
+    p = t.net_parse(C17)
+    assert p.len == 30 and len(p.inputs) == 5 and len(p.outputs) == 2
+    # the circuit's function: all 32 assignments in lanes 0..31 of one group on the oracle machine
+    mem, rr = O.reset_lines(p, 1)
+    for k, cell in enumerate(p.inputs):
+        mem[0, cell, 0] = sum(((x >> (4 - k)) & 1) << x for x in range(32))
+    mem, rr = O.run_lines(p, mem, rr, 1)
+    for x in range(32):
+        a = [(x >> (4 - k)) & 1 for k in range(5)]  # inputs 1, 2, 3, 6, 7
+        n10, n11 = 1 - (a[0] & a[2]), 1 - (a[2] & a[3])
+        n16, n19 = 1 - (a[1] & n11), 1 - (n11 & a[4])
+        want = [1 - (n10 & n16), 1 - (n16 & n19)]
+        assert [int(mem[0, c, 0]) >> x & 1 for c in p.outputs] == want
+
+
+def test_netlists_same_program_as_reference(ref):
+    from tesseract_amd import netlists as N
+    texts = [C17, "INPUT(a)\nOUTPUT(q)\nq = DFF(d)\nd = XOR(a, q)\n",
+             "INPUT(x)\nINPUT(y)\nOUTPUT(o)\nOUTPUT(p)\no = NOR(x, y)\np = XNOR(o, y)\n"]
+    rng = np.random.default_rng(3)
+    for k in range(4):
+        c = N.circuit(int(rng.integers(4, 20)), int(rng.integers(1, 9)), int(rng.integers(20, 600)), rng)
+        texts += [c.bench(), N.rewrite(c, rng).bench()]
+    for text in texts:
+        ours, theirs = t.net_parse(text), _ref_net(ref, text)
+        oa, aa = ours.code()
+        ob, ab = theirs.code()
+        assert ours.len == theirs.s.len and (oa == ob).all() and (aa == ab).all()
+        assert [ours.name(i) for i in range(ours.ncells)] == theirs.names()
+
+
 def test_generated_kernels_compile_for_gfx950():
     # hipRTC needs no GPU: every kernel form the engine emits must compile for gfx950
     progs = [t.gen_adder(8), t.gen_rank(12), t.gen_perm8(), t.random_program(30, 300, 1)]

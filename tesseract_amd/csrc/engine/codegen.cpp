@@ -6,6 +6,9 @@
 // leave the LUT network (stores, cross-lane moves) are kept in the polarity most of their uses
 // want, so a NAND result stored to a cell costs no separate NOT.
 #include <algorithm>
+#include <array>
+#include <functional>
+#include <map>
 #include <cstdio>
 #include <cstring>
 #include <sstream>
@@ -201,18 +204,106 @@ static void map_luts(const Aig &g, const std::vector<Lit> &roots, bool lut3, Map
         cuts[n].push_back(trivial);  // the node itself as a leaf of a bigger cut
     }
 
-    // cover from the roots down
+    // Area recovery with exact local area (reference-counted cover): refs[n] = LUTs of the cover
+    // that read n, plus one for a forced node. Dereferencing a LUT's cut frees every LUT only it
+    // kept alive; the exact area of a cut is the number of LUTs its selection adds. Each cover node
+    // in topological order takes the cut of least exact area among its kept cuts; two passes.
+    std::vector<uint32_t> refs(N, 0);
+    std::function<uint32_t(uint32_t, const Cut &)> ref_cut, deref_cut;
+    auto is_and = [&](uint32_t l) { return g.nodes[l].kind == N_AND && !forced[l]; };
+    ref_cut = [&](uint32_t n, const Cut &c) -> uint32_t {
+        uint32_t area = 1;
+        for (int i = 0; i < c.n; i++)
+            if (c.leaf[i] != n && is_and(c.leaf[i]) && refs[c.leaf[i]]++ == 0)
+                area += ref_cut(c.leaf[i], m.best[c.leaf[i]]);
+        return area;
+    };
+    deref_cut = [&](uint32_t n, const Cut &c) -> uint32_t {
+        uint32_t area = 1;
+        for (int i = 0; i < c.n; i++)
+            if (c.leaf[i] != n && is_and(c.leaf[i]) && --refs[c.leaf[i]] == 0)
+                area += deref_cut(c.leaf[i], m.best[c.leaf[i]]);
+        return area;
+    };
     for (size_t n = 0; n < N; n++)
-        if (m.in_cone[n] && forced[n] && g.nodes[n].kind == N_AND)
-            m.is_lut[n] = 1;
-    for (size_t i = N; i-- > 0;) {
-        if (!m.is_lut[i])
-            continue;
-        const Cut &c = m.best[i];
-        for (int k = 0; k < c.n; k++)
-            if (g.nodes[c.leaf[k]].kind == N_AND)
-                m.is_lut[c.leaf[k]] = 1;
+        if (m.in_cone[n] && forced[n] && g.nodes[n].kind == N_AND) {
+            refs[n] = 1;
+            ref_cut((uint32_t)n, m.best[n]);
+        }
+    for (int pass = 0; lut3 && pass < 2; pass++)
+        for (size_t n = 0; n < N; n++) {
+            if (!m.in_cone[n] || g.nodes[n].kind != N_AND || !refs[n])
+                continue;
+            deref_cut((uint32_t)n, m.best[n]);
+            uint32_t best_area = UINT32_MAX;
+            size_t best_k = 0;
+            for (size_t k = 0; k < cuts[n].size(); k++) {
+                const Cut &c = cuts[n][k];
+                if (c.n == 1 && c.leaf[0] == n)
+                    continue;  // the trivial cut
+                uint32_t a = ref_cut((uint32_t)n, c);
+                deref_cut((uint32_t)n, c);
+                if (a < best_area) {
+                    best_area = a;
+                    best_k = k;
+                }
+            }
+            m.best[n] = cuts[n][best_k];
+            ref_cut((uint32_t)n, m.best[n]);
+        }
+
+    // Joint moves for pairs of cover nodes that read the same leaves (a full adder's sum and
+    // carry over the two halves of an inner XOR): moving one alone never pays, since the other
+    // keeps the shared leaves alive, but moving both onto another cut they both have can free
+    // those leaves.
+    if (lut3) {
+        std::map<std::array<uint32_t, 4>, std::vector<uint32_t>> by_leaves;
+        for (size_t n = 0; n < N; n++)
+            if (m.in_cone[n] && g.nodes[n].kind == N_AND && refs[n] && m.best[n].n == 3)
+                by_leaves[{m.best[n].leaf[0], m.best[n].leaf[1], m.best[n].leaf[2], 3}].push_back(
+                    (uint32_t)n);
+        for (auto &kv : by_leaves) {
+            const std::vector<uint32_t> &grp = kv.second;
+            for (size_t i = 0; i + 1 < grp.size(); i += 2) {
+                const uint32_t u = grp[i], v = grp[i + 1];
+                if (!refs[u] || !refs[v] || !same_cut(m.best[u], m.best[v]))
+                    continue;
+                deref_cut(u, m.best[u]);
+                deref_cut(v, m.best[v]);
+                uint32_t cur = ref_cut(u, m.best[u]) + ref_cut(v, m.best[v]);
+                deref_cut(v, m.best[v]);
+                deref_cut(u, m.best[u]);
+                const Cut *pick = nullptr;
+                for (const Cut &c : cuts[u]) {
+                    if (c.n == 1 && c.leaf[0] == u)
+                        continue;
+                    bool in_v = false;
+                    for (const Cut &d : cuts[v])
+                        in_v |= same_cut(c, d);
+                    if (!in_v || (c.n == 1 && c.leaf[0] == v))
+                        continue;
+                    uint32_t a = ref_cut(u, c) + ref_cut(v, c);
+                    deref_cut(v, c);
+                    deref_cut(u, c);
+                    if (a < cur) {
+                        cur = a;
+                        pick = &c;
+                    }
+                }
+                if (pick) {
+                    m.best[u] = *pick;
+                    m.best[v] = *pick;
+                }
+                ref_cut(u, m.best[u]);
+                ref_cut(v, m.best[v]);
+            }
+        }
     }
+
+    // the cover: forced nodes and every AND node a chosen cut reads
+    for (size_t n = 0; n < N; n++)
+        if (m.in_cone[n] && g.nodes[n].kind == N_AND && (forced[n] || refs[n]))
+            m.is_lut[n] = 1;
 
     // polarity: uses outside the LUT network (roots and PERM operands) decide
     std::vector<int> pos(N, 0), neg(N, 0);

@@ -27,6 +27,7 @@ struct gpu_code {
     hipModule_t            mod = nullptr;
     std::vector<hipFunction_t> fns;
     std::vector<uint32_t>  slices;      // per segment
+    std::vector<uint32_t>  always;      // per segment: slices every pass runs (the rest: last pass)
     gpu_code_info          info{};
     gpu_tune               tune{};
     uint32_t               ncells = 0;
@@ -300,8 +301,10 @@ extern "C" int gpu_compile(gpu_engine *e, const ts_program *p, const gpu_tune *t
         HIPCHK(hipModuleGetFunction(&f, c->mod, name.c_str()));
         c->fns.push_back(f);
     }
-    for (const Segment &s : c->lw.segs)
+    for (const Segment &s : c->lw.segs) {
         c->slices.push_back((uint32_t)s.slices.size());
+        c->always.push_back((uint32_t)s.always);
+    }
     fill_info(c->lw, ks, c->info);
     c->info.compile_seconds = secs;
     *out = c.release();
@@ -528,14 +531,17 @@ extern "C" int gpu_run(gpu_code *c, gpu_state *s, uint64_t passes, char *err, si
             left -= now;
         }
     } else {
-        // one pass: every segment's launch, parity flipping after the pass
-        auto one_pass = [&](int latest) -> int {
+        // one pass: every segment's launch, parity flipping after the pass. Only the last pass of
+        // the call (`full`) computes the cells no later pass reads before writing them (the
+        // caller may read them afterwards); the others run each segment's `always` slices.
+        auto one_pass = [&](int latest, bool full) -> int {
             uint32_t *prev = s->slots + (size_t)latest * lw.nslots * nw;
             uint32_t *cur = s->slots + (size_t)(1 - latest) * lw.nslots * nw;
             for (size_t k = 0; k < c->fns.size(); k++) {
-                if (!c->slices[k])
+                const uint32_t nslices = full ? c->slices[k] : c->always[k];
+                if (!nslices)
                     continue;
-                uint64_t threads = (uint64_t)c->slices[k] * nw;
+                uint64_t threads = (uint64_t)nslices * nw;
                 void *args[] = {&prev, &cur, &s->init, &nw};
                 if (!launch(e, c->fns[k], (unsigned)((threads + block - 1) / block), block, args, err,
                             errlen))
@@ -543,16 +549,17 @@ extern "C" int gpu_run(gpu_code *c, gpu_state *s, uint64_t passes, char *err, si
             }
             return 1;
         };
+        // captured graphs hold passes before the last one; the last pass is launched directly
         const unsigned gp = c->tune.graph_passes;
         uint64_t p = 0;
-        if (gp > 1 && passes >= gp) {
+        if (gp > 1 && passes > gp) {
             if (!s->graph[s->latest]) {
                 // capture gp passes starting from this parity (kernel launches only)
                 hipGraph_t g;
                 HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
                 int lat = s->latest, ok = 1;
                 for (unsigned q = 0; q < gp && ok; q++, lat = 1 - lat)
-                    ok = one_pass(lat);
+                    ok = one_pass(lat, false);
                 hipError_t ce = hipStreamEndCapture(e->stream, &g);
                 if (!ok || ce != hipSuccess) {
                     say(err, errlen, "graph capture failed: %s", hipGetErrorString(ce));
@@ -565,16 +572,16 @@ extern "C" int gpu_run(gpu_code *c, gpu_state *s, uint64_t passes, char *err, si
                     return 0;
                 }
             }
-            for (; p + gp <= passes; p += gp) {
+            for (; p + gp < passes; p += gp) {
                 HIPCHK(hipGraphLaunch(s->graph[s->latest], e->stream));
                 if (gp & 1)
                     s->latest = 1 - s->latest;
-                if (!s->graph[s->latest] && p + 2 * gp <= passes)
+                if (!s->graph[s->latest] && p + 2 * gp < passes)
                     break;  // an odd pass count changed the parity: capture that one next call
             }
         }
         for (; p < passes; p++) {
-            if (!one_pass(s->latest))
+            if (!one_pass(s->latest, p + 1 == passes))
                 return 0;
             s->latest = 1 - s->latest;
         }

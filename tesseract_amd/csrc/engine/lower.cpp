@@ -228,13 +228,23 @@ static void cone(const Aig &g, const std::vector<Lit> &roots, std::vector<uint8_
 
 // Splits a segment's stored outputs into slices of about `target` AIG nodes each, in output order
 // (program order keeps neighbouring cells, which share inputs, together).
+static void slice_outputs(Segment &s, std::vector<uint32_t> stored, unsigned target);
+
+// Slices of the outputs every pass stores, then slices of the final_only outputs.
 static void make_slices(Segment &s, unsigned target)
 {
-    std::vector<uint32_t> stored;
+    std::vector<uint32_t> always, last;
     for (uint32_t i = 0; i < s.outs.size(); i++)
         if (s.outs[i].slot >= 0)
-            stored.push_back(i);
+            (s.outs[i].final_only ? last : always).push_back(i);
     s.slices.clear();
+    slice_outputs(s, always, target);
+    s.always = s.slices.size();
+    slice_outputs(s, last, target);
+}
+
+static void slice_outputs(Segment &s, std::vector<uint32_t> stored, unsigned target)
+{
     if (stored.empty())
         return;
     if (!target) {
@@ -365,10 +375,27 @@ bool lower_program(const ts_program *p, const LowerOptions &opt, Lowered &lw, st
             }
         }
     }
+    // a cell is upward exposed when a pass reads its start-of-pass value; a version is read later
+    // in the pass when some segment loads it
+    std::vector<uint8_t> upward(total, 0);
+    std::vector<std::vector<uint8_t>> read_later(total);
+    for (const Version &ver : lw.versions) {
+        if (ver.seg < 0) {
+            upward[ver.cell] = 1;
+        } else {
+            auto &r = read_later[ver.cell];
+            if (r.size() <= (size_t)ver.seg)
+                r.resize((size_t)ver.seg + 1, 0);
+            r[(size_t)ver.seg] = 1;
+        }
+    }
     for (size_t s = 0; s < lw.segs.size(); s++)
         for (SegOut &o : lw.segs[s].outs) {
             auto &v = slot_of[o.cell];
             o.slot = s < v.size() ? v[s] : -1;
+            const auto &r = read_later[o.cell];
+            const bool later = s < r.size() && r[s];
+            o.final_only = o.slot >= 0 && (int)s == lw.last_seg[o.cell] && !upward[o.cell] && !later;
         }
 
     // resident: one segment and every read of a start-of-pass value inside the wave

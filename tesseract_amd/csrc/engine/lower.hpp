@@ -73,13 +73,18 @@ struct SegOut {
     uint32_t cell;  // ncells = RR
     Lit      v;
     int32_t  slot;  // storage slot, -1 if nobody reads this version
+    // only the caller can read it: the cell's value after the pass, which no later segment reads
+    // and which the next pass writes before reading (needed after the last pass of a run only)
+    bool     final_only = false;
 };
 
 struct Segment {
     Aig                 g;
     std::vector<SegOut> outs;
-    // slices: output index ranges, each computed by its own threads (cones duplicated)
+    // slices: output index ranges, each computed by its own threads (cones duplicated); slices
+    // [0, always) hold the outputs every pass needs, the rest only final_only outputs
     std::vector<std::vector<uint32_t>> slices;
+    size_t always = 0;
 };
 
 // Where a version lives at run time.

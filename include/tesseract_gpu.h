@@ -115,8 +115,9 @@ void gpu_enum_free(gpu_enum_code *c);
 
 /* Batched cycle search: many link-free programs, each on `groups` x 512 machines started from the
    reference's t1_start stream of its seed (group 0 exactly t1_start, src/tester.c:83-91; later
-   groups continue the stream), every machine running `max_passes` passes, its cycle length found
-   with the reference's Brent schedule (src/tester.c:96-164). The counterpart of tester 1's worker
+   groups continue the stream), each machine's cycle length found with the reference's Brent
+   schedule (src/tester.c:96-164); a group stops once all its machines found theirs, or after
+   `max_passes` passes, as the reference's does. The counterpart of tester 1's worker
    pool (src/tester.c:182-279): programs are data for a persistent interpreter kernel fed by a device
    work queue, not compiled one by one. stats[i] is program i's histogram (bucket b: cycle length
    in (2^(b-1), 2^b], src/tester.c:42-48; bucket 63: none within max_passes). plain = 1 runs the same
@@ -130,6 +131,7 @@ typedef struct {
     uint64_t hist[64];
     uint64_t lam_sum;   /* sum of the cycle lengths found */
     uint64_t longest;
+    uint64_t passes;    /* passes run, summed over the groups (the reference's T1Size.passes) */
 } gpu_batch_stats;
 
 typedef struct {

@@ -91,12 +91,15 @@ def enum(mode, a, b=None, b_in=None, b_out=None, out_cells=None, expect=None, ta
     return {"space": r.space, "tried": r.tried, "hits": r.hits, "hit": list(r.hit[:r.nhit]), "nhit": r.nhit}
 
 
-def brent_lines(prog, mem, rr, max_passes):
+def brent_lines(prog, mem, rr, max_passes, with_passes=False):
+    """Cycle length per lane of one 512-machine group (0 = none within max_passes); with
+    with_passes also the passes the group ran (the reference's t1_cycles return value)."""
     lam = np.zeros(512, np.uint64)
     mem = np.ascontiguousarray(mem, dtype=np.uint64)
     rr = np.ascontiguousarray(rr, dtype=np.uint64)
-    lib().oracle_brent_lines(_addr(prog), mem.ctypes.data, rr.ctypes.data, max_passes, lam.ctypes.data)
-    return lam
+    passes = lib().oracle_brent_lines(_addr(prog), mem.ctypes.data, rr.ctypes.data, max_passes,
+                                      lam.ctypes.data)
+    return (lam, int(passes)) if with_passes else lam
 
 
 def exact_cycle(prog, mem, rr, lane):

@@ -61,7 +61,8 @@ class GpuBatchItem(C.Structure):
 
 
 class GpuBatchStats(C.Structure):
-    _fields_ = [("hist", C.c_uint64 * 64), ("lam_sum", C.c_uint64), ("longest", C.c_uint64)]
+    _fields_ = [("hist", C.c_uint64 * 64), ("lam_sum", C.c_uint64), ("longest", C.c_uint64),
+                ("passes", C.c_uint64)]
 
 
 class GpuBatchTune(C.Structure):
@@ -573,7 +574,7 @@ class Engine:
 
     def batch_run(self, progs, seeds, groups=2048, max_passes=64, plain=False, lds_cells=0,
                   waves_per_cu=0):
-        """gpu_batch_run: per program a dict(hist, lam_sum, longest); plus the run info."""
+        """gpu_batch_run: per program a dict(hist, lam_sum, longest, passes); plus the run info."""
         n = len(progs)
         items = (GpuBatchItem * n)()
         for i, (p, sd) in enumerate(zip(progs, seeds)):
@@ -585,7 +586,8 @@ class Engine:
         e = _err()
         if not lib().gpu_batch_run(self._e, items, n, C.byref(tune), stats, C.byref(info), e, len(e)):
             raise RuntimeError("gpu_batch_run: " + e.value.decode())
-        out = [{"hist": list(st.hist), "lam_sum": st.lam_sum, "longest": st.longest} for st in stats]
+        out = [{"hist": list(st.hist), "lam_sum": st.lam_sum, "longest": st.longest,
+                "passes": st.passes} for st in stats]
         return out, {"ms": info.ms, "ops": info.ops, "launches": info.launches,
                      "class_ms": list(info.class_ms), "class_ops": list(info.class_ops)}
 

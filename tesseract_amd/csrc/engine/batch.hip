@@ -15,9 +15,11 @@
 // A work item is 64 words (2,048 machines) of one program: one wave. The machines' cells live in
 // LDS (64 lanes x cells x 4 bytes, hare and tortoise) for small programs, or in a per-wave region
 // of HBM for large ones (coalesced: 256 contiguous bytes per cell). Start states come from the
-// reference's t1_start stream, generated in registers. Every machine runs exactly max_passes
-// passes; Brent's schedule gives each machine's cycle length, and per-program histograms are
-// reduced per wave before one atomic per bucket.
+// reference's t1_start stream, generated in registers. Brent's schedule gives each machine's
+// cycle length; a 512-machine group stops once all its machines found theirs or after max_passes
+// passes (the reference's t1_cycles stops a group the same way, src/tester.c:131-144), a wave
+// once its four groups have; per-program histograms and the groups' pass counts are reduced per
+// wave before one atomic each.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,7 +36,7 @@ namespace {
 
 constexpr unsigned kWin = 8;          // instructions per window (two 16-byte records)
 constexpr unsigned kLanes = 64;       // words per work item (one wave)
-constexpr unsigned kStats = 66;       // hist[64], lam_sum, longest
+constexpr unsigned kStats = 67;       // hist[64], lam_sum, longest, passes (sum over groups)
 constexpr unsigned kSample = 64;      // cells compared before a full comparison (reference: 64)
 
 struct BatchProg {
@@ -130,7 +132,8 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
                                                const uint32_t *__restrict__ samples,
                                                uint32_t *__restrict__ queue,
                                                uint32_t *__restrict__ scratch, uint32_t region_cells,
-                                               uint64_t *__restrict__ stats, uint32_t max_passes)
+                                               uint64_t *__restrict__ stats, uint32_t max_passes,
+                                               uint32_t prog_words)
 {
     extern __shared__ uint32_t lds[];
     const uint32_t lane = threadIdx.x;
@@ -155,6 +158,8 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
         // start state: group g = word / 16, u32 word j of its line; the stream continues group
         // after group, cells 2.. then RR, 8 u64 words each (reference src/tester.c:83-91)
         const uint32_t word = (item - P.first_item) * kLanes + lane;
+        // a program of fewer than 4 groups fills part of its last item: the other lanes run a
+        // throwaway machine and record nothing
         const uint64_t g = word >> 4;
         const uint32_t w64 = (word & 15) >> 1, hi = word & 1;
         const uint64_t s0 = P.seed ^ 0x5DEECE66Dull, per_group = (uint64_t)(ncells - 1) * 8;
@@ -178,8 +183,15 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
         uint32_t trr = rr;
         run_pass(hare, lane, rec, P.nrec, rr);
         uint64_t lam = 1, power = 1, steps = 1;
-        uint32_t active = 0xFFFFFFFFu;
+        uint32_t active = word < prog_words ? 0xFFFFFFFFu : 0u;
         const uint32_t *smp = samples + P.sample_off;
+        // the wave's 4 groups (16 lanes each) that hold machines of the program, and those still
+        // searching; a group's passes are the steps at which its last machine found its cycle
+        const uint64_t real = __ballot(word < prog_words);
+        uint32_t open_groups = 0;
+        for (int k = 0; k < 4; k++)
+            open_groups |= ((real >> (16 * k)) & 0xFFFFull) ? 1u << k : 0u;
+        uint64_t passes = 0;
         for (;;) {
             uint32_t d = rr ^ trr;
             for (uint32_t i = 0; i < P.nsample; i++) {
@@ -195,8 +207,18 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
             if (__any(found))
                 record(st, found, lam, lane);
             active &= d;
-            if (steps >= max_passes)
+            const uint64_t searching = __ballot(active != 0);
+            for (int k = 0; k < 4; k++)
+                if ((open_groups >> k & 1) && !((searching >> (16 * k)) & 0xFFFFull)) {
+                    open_groups &= ~(1u << k);
+                    passes += steps;
+                }
+            if (!open_groups)
                 break;
+            if (steps >= max_passes) {
+                passes += steps * (uint64_t)__builtin_popcount(open_groups);
+                break;
+            }
             if (power == lam) {
                 for (uint32_t c = 2; c < ncells; c++)
                     tort.st(c, lane, hare.ld(c, lane));
@@ -211,6 +233,8 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
         uint32_t left = wave_sum((uint32_t)__builtin_popcount(active));
         if (left && lane == 0)
             atomicAdd((unsigned long long *)&st[63], (unsigned long long)left);
+        if (lane == 0)
+            atomicAdd((unsigned long long *)&st[66], (unsigned long long)passes);
         __syncthreads();  // one wave per block: orders this item's LDS use before the next
     }
 }
@@ -319,7 +343,8 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
             }
     }
     BCHK(hipSetDevice(e->device));
-    const uint32_t items_per_prog = tune.groups * 16 / kLanes;
+    const uint32_t prog_words = tune.groups * 16;
+    const uint32_t items_per_prog = (prog_words + kLanes - 1) / kLanes;
 
     // host encoding, by residency class: 0 = LDS (<= 32 cells with the sink), 1 = LDS (<= lds_cells),
     // 2 = HBM; each class is one persistent launch sized by what a CU can hold
@@ -401,11 +426,11 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
             const size_t lds_bytes = (size_t)maxcells[cls] * kLanes * 4;
             hipLaunchKernelGGL(ts_batch<true>, dim3(grid), dim3(kLanes), lds_bytes, st, d_progs.p,
                                ip, first, count, d_recs.p, d_samples.p, queue, sc, maxcells[cls],
-                               d_stats.p, tune.max_passes);
+                               d_stats.p, tune.max_passes, prog_words);
         } else {
             hipLaunchKernelGGL(ts_batch<false>, dim3(grid), dim3(kLanes), 0, st, d_progs.p, ip,
                                first, count, d_recs.p, d_samples.p, queue, sc, maxcells[cls],
-                               d_stats.p, tune.max_passes);
+                               d_stats.p, tune.max_passes, prog_words);
         }
         launches++;
         return hipGetLastError() == hipSuccess;
@@ -464,7 +489,12 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
         memcpy(stats_out[i].hist, &st[i * kStats], 64 * 8);
         stats_out[i].lam_sum = st[i * kStats + 64];
         stats_out[i].longest = st[i * kStats + 65];
+        stats_out[i].passes = st[i * kStats + 66];
     }
+    // executed work, counted as the reference counts it (src/tester.c:305): passes x len x 512
+    ops = 0;
+    for (size_t i = 0; i < n; i++)
+        ops += st[i * kStats + 66] * items[i].prog->len * 512ull;
     if (info && !tune.plain)
         for (int k = 0; k < 3; k++) {
             float cms = 0;
@@ -474,8 +504,7 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
         }
     if (info) {
         for (size_t i = 0; i < n; i++)
-            info->class_ops[class_of(items[i].prog)] +=
-                (uint64_t)items[i].prog->len * tune.max_passes * tune.groups * 512ull;
+            info->class_ops[class_of(items[i].prog)] += st[i * kStats + 66] * items[i].prog->len * 512ull;
         info->ms = ms;
         info->ops = ops;
         info->launches = launches;

@@ -20,9 +20,11 @@ def bucket(lam):
 def expected(prog, seed, groups, max_passes):
     mem, rr = t.t1_start(prog, seed, groups)
     hist = [0] * 64
-    total = longest = 0
+    total = longest = passes = 0
     for g in range(groups):
-        for lam in O.brent_lines(prog, mem[g], rr[g], max_passes):
+        lams, ran = O.brent_lines(prog, mem[g], rr[g], max_passes, with_passes=True)
+        passes += ran
+        for lam in lams:
             lam = int(lam)
             if lam == 0:
                 hist[63] += 1
@@ -30,7 +32,7 @@ def expected(prog, seed, groups, max_passes):
                 hist[bucket(lam)] += 1
                 total += lam
                 longest = max(longest, lam)
-    return {"hist": hist, "lam_sum": total, "longest": longest}
+    return {"hist": hist, "lam_sum": total, "longest": longest, "passes": passes}
 
 
 def programs():
@@ -52,7 +54,8 @@ def test_batch_matches_oracle(engine, plain):
     for (p, s), g in zip(progs, got):
         want = expected(p, s, 4, 64)
         assert g == want, (p.ncells, p.len)
-    assert info["ops"] == sum(p.len for p, _ in progs) * 64 * 2048
+    # executed work as the reference counts it: passes x len x 512 (src/tester.c:305)
+    assert info["ops"] == sum(p.len * g["passes"] * 512 for (p, _), g in zip(progs, got))
 
 
 def test_batch_long_cycles(engine):
@@ -62,3 +65,38 @@ def test_batch_long_cycles(engine):
                               max_passes=300)
     for (p, s), g in zip(progs, got):
         assert g == expected(p, s, 4, 300)
+
+
+def counter(bits):
+    """A bits-bit binary counter: every pass adds one, so every machine cycles with length
+    2^bits once the carry cells have been rewritten (a known answer for the cycle search)."""
+    b = t.Builder()
+    x = [b.cell("x%d" % i) for i in range(bits)]
+    c, tmp = b.cell("c"), b.cell("t")
+    b.gate(t.G_BUF, c, [t.CELL_ONE])
+    for xi in x:
+        b.gate(t.G_AND, tmp, [xi, c])
+        b.gate(t.G_XOR, xi, [xi, c])
+        b.gate(t.G_BUF, c, [tmp])
+    return b.finish()
+
+
+def test_batch_reference_scale_passes(engine):
+    # the reference's tester searches cycles over up to 2^24 passes: 2^20 here, with counters
+    # whose cycle (2^12, 2^18) is found and one (2^21) that is not. The 2^12 counter is checked
+    # against the oracle's Brent; the others against the closed form (every machine: lambda =
+    # 2^bits, found once Brent's window reaches it, within 2^(bits+1) + 1 passes).
+    progs = [(counter(12), 31), (counter(18), 32), (counter(21), 33)]
+    print("2^20 passes", flush=True)
+    got, _ = engine.batch_run([p for p, _ in progs], [s for _, s in progs], groups=1,
+                              max_passes=1 << 20)
+    assert got[0] == expected(progs[0][0], progs[0][1], 1, 1 << 20)
+    for g, bits in zip(got, (12, 18, 21)):
+        # tail 1 (the carry cells), so Brent's tortoise waits at pass 2^k - 1 >= 1 with window
+        # 2^k >= 2^bits: found at pass 2^bits - 1 + 2^bits
+        found = (1 << (bits + 1)) - 1 <= 1 << 20
+        hist = [0] * 64
+        hist[bits if found else 63] = 512
+        assert g == {"hist": hist, "lam_sum": 512 << bits if found else 0,
+                     "longest": 1 << bits if found else 0,
+                     "passes": (1 << (bits + 1)) - 1 if found else 1 << 20}

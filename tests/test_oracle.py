@@ -93,9 +93,12 @@ def test_brent_against_reference(ref):
         theirs = ref.make("random_program", 40 + 10 * k, 120, 500 + k)
         lam = np.zeros(512, np.uint64)
         aborted = C.c_int()
-        ref.lib().t1_cycles(theirs.ptr, 500 + k, 4096, lam.ctypes.data, None, C.byref(aborted))
+        steps = ref.lib().t1_cycles(theirs.ptr, 500 + k, 4096, lam.ctypes.data, None,
+                                    C.byref(aborted))
         mem, rr = t.t1_start(prog, 500 + k)
-        assert (O.brent_lines(prog, mem[0], rr[0], 4096) == lam).all()
+        ours, passes = O.brent_lines(prog, mem[0], rr[0], 4096, with_passes=True)
+        assert (ours == lam).all()
+        assert passes == steps  # the passes the group ran before every machine found its cycle
 
 
 def test_life_scalar_patterns():

@@ -77,6 +77,14 @@ int  gpu_state_get(gpu_state *s, unsigned group0, unsigned ngroups, ts_line *mem
 /* `passes` whole passes of every machine of the state, cubes in lockstep: the counterpart of
    jit_run (src/jit.h:54) and jit_cube_run (src/jit.h:59). */
 int  gpu_run(gpu_code *c, gpu_state *s, uint64_t passes, char *err, size_t errlen);
+/* `steps` instructions in every machine, starting at the state's next instruction (pc) and
+   wrapping into further passes: whole passes run as gpu_run, a partial pass as the sub-program of
+   its instructions (compiled on first use, kept with the code). The counterpart of the reference's
+   single steps (ref_cube_step, src/ref.c:119-130) and of finishing a started pass
+   (src/jit.c:743-746, src/check.c:89-107). gpu_run refuses a state whose pass is in progress;
+   gpu_state_reset sets pc to 0. */
+int  gpu_run_steps(gpu_code *c, gpu_state *s, uint64_t steps, char *err, size_t errlen);
+uint64_t gpu_state_pc(const gpu_state *s);
 /* The device's measured rate of 32-bit v_bitop3_b32 lane-operations per second (a calibration
    launch that fills every SIMD), for stating results against the real integer-VALU roofline. */
 int  gpu_valu_rate(gpu_engine *e, double *lane_ops_per_s, char *err, size_t errlen);

@@ -77,10 +77,11 @@ static void put_bit(ts_line *l, unsigned k, int v)
 /* One cube of M machines, one byte per cell: mem[m * ncells + c], rr[m].
    Machine by machine within an instruction equals lockstep: a NAND changes only RR and nobody
    reads during a STO (reference src/ref.c:117-118). */
-static void cube_pass(const ts_program *p, const decoded *d, uint32_t M, uint8_t *mem, uint8_t *rr)
+static void cube_steps(const ts_program *p, const decoded *d, uint32_t M, uint8_t *mem, uint8_t *rr,
+                       size_t from, size_t to)
 {
     const uint32_t n = p->ncells;
-    for (size_t i = 0; i < p->len; i++) {
+    for (size_t i = from; i < to; i++) {
         const decoded *in = &d[i];
         if (in->op == 2)
             continue;
@@ -101,6 +102,55 @@ static void cube_pass(const ts_program *p, const decoded *d, uint32_t M, uint8_t
             rr[m] = (uint8_t)!(rr[m] & v);
         }
     }
+}
+
+static void cube_pass(const ts_program *p, const decoded *d, uint32_t M, uint8_t *mem, uint8_t *rr)
+{
+    cube_steps(p, d, M, mem, rr, 0, p->len);
+}
+
+/* `steps` instructions in every machine from instruction pc (wrapping at the end of the
+   program), reference src/ref.c:117-130 (ref_cube_step); returns the new pc, or -1 if groups is
+   not a whole number of cubes. */
+int64_t oracle_steps_lines(const ts_program *p, unsigned groups, ts_line *mem, ts_line *rr,
+                           uint64_t pc, uint64_t steps)
+{
+    unsigned dims = cube_dims(p), cube_groups = 1u << (dims - TS_CUBE_DIMS);
+    if (groups % cube_groups || (p->len && pc >= p->len))
+        return -1;
+    const uint32_t n = p->ncells, M = cube_groups * TS_CUBE_SIZE;
+    decoded *d = decode(p);
+    uint8_t *bm = malloc((size_t)M * n), *br = malloc(M);
+    uint64_t end_pc = pc;
+    for (unsigned c0 = 0; c0 < groups; c0 += cube_groups) {
+        for (uint32_t m = 0; m < M; m++) {
+            const ts_line *g = mem + (size_t)(c0 + m / TS_CUBE_SIZE) * n;
+            for (uint32_t c = 0; c < n; c++)
+                bm[(size_t)m * n + c] = (uint8_t)get_bit(&g[c], m % TS_CUBE_SIZE);
+            bm[(size_t)m * n + TS_CELL_ZERO] = 0;
+            bm[(size_t)m * n + TS_CELL_ONE] = 1;
+            br[m] = (uint8_t)get_bit(&rr[c0 + m / TS_CUBE_SIZE], m % TS_CUBE_SIZE);
+        }
+        uint64_t at = pc, left = p->len ? steps : 0;
+        while (left) {
+            size_t to = p->len - at < left ? p->len : (size_t)(at + left);
+            cube_steps(p, d, M, bm, br, (size_t)at, to);
+            left -= to - at;
+            at = to == p->len ? 0 : to;
+        }
+        end_pc = at;
+        for (uint32_t m = 0; m < M; m++) {
+            ts_line *g = mem + (size_t)(c0 + m / TS_CUBE_SIZE) * n;
+            for (uint32_t c = TS_CELL_FIRST_FREE; c < n; c++)
+                if (!readonly(p, c))
+                    put_bit(&g[c], m % TS_CUBE_SIZE, bm[(size_t)m * n + c]);
+            put_bit(&rr[c0 + m / TS_CUBE_SIZE], m % TS_CUBE_SIZE, br[m]);
+        }
+    }
+    free(bm);
+    free(br);
+    free(d);
+    return (int64_t)end_pc;
 }
 
 int oracle_run_lines(const ts_program *p, unsigned groups, ts_line *mem, ts_line *rr,

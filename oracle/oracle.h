@@ -9,6 +9,10 @@
                            tests against the reference's own build (oracle/_ref) on random
                            programs with id and link cells, and by closed forms (adders,
                            multipliers, toyhash, Life) that a dropped term or a wrong index fails.
+     oracle_steps_lines    the same machine one instruction at a time (ref_cube_step, src/ref.c:
+                           119-130), from any pc, wrapping into the next pass; pinned against
+                           oracle/_ref's ref_cube_step and against oracle_run_lines for whole
+                           passes.
      oracle_enum           enum_run's assignment mapping and hit rules (src/enum.c:150-213,
                            src/enum.h:3-5); pinned by the reference's known answers (adder8
                            131,072 tried / 0 hits and 65,536 hits with hit[] = 1,3,..,31), shares
@@ -33,6 +37,8 @@
    cell, one instruction at a time, cubes in lockstep). groups = number of 512-machine groups; it
    must be a multiple of the program's cube size in groups (2^(dims-9)); consecutive blocks form
    independent cubes. mem[g * ncells + c] / rr[g] are 512-machine lines, updated in place. */
+int64_t  oracle_steps_lines(const ts_program *p, unsigned groups, ts_line *mem, ts_line *rr,
+                            uint64_t pc, uint64_t steps);
 int      oracle_run_lines(const ts_program *p, unsigned groups, ts_line *mem, ts_line *rr,
                           uint64_t passes);
 

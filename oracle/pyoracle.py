@@ -36,6 +36,8 @@ def lib():
         L.oracle_enum.argtypes = [C.c_int, vp, vp, vp, vp, vp, C.c_int, vp, u64, C.c_uint, C.c_int,
                                   C.c_int, C.c_int, C.POINTER(OracleEnumResult)]
         L.oracle_enum.restype = C.c_int
+        L.oracle_steps_lines.argtypes = [vp, C.c_uint, vp, vp, u64, u64]
+        L.oracle_steps_lines.restype = C.c_int64
         L.oracle_brent_lines.argtypes = [vp, vp, vp, u64, vp]
         L.oracle_brent_lines.restype = u64
         L.oracle_exact_cycle.argtypes = [vp, vp, vp, C.c_uint]
@@ -89,6 +91,16 @@ def enum(mode, a, b=None, b_in=None, b_out=None, out_cells=None, expect=None, ta
     if not ok:
         raise ValueError("oracle_enum refused the program")
     return {"space": r.space, "tried": r.tried, "hits": r.hits, "hit": list(r.hit[:r.nhit]), "nhit": r.nhit}
+
+
+def steps_lines(prog, mem, rr, pc, steps):
+    """(mem, rr, pc) after `steps` instructions from instruction pc on the reference machine."""
+    mem = np.array(mem, dtype=np.uint64, order="C")
+    rr = np.array(rr, dtype=np.uint64, order="C")
+    pc = lib().oracle_steps_lines(_addr(prog), rr.shape[0], mem.ctypes.data, rr.ctypes.data, pc, steps)
+    if pc < 0:
+        raise ValueError("groups not a multiple of the cube, or pc past the program")
+    return mem, rr, int(pc)
 
 
 def brent_lines(prog, mem, rr, max_passes, with_passes=False):

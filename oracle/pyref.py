@@ -76,6 +76,7 @@ def lib():
         L.ref_cube_init.restype = C.c_int
         L.ref_cube_free.argtypes = [vp]
         L.ref_cube_tick.argtypes = [vp]
+        L.ref_cube_step.argtypes = [vp]
         L.ref_cube_machines.argtypes = [vp]
         L.ref_cube_machines.restype = C.c_uint
         L.ref_set.argtypes = [vp, C.c_uint32, C.c_uint8]
@@ -138,8 +139,9 @@ def enum_run(mode, a_ptr, b_ptr=None, spec_text=None, spec_inputs=True, ticks=1,
             "nhit": r.nhit, "seconds": r.seconds, "ops": r.ops, "threads": r.threads}
 
 
-def cube_run(prog_ptr, ncells, mem, rr, passes):
-    """The reference machine cube (ref_cube_*) from line state, as the oracle's run_lines."""
+def cube_run(prog_ptr, ncells, mem, rr, passes, steps=0):
+    """The reference machine cube (ref_cube_*) from line state, as the oracle's run_lines: whole
+    passes (ref_cube_tick), then `steps` single instructions (ref_cube_step)."""
     L = lib()
     n = L.ref_cube_machines(prog_ptr)
     cube = (RefMachine * n)()
@@ -156,6 +158,8 @@ def cube_run(prog_ptr, ncells, mem, rr, passes):
         cube[k].rr = int(rbits[k // 512, k % 512])
     for _ in range(passes):
         L.ref_cube_tick(C.cast(cube, C.c_void_p))
+    for _ in range(steps):
+        L.ref_cube_step(C.cast(cube, C.c_void_p))
     out = bits.copy()
     outr = rbits.copy()
     for k in range(n):

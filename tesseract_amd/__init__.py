@@ -166,6 +166,8 @@ def lib():
             "gpu_state_free": (None, [vp]),
             "gpu_state_reset": (i32, [vp, C.c_char_p, sz]),
             "gpu_state_put": (i32, [vp, C.c_uint, C.c_uint, vp, vp, C.c_char_p, sz]),
+            "gpu_run_steps": (i32, [vp, vp, u64, C.c_char_p, sz]),
+            "gpu_state_pc": (u64, [vp]),
             "gpu_state_get": (i32, [vp, C.c_uint, C.c_uint, vp, vp, C.c_char_p, sz]),
             "gpu_run": (i32, [vp, vp, u64, C.c_char_p, sz]),
             "gpu_enum_run": (i32, [vp, C.POINTER(GpuEnumStruct), C.POINTER(EnumResultStruct),
@@ -757,3 +759,13 @@ class State:
         if not lib().gpu_run(self.code._c, self._s, passes, e, len(e)):
             raise RuntimeError("gpu_run: " + e.value.decode())
         return self.code.engine.last_ms
+
+    def run_steps(self, steps):
+        """gpu_run_steps: `steps` instructions in every machine from the current pc."""
+        e = _err()
+        if not lib().gpu_run_steps(self.code._c, self._s, steps, e, len(e)):
+            raise RuntimeError("gpu_run_steps: " + e.value.decode())
+
+    @property
+    def pc(self):
+        return int(lib().gpu_state_pc(self._s))

@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -33,6 +34,9 @@ struct gpu_code {
     uint32_t               ncells = 0;
     std::vector<uint8_t>   kind;        // per cell: TS_CK_*; constants as plain
     std::vector<uint8_t>   id_dim;
+    // partial passes (gpu_run_steps): a copy of the program and the compiled instruction ranges
+    ts_program             prog{};
+    std::map<std::pair<uint64_t, uint64_t>, gpu_code *> parts;
 };
 
 struct gpu_state {
@@ -45,6 +49,7 @@ struct gpu_state {
     // parity and replayed, instead of one launch per segment per pass
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     unsigned       graph_passes = 0;
+    uint64_t       pc = 0;   // the next instruction of every machine (0 = between passes)
 };
 
 namespace {
@@ -307,6 +312,10 @@ extern "C" int gpu_compile(gpu_engine *e, const ts_program *p, const gpu_tune *t
     }
     fill_info(c->lw, ks, c->info);
     c->info.compile_seconds = secs;
+    if (!ts_program_copy(&c->prog, p)) {
+        say(err, errlen, "out of memory");
+        return 0;
+    }
     *out = c.release();
     return 1;
 }
@@ -348,6 +357,9 @@ extern "C" void gpu_code_free(gpu_code *c)
         (void)hipSetDevice(c->e->device);
         (void)hipModuleUnload(c->mod);
     }
+    for (auto &kv : c->parts)
+        gpu_code_free(kv.second);
+    ts_program_free(&c->prog);
     delete c;
 }
 
@@ -441,6 +453,7 @@ extern "C" int gpu_state_reset(gpu_state *s, char *err, size_t errlen)
     gpu_code *c = s->c;
     HIPCHK(hipSetDevice(c->e->device));
     s->latest = 1;
+    s->pc = 0;
     size_t slot_bytes = (size_t)2 * c->lw.nslots * s->nw * 4, init_bytes = (size_t)c->lw.ninit * s->nw * 4;
     if (slot_bytes)
         HIPCHK(hipMemsetAsync(s->slots, 0, slot_bytes, c->e->stream));
@@ -510,6 +523,11 @@ extern "C" int gpu_run(gpu_code *c, gpu_state *s, uint64_t passes, char *err, si
 {
     if (s->c != c) {
         say(err, errlen, "the state belongs to another program");
+        return 0;
+    }
+    if (s->pc) {
+        say(err, errlen, "a pass is in progress (next instruction %llu): finish it with "
+                         "gpu_run_steps", (unsigned long long)s->pc);
         return 0;
     }
     gpu_engine *e = c->e;
@@ -593,6 +611,71 @@ extern "C" int gpu_run(gpu_code *c, gpu_state *s, uint64_t passes, char *err, si
     e->last_ms = ms;
     return 1;
 }
+
+// ---- partial passes -------------------------------------------------------------------------
+
+namespace {
+
+// instructions [from, from + n) of c's program in every machine: the sub-program of those
+// instructions, compiled once, run for one pass on a state of its own, the machines' cells and
+// RR carried over in the reference's line layout (a sub-program has its own slot layout)
+int run_part(gpu_code *c, gpu_state *s, uint64_t from, uint64_t n, char *err, size_t errlen)
+{
+    gpu_code *&sub = c->parts[{from, n}];
+    if (!sub) {
+        ts_program p = c->prog;
+        p.code = c->prog.code + from;
+        p.len = (size_t)n;
+        if (!gpu_compile(c->e, &p, &c->tune, &sub, nullptr, err, errlen)) {
+            c->parts.erase({from, n});
+            return 0;
+        }
+    }
+    gpu_state *ss = nullptr;
+    if (!gpu_state_new(sub, s->groups, &ss, err, errlen))
+        return 0;
+    std::vector<ts_line> mem((size_t)s->groups * c->ncells), rr(s->groups);
+    int ok = gpu_state_get(s, 0, s->groups, mem.data(), rr.data(), err, errlen) &&
+             gpu_state_put(ss, 0, s->groups, mem.data(), rr.data(), err, errlen) &&
+             gpu_run(sub, ss, 1, err, errlen) &&
+             gpu_state_get(ss, 0, s->groups, mem.data(), rr.data(), err, errlen) &&
+             gpu_state_put(s, 0, s->groups, mem.data(), rr.data(), err, errlen);
+    gpu_state_free(ss);
+    return ok;
+}
+
+}  // namespace
+
+// `steps` instructions in every machine from the state's pc, whole passes through gpu_run: the
+// counterpart of the reference's single steps (ref_cube_step, src/ref.c:119-130) and of finishing
+// a started pass (src/jit.c:743-746, src/check.c:89-107)
+extern "C" int gpu_run_steps(gpu_code *c, gpu_state *s, uint64_t steps, char *err, size_t errlen)
+{
+    if (s->c != c) {
+        say(err, errlen, "the state belongs to another program");
+        return 0;
+    }
+    const uint64_t len = c->prog.len;
+    if (!len)
+        return 1;
+    while (steps) {
+        if (s->pc == 0 && steps >= len) {
+            const uint64_t whole = steps / len;
+            if (!gpu_run(c, s, whole, err, errlen))
+                return 0;
+            steps -= whole * len;
+            continue;
+        }
+        const uint64_t n = std::min(steps, len - s->pc);
+        if (!run_part(c, s, s->pc, n, err, errlen))
+            return 0;
+        s->pc = (s->pc + n) % len;
+        steps -= n;
+    }
+    return 1;
+}
+
+extern "C" uint64_t gpu_state_pc(const gpu_state *s) { return s->pc; }
 
 // ---- enumeration ----------------------------------------------------------------------------
 

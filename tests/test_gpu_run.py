@@ -114,3 +114,28 @@ def test_graph_replays_match_eager(engine):
     st1.put(mem, rr)
     st1.run(done)
     assert all(np.array_equal(a, b) for a, b in zip(st1.get(), st.get()))
+
+
+@pytest.mark.parametrize("dims,seed", [(9, 41), (12, 42)])
+def test_run_steps_against_oracle(engine, dims, seed):
+    # partial passes: single instructions, a pass finished from the middle, whole passes from
+    # pc 0 and a run wrapping from the middle of one pass into the middle of another, each
+    # compared with the oracle machine stepped the same way (every cell of every machine)
+    prog = random_cube_program(12, 120, dims, seed)
+    groups = prog.groups
+    mem, rr = O.random_lines(prog, groups, seed)
+    code = engine.compile(prog)
+    st = code.state(groups)
+    st.put(mem, rr)
+    writable = [c for c in range(2, prog.ncells) if prog.kind(c) == t.CK_PLAIN]
+    pc = 0
+    print("steps, dims", dims, flush=True)
+    for steps in (1, 4, 115, 240, 3 * 120, 50, 130, 59):
+        st.run_steps(steps)
+        mem, rr, pc = O.steps_lines(prog, mem, rr, pc, steps)
+        gm, gr = st.get()
+        assert st.pc == pc
+        assert np.array_equal(gr, rr), steps
+        assert np.array_equal(gm[:, writable], mem[:, writable]), steps
+    with pytest.raises(RuntimeError, match="pass is in progress"):
+        st.run(1)

@@ -24,6 +24,33 @@ def test_run_lines_against_reference_cube(ref, dims, seed):
     assert np.array_equal(got[0][:, writable], want[0][:, writable])
 
 
+@pytest.mark.parametrize("dims,seed", [(9, 4), (10, 5)])
+def test_steps_lines_against_reference_cube(ref, dims, seed):
+    # single instructions from pc 0 (ref_cube_step), into a second pass, against the reference
+    prog = _cube_prog(12, 150, dims, seed)
+    mem, rr = O.random_lines(prog, prog.groups, seed)
+    writable = [c for c in range(2, prog.ncells) if prog.kind(c) == t.CK_PLAIN]
+    for steps in (1, 37, 150 + 61):
+        want = ref.cube_run(ref_ptr(prog), prog.ncells, mem, rr, 0, steps)
+        gm, gr, pc = O.steps_lines(prog, mem, rr, 0, steps)
+        assert pc == steps % 150
+        assert np.array_equal(gr, want[1])
+        assert np.array_equal(gm[:, writable], want[0][:, writable])
+
+
+def test_steps_lines_compose():
+    # steps from any pc compose: a + b steps = a steps then b steps; whole passes = run_lines
+    prog = _cube_prog(10, 90, 9, 6)
+    mem, rr = O.random_lines(prog, prog.groups, 6)
+    m1, r1, pc1 = O.steps_lines(prog, mem, rr, 0, 53)
+    m2, r2, pc2 = O.steps_lines(prog, m1, r1, pc1, 200)
+    m3, r3, pc3 = O.steps_lines(prog, mem, rr, 0, 253)
+    assert pc2 == pc3 == 253 % 90 and np.array_equal(m2, m3) and np.array_equal(r2, r3)
+    m4, r4, pc4 = O.steps_lines(prog, mem, rr, 0, 2 * 90)
+    w4, wr4 = O.run_lines(prog, mem, rr, 2)
+    assert pc4 == 0 and np.array_equal(m4, w4) and np.array_equal(r4, wr4)
+
+
 def ref_ptr(prog):
     import ctypes as C
     return C.cast(prog.ptr, C.c_void_p)

@@ -2,6 +2,7 @@
 #include "lower.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstdio>
 
 namespace ts {
@@ -296,12 +297,13 @@ static void slice_outputs(Segment &s, std::vector<uint32_t> stored, unsigned tar
 
 // Slice size when the caller leaves it open. A big cube (2^14 machines and more) is usually one
 // state of a few cubes, so its parallelism must come from slices: aim at kSliceThreads threads per
-// segment launch (8 waves per SIMD of a 256-CU device), at least 200 nodes per slice so the
+// segment launch (8 waves per SIMD of a 256-CU device), at least 50 nodes per slice so the
 // re-computation of shared logic stays small. Smaller cubes run many cubes per state and keep
-// large slices. (Life on 2^16 machines: 1,500-node slices 27.8 us/generation, 200-node 12.7 us.)
+// large slices. (Life on 2^16 machines: 1,500-node slices 27.8 us/generation, 200-node 12.7 us;
+// with last-pass-only outputs, a 200-node floor 8.3 us and a 50-node floor 8.0 us.)
 static unsigned auto_slice_nodes(const Segment &s, unsigned dims)
 {
-    const unsigned kLarge = 1500, kSmall = 200;
+    const unsigned kLarge = 1500, kSmall = 50;
     const uint64_t kSliceThreads = 256ull * 4 * 8 * 64;
     if (dims < 14)
         return kLarge;

@@ -77,6 +77,14 @@ int  gpu_state_get(gpu_state *s, unsigned group0, unsigned ngroups, ts_line *mem
 /* `passes` whole passes of every machine of the state, cubes in lockstep: the counterpart of
    jit_run (src/jit.h:54) and jit_cube_run (src/jit.h:59). */
 int  gpu_run(gpu_code *c, gpu_state *s, uint64_t passes, char *err, size_t errlen);
+/* One machine's cell, machine = group * 512 + lane, cell == ncells for RR: the counterparts of
+   lane_get / lane_set (src/lanes.h:60-63) and ref_get / ref_set (src/ref.h:28-29) for `run
+   --table`. peek reads any cell, a link cell through its chain from the neighbouring machine;
+   poke ignores read-only cells (ZERO, ONE, id, link) as the reference does. */
+int  gpu_state_peek(gpu_state *s, uint64_t machine, uint32_t cell, uint8_t *v, char *err,
+                    size_t errlen);
+int  gpu_state_poke(gpu_state *s, uint64_t machine, uint32_t cell, uint8_t v, char *err,
+                    size_t errlen);
 /* `steps` instructions in every machine, starting at the state's next instruction (pc) and
    wrapping into further passes: whole passes run as gpu_run, a partial pass as the sub-program of
    its instructions (compiled on first use, kept with the code). The counterpart of the reference's

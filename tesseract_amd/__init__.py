@@ -167,6 +167,8 @@ def lib():
             "gpu_state_reset": (i32, [vp, C.c_char_p, sz]),
             "gpu_state_put": (i32, [vp, C.c_uint, C.c_uint, vp, vp, C.c_char_p, sz]),
             "gpu_run_steps": (i32, [vp, vp, u64, C.c_char_p, sz]),
+            "gpu_state_peek": (i32, [vp, u64, u32, C.POINTER(C.c_uint8), C.c_char_p, sz]),
+            "gpu_state_poke": (i32, [vp, u64, u32, C.c_uint8, C.c_char_p, sz]),
             "gpu_state_pc": (u64, [vp]),
             "gpu_state_get": (i32, [vp, C.c_uint, C.c_uint, vp, vp, C.c_char_p, sz]),
             "gpu_run": (i32, [vp, vp, u64, C.c_char_p, sz]),
@@ -249,6 +251,10 @@ class Program:
 
     def kind(self, cell):
         return self._p.special[cell].kind if self._p.special else CK_PLAIN
+
+    def cell_dim(self, cell):
+        """The dimension of an id or link cell."""
+        return self._p.special[cell].dim if self._p.special else 0
 
     def code(self):
         """(ops, addrs) as numpy arrays."""
@@ -769,3 +775,17 @@ class State:
     @property
     def pc(self):
         return int(lib().gpu_state_pc(self._s))
+
+    def peek(self, machine, cell):
+        """gpu_state_peek: one machine's cell (cell == ncells: RR), links resolved."""
+        v = C.c_uint8()
+        e = _err()
+        if not lib().gpu_state_peek(self._s, machine, cell, C.byref(v), e, len(e)):
+            raise RuntimeError(e.value.decode())
+        return int(v.value)
+
+    def poke(self, machine, cell, value):
+        """gpu_state_poke: set one machine's cell (ignored for read-only cells)."""
+        e = _err()
+        if not lib().gpu_state_poke(self._s, machine, cell, int(value), e, len(e)):
+            raise RuntimeError(e.value.decode())

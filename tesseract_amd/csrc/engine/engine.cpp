@@ -612,6 +612,72 @@ extern "C" int gpu_run(gpu_code *c, gpu_state *s, uint64_t passes, char *err, si
     return 1;
 }
 
+// ---- one machine's cells --------------------------------------------------------------------
+
+namespace {
+
+// the group line holding `cell` of `machine` (after link resolution) and the bit within it
+int machine_line(gpu_state *s, uint64_t machine, uint32_t cell, uint32_t *base, uint64_t *at,
+                 char *err, size_t errlen)
+{
+    const gpu_code *c = s->c;
+    if (machine >= (uint64_t)s->groups * TS_CUBE_SIZE || cell > c->ncells) {
+        say(err, errlen, "machine %llu cell %u outside the state (%llu machines, %u cells + RR)",
+            (unsigned long long)machine, cell, (unsigned long long)s->groups * TS_CUBE_SIZE,
+            c->ncells);
+        return 0;
+    }
+    unsigned mask = 0;
+    *base = cell == c->ncells ? cell : ts_program_resolve(&c->prog, cell, &mask);
+    const uint64_t cube = (uint64_t)1 << c->lw.dims;
+    *at = machine / cube * cube + ((machine % cube) ^ mask);
+    return 1;
+}
+
+}  // namespace
+
+// lane_get / ref_get (src/lanes.h:60-62, src/ref.h:28): any cell, links and ids included;
+// cell == ncells is RR
+extern "C" int gpu_state_peek(gpu_state *s, uint64_t machine, uint32_t cell, uint8_t *v, char *err,
+                              size_t errlen)
+{
+    uint32_t base;
+    uint64_t at;
+    if (!machine_line(s, machine, cell, &base, &at, err, errlen))
+        return 0;
+    const uint32_t n = s->c->ncells;
+    std::vector<ts_line> mem(n);
+    ts_line rr;
+    if (!gpu_state_get(s, (unsigned)(at / TS_CUBE_SIZE), 1, mem.data(), &rr, err, errlen))
+        return 0;
+    const ts_line &l = base == n ? rr : mem[base];
+    const unsigned k = (unsigned)(at % TS_CUBE_SIZE);
+    *v = (uint8_t)(l.w[k / 64] >> (k % 64) & 1);
+    return 1;
+}
+
+// lane_set / ref_set (src/lanes.h:61,63, src/ref.h:29): ignored for read-only cells
+extern "C" int gpu_state_poke(gpu_state *s, uint64_t machine, uint32_t cell, uint8_t v, char *err,
+                              size_t errlen)
+{
+    uint32_t base;
+    uint64_t at;
+    if (!machine_line(s, machine, cell, &base, &at, err, errlen))
+        return 0;
+    const uint32_t n = s->c->ncells;
+    if (cell < n && ts_program_readonly(&s->c->prog, cell))
+        return 1;
+    const unsigned g = (unsigned)(machine / TS_CUBE_SIZE), k = (unsigned)(machine % TS_CUBE_SIZE);
+    std::vector<ts_line> mem(n);
+    ts_line rr;
+    if (!gpu_state_get(s, g, 1, mem.data(), &rr, err, errlen))
+        return 0;
+    ts_line &l = cell == n ? rr : mem[cell];
+    const uint64_t bit = 1ull << (k % 64);
+    l.w[k / 64] = v ? l.w[k / 64] | bit : l.w[k / 64] & ~bit;
+    return gpu_state_put(s, g, 1, mem.data(), &rr, err, errlen);
+}
+
 // ---- partial passes -------------------------------------------------------------------------
 
 namespace {

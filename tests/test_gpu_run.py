@@ -139,3 +139,39 @@ def test_run_steps_against_oracle(engine, dims, seed):
         assert np.array_equal(gm[:, writable], mem[:, writable]), steps
     with pytest.raises(RuntimeError, match="pass is in progress"):
         st.run(1)
+
+
+def test_peek_poke(engine):
+    # one machine's cells after a pass, links and ids included, as the oracle machine has them;
+    # poke changes one machine's cell and leaves read-only cells alone
+    prog = random_cube_program(10, 80, 10, 43)
+    groups = prog.groups
+    mem, rr = O.random_lines(prog, groups, 43)
+    code = engine.compile(prog)
+    st = code.state(groups)
+    st.put(mem, rr)
+    st.run(1)
+    want_mem, want_rr = O.run_lines(prog, mem, rr, 1)
+    bit = lambda line, k: int(line[k // 64]) >> (k % 64) & 1
+    cube = 1 << prog.dims
+    n = prog.ncells
+    machines = [0, 1, 511, 512, 700, groups * 512 - 1]
+    for m in machines:
+        assert st.peek(m, n) == bit(want_rr[m // 512], m % 512)
+        for cell in range(n):
+            kind = prog.kind(cell)
+            if kind == t.CK_PLAIN:
+                want = bit(want_mem[m // 512, cell], m % 512)
+            elif kind == t.CK_ID:
+                want = (m % cube) >> prog.cell_dim(cell) & 1
+            else:
+                continue  # links: checked through their base cell below
+            assert st.peek(m, cell) == want, (m, cell)
+    for cell in range(n):
+        before = [st.peek(m, cell) for m in machines]
+        st.poke(700, cell, 1 - before[4])
+        after = [st.peek(m, cell) for m in machines]
+        want = list(before)
+        if prog.kind(cell) == t.CK_PLAIN and cell >= 2:
+            want[4] = 1 - before[4]
+        assert after == want, cell

@@ -100,3 +100,14 @@ def test_batch_reference_scale_passes(engine):
         assert g == {"hist": hist, "lam_sum": 512 << bits if found else 0,
                      "longest": 1 << bits if found else 0,
                      "passes": (1 << (bits + 1)) - 1 if found else 1 << 20}
+
+
+def test_batch_program_without_instructions(engine):
+    # nothing changes, so every machine is back at its start after one pass: cycle length 1
+    b = t.Builder()
+    b.cell("x")
+    b.cell("y")
+    prog = b.finish()
+    got, info = engine.batch_run([prog], [7], groups=4, max_passes=64)
+    assert got[0] == expected(prog, 7, 4, 64)
+    assert got[0]["hist"][0] == 4 * 512 and got[0]["passes"] == 4

@@ -123,3 +123,17 @@ def test_toyhash32_search_random_target(engine):
     assert _rev(x, 32) in r["hit"] or r["hits"] > 16
     for h in r["hit"]:
         assert O.toyhash(_rev(h, 32), 32, 8) == target
+
+
+def test_empty_program_enumeration(engine):
+    b = t.Builder()
+    x = [b.cell("x%d" % i) for i in range(3)]
+    for c in x:
+        b.input(c)
+        b.output(c)
+    prog = b.finish()  # no instructions: outputs are the inputs
+    r = engine.enum_run(t.EN_VERIFY, prog, spec="x = x")
+    assert r["tried"] == 8 and r["hits"] == 0
+    for tune in (None, {"tape": 1}):
+        r = engine.enum_run(t.EN_SEARCH, prog, spec="x = 5", tune=tune)
+        assert r["hits"] == 1 and r["hit"] == [5]

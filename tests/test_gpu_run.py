@@ -175,3 +175,21 @@ def test_peek_poke(engine):
         if prog.kind(cell) == t.CK_PLAIN and cell >= 2:
             want[4] = 1 - before[4]
         assert after == want, cell
+
+
+def test_empty_program(engine):
+    # a program of no instructions still runs passes: every cell and RR keep their values
+    # (reference src/ref.c:52-55)
+    b = t.Builder()
+    x = b.cell("x")
+    b.input(x)
+    b.output(x)
+    prog = b.finish()
+    mem, rr = O.random_lines(prog, 1, 9)
+    st = engine.compile(prog).state(1)
+    st.put(mem, rr)
+    st.run(5)
+    st.run_steps(3)
+    gm, gr = st.get()
+    assert np.array_equal(gr, rr) and np.array_equal(gm[:, 2:], mem[:, 2:])
+    assert st.pc == 0

@@ -139,6 +139,44 @@ def enum_run(mode, a_ptr, b_ptr=None, spec_text=None, spec_inputs=True, ticks=1,
             "nhit": r.nhit, "seconds": r.seconds, "ops": r.ops, "threads": r.threads}
 
 
+def t1_cycles(prog, seed, max_passes):
+    """The reference's cycle search on one group of 512 machines (its jit-compiled code):
+    (lambda per lane, passes run)."""
+    lam = np.zeros(512, np.uint64)
+    aborted = C.c_int()
+    passes = lib().t1_cycles(prog.ptr, seed, max_passes, lam.ctypes.data, None, C.byref(aborted))
+    return lam, int(passes)
+
+
+def t1_sample(progs, seeds, max_passes, threads, seconds):
+    """CPU baseline of the batch workload: t1_cycles on group 0 of each program, programs taken in
+    order by `threads` workers (ctypes releases the GIL) until `seconds` have passed. Returns
+    (programs done, 1-bit ops as the reference counts them, seconds)."""
+    import threading
+    import time
+    lock = threading.Lock()
+    state = {"next": 0, "done": 0, "ops": 0}
+    t0 = time.perf_counter()
+
+    def worker():
+        while True:
+            with lock:
+                k = state["next"]
+                if k >= len(progs) or time.perf_counter() - t0 > seconds:
+                    return
+                state["next"] += 1
+            _, ran = t1_cycles(progs[k], seeds[k], max_passes)
+            with lock:
+                state["done"] += 1
+                state["ops"] += ran * progs[k].s.len * 512  # src/tester.c:305
+    pool = [threading.Thread(target=worker) for _ in range(threads)]
+    for th in pool:
+        th.start()
+    for th in pool:
+        th.join()
+    return state["done"], state["ops"], time.perf_counter() - t0
+
+
 def cube_run(prog_ptr, ncells, mem, rr, passes, steps=0):
     """The reference machine cube (ref_cube_*) from line state, as the oracle's run_lines: whole
     passes (ref_cube_tick), then `steps` single instructions (ref_cube_step)."""

@@ -41,7 +41,31 @@ def batch_mix(n, seed):
     return progs, seeds
 
 
-def run_batch(engine, n, groups, passes, seed, plain_too=True, lds_cells=0, waves_per_cu=0):
+def cpu_batch_baseline(n, seed, passes, seconds, threads=16):
+    """The reference's own t1_cycles (oracle/_ref, jit-compiled) on group 0 of the programs of the
+    same mix, on `threads` host threads for about `seconds`: a reported baseline, not the measured
+    path. Like the reference's tester, each call compiles the program for its 512 machines."""
+    from oracle import pyref
+    if not pyref.available():
+        return None
+    s = C.c_uint64(seed)
+    progs, seeds = [], []
+    for k in range(n):
+        d0 = t.lib().ts_splitmix64(C.byref(s))
+        d1 = t.lib().ts_splitmix64(C.byref(s))
+        cells = SIZES[d0 % 5]
+        length = int(math.floor(10 ** (2 + 3 * ((d1 >> 11) / float(1 << 53)))))
+        ps = t.t1_program_seed(seed, cells, k)
+        progs.append(pyref.make("random_program", cells, length, ps))
+        seeds.append(ps)
+    done, ops, secs = pyref.t1_sample(progs, seeds, passes, threads, seconds)
+    return {"value": ops / secs, "unit": "1-bit ops/s", "cores": threads, "kind": "reference",
+            "sample": "reference t1_cycles on group 0 (512 machines) of the first %d programs of "
+                      "the mix, %.1f s" % (done, secs)}
+
+
+def run_batch(engine, n, groups, passes, seed, plain_too=True, lds_cells=0, waves_per_cu=0,
+              cpu_seconds=0):
     progs, seeds = batch_mix(n, seed)
     out = {"workload": "batch", "programs": n, "machines_per_program": groups * 512,
            "passes": passes, "instructions": sum(p.len for p in progs)}
@@ -63,6 +87,8 @@ def run_batch(engine, n, groups, passes, seed, plain_too=True, lds_cells=0, wave
         out["launches_plain"] = info_p["launches"]
         out["batched_over_plain"] = info_p["ms"] / info["ms"]
         out["same_results_alone_and_batched"] = stats == stats_p
+    if cpu_seconds:  # the configuration's 10,000 programs, as many as fit the time budget
+        out["cpu_baseline"] = cpu_batch_baseline(max(n, 10000), seed, passes, cpu_seconds)
     hist = np.zeros(64, np.uint64)
     for st in stats:
         hist += np.array(st["hist"], np.uint64)
@@ -201,11 +227,13 @@ def main():
     ap.add_argument("--cubes", type=int, default=4096)
     ap.add_argument("--lds-cells", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=0,
+                    help="batch: also time the reference's t1_cycles on host threads this long")
     a = ap.parse_args()
     eng = t.Engine(0)
     if a.workload == "batch":
         out = run_batch(eng, a.programs, a.groups, a.passes, a.seed, not a.no_plain, a.lds_cells,
-                        a.waves_per_cu)
+                        a.waves_per_cu, a.cpu_seconds)
     elif a.workload == "life":
         out = run_life(eng, a.gens)
     elif a.workload == "serve":

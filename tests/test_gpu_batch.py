@@ -111,3 +111,23 @@ def test_batch_program_without_instructions(engine):
     got, info = engine.batch_run([prog], [7], groups=4, max_passes=64)
     assert got[0] == expected(prog, 7, 4, 64)
     assert got[0]["hist"][0] == 4 * 512 and got[0]["passes"] == 4
+
+
+def test_batch_matches_reference_t1_cycles(engine, ref):
+    # group 0 of each program is exactly the reference's t1_cycles run (its own jit-compiled
+    # code): same cycle lengths, same passes
+    from tesseract_amd.workloads import batch_mix
+    progs, seeds = batch_mix(40, 5)
+    keep = [k for k, p in enumerate(progs) if p.len <= 3000]
+    progs, seeds = [progs[k] for k in keep], [seeds[k] for k in keep]
+    got, _ = engine.batch_run(progs, seeds, groups=1, max_passes=64)
+    for p, s, g in zip(progs, seeds, got):
+        theirs = ref.make("random_program", p.ncells - 2, p.len, s)
+        lam, ran = ref.t1_cycles(theirs, s, 64)
+        hist = [0] * 64
+        for x in lam:
+            x = int(x)
+            hist[63 if x == 0 else bucket(x)] += 1
+        found = [int(x) for x in lam if x]
+        assert g == {"hist": hist, "lam_sum": sum(found), "longest": max(found, default=0),
+                     "passes": ran}, (p.ncells, p.len)

@@ -114,8 +114,15 @@ def run_life(engine, gens, check=True, tune=None):
     if check:
         from oracle import pyoracle as O  # the checker only: the run above is finished
         start = life.start_bits()
+        t0 = time.perf_counter()
         want = O.life_words(start, gens + 64)
+        cpu_s = time.perf_counter() - t0
         out["bit_exact_vs_plain_c"] = bool(np.array_equal(life.board_bits(), want))
+        # the checker's bit-parallel plain C Life (64 cells per word, one thread) as the baseline
+        out["cpu_baseline"] = {"value": 8192.0 * 8192 * (gens + 64) / cpu_s, "unit": "cell updates/s",
+                               "cores": 1, "kind": "port",
+                               "sample": "the oracle's bit-parallel C Life, %d generations of the "
+                                         "same board, %.1f s" % (gens + 64, cpu_s)}
     return out
 
 
@@ -149,7 +156,32 @@ def run_hash(engine, target=0x165C2A07):
             "ms": engine.last_ms, "ops_per_s": r["ops"] / (engine.last_ms * 1e-3)}
 
 
-def run_serve(engine, n, rate, seed, streams=4, chunk_words=1 << 24, check=True):
+def cpu_serve_baseline(reqs, seconds, threads=16):
+    """The reference's own enum_run (oracle/_ref: its netlist parser and jit, `threads` host
+    threads) on the requests in order until `seconds` have passed: a reported baseline."""
+    from oracle import pyref
+    if not pyref.available():
+        return None
+    t0 = time.perf_counter()
+    ops = done = 0
+    for a, b, c, r, n_in, faulty in reqs:
+        if time.perf_counter() - t0 > seconds:
+            break
+        ra, rb = pyref.Prog(), pyref.Prog()
+        e = C.create_string_buffer(512)
+        for prog, circ in ((ra, c), (rb, r)):
+            if not pyref.lib().net_parse(circ.bench().encode(), b"net", prog.ptr, e, len(e)):
+                raise ValueError(e.value.decode())
+        res = pyref.enum_run(t.EN_EQUIV, ra.ptr, rb.ptr, threads=threads)
+        ops += res["ops"]
+        done += 1
+    secs = time.perf_counter() - t0
+    return {"value": ops / secs, "unit": "1-bit ops/s", "cores": threads, "kind": "reference",
+            "sample": "reference enum_run (jit) on the first %d requests in order, %.1f s"
+                      % (done, secs)}
+
+
+def run_serve(engine, n, rate, seed, streams=4, chunk_words=1 << 24, check=True, cpu_seconds=0):
     """A seeded Poisson stream of equiv requests (netlists.request) submitted at their arrival
     times; latency = done - arrival. Offered load is the stream's; served load is what finished."""
     import threading
@@ -201,6 +233,8 @@ def run_serve(engine, n, rate, seed, streams=4, chunk_words=1 << 24, check=True)
                 "n": int(m.sum()), "p50_ms": float(np.percentile(lat[m], 50) * 1e3),
                 "p99_ms": float(np.percentile(lat[m], 99) * 1e3)}
     out["latency_by_input_space"] = classes
+    if cpu_seconds:
+        out["cpu_baseline"] = cpu_serve_baseline(reqs, cpu_seconds)
     if check:
         bad = 0
         for (a, b, c, r, ni, faulty), res in zip(reqs, results):
@@ -228,7 +262,7 @@ def main():
     ap.add_argument("--lds-cells", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=0,
-                    help="batch: also time the reference's t1_cycles on host threads this long")
+                    help="batch, serve: also time the reference's own code on host threads this long")
     a = ap.parse_args()
     eng = t.Engine(0)
     if a.workload == "batch":
@@ -237,7 +271,7 @@ def main():
     elif a.workload == "life":
         out = run_life(eng, a.gens)
     elif a.workload == "serve":
-        out = run_serve(eng, a.requests, a.rate, a.seed, a.streams)
+        out = run_serve(eng, a.requests, a.rate, a.seed, a.streams, cpu_seconds=a.cpu_seconds)
     elif a.workload == "benes":
         out = run_benes(eng, a.cubes, a.seed)
     else:

@@ -12,6 +12,13 @@ v_bitop3_b32 evaluates several NANDs at once).
 
 The CPU baseline is the reference's own AVX2 JIT engine (enum_run built from its sources into
 oracle/_ref by oracle/Makefile) on a bounded share of the same enumeration on this host's cores.
+
+--workload life|batch|serve runs one of the secondary workloads of tesseract_amd.workloads instead
+and adds beside its device figures a CPU baseline and the check of its results:
+  life   Life B3/S23 on the 8192 x 8192 torus; the final board compared bit for bit with the
+         oracle's bit-parallel C Life from the same start; that C Life timed as the baseline
+  batch  the random-program mix; the reference's t1_cycles on group 0 of the first programs
+  serve  the seeded equiv request stream; the reference's enum_run on the first requests
 """
 import argparse
 import json
@@ -46,6 +53,107 @@ def cpu_baseline(threads, parts):
             "seconds_full_2p32_extrapolated": r["seconds"] * (1 << 32) / r["tried"]}
 
 
+def cpu_batch_baseline(n, seed, passes, seconds, threads):
+    """The reference's own t1_cycles (jit-compiled, oracle/_ref) on group 0 (512 machines) of the
+    programs of the batch mix, in order, on `threads` host threads for about `seconds`. Like the
+    reference's tester, each call compiles the program for its machines."""
+    import ctypes as C
+    import math
+    from oracle import pyref
+    import tesseract_amd as t
+    from tesseract_amd.workloads import SIZES
+    if not pyref.available():
+        return None
+    s = C.c_uint64(seed)
+    progs, seeds = [], []
+    for k in range(n):  # the mix of tesseract_amd.workloads.batch_mix, built by the reference
+        d0 = t.lib().ts_splitmix64(C.byref(s))
+        d1 = t.lib().ts_splitmix64(C.byref(s))
+        cells = SIZES[d0 % 5]
+        length = int(math.floor(10 ** (2 + 3 * ((d1 >> 11) / float(1 << 53)))))
+        ps = t.t1_program_seed(seed, cells, k)
+        progs.append(pyref.make("random_program", cells, length, ps))
+        seeds.append(ps)
+    done, ops, secs = pyref.t1_sample(progs, seeds, passes, threads, seconds)
+    return {"value": ops / secs, "unit": "1-bit ops/s", "cores": threads, "kind": "reference",
+            "sample": "reference t1_cycles on group 0 (512 machines) of the first %d programs of "
+                      "the mix, %.1f s" % (done, secs)}
+
+
+def cpu_serve_baseline(reqs, seconds, threads):
+    """The reference's own enum_run (its netlist parser and jit, oracle/_ref) on the requests in
+    arrival order until `seconds` have passed."""
+    import ctypes as C
+    from oracle import pyref
+    import tesseract_amd as t
+    if not pyref.available():
+        return None
+    t0 = time.perf_counter()
+    ops = done = 0
+    for a, b, c, r, n_in, faulty in reqs:
+        if time.perf_counter() - t0 > seconds:
+            break
+        ra, rb = pyref.Prog(), pyref.Prog()
+        e = C.create_string_buffer(512)
+        for prog, circ in ((ra, c), (rb, r)):
+            if not pyref.lib().net_parse(circ.bench().encode(), b"net", prog.ptr, e, len(e)):
+                raise ValueError(e.value.decode())
+        res = pyref.enum_run(t.EN_EQUIV, ra.ptr, rb.ptr, threads=threads)
+        ops += res["ops"]
+        done += 1
+    secs = time.perf_counter() - t0
+    return {"value": ops / secs, "unit": "1-bit ops/s", "cores": threads, "kind": "reference",
+            "sample": "reference enum_run (jit) on the first %d requests in order, %.1f s"
+                      % (done, secs)}
+
+
+def life_check(life, gens):
+    """The final board against the oracle's bit-parallel C Life (64 cells per word, one thread)
+    run from the same start for the same generations; that run timed as the baseline."""
+    import numpy as np
+    from oracle import pyoracle as O
+    start = life.start_bits()
+    t0 = time.perf_counter()
+    want = O.life_words(start, gens)
+    cpu_s = time.perf_counter() - t0
+    exact = bool(np.array_equal(life.board_bits(), want))
+    base = {"value": 8192.0 * 8192 * gens / cpu_s, "unit": "cell updates/s", "cores": 1,
+            "kind": "port",
+            "sample": "the oracle's bit-parallel C Life, %d generations of the same board, %.1f s"
+                      % (gens, cpu_s)}
+    return exact, base
+
+
+def run_workload(args):
+    """One secondary workload, its checks and its CPU baseline, as one JSON line."""
+    import tesseract_amd as t
+    from tesseract_amd import workloads as W
+    eng = t.Engine(0)
+    keep = {}
+    if args.workload == "life":
+        out = W.run_life(eng, args.gens, keep=keep)
+        if not args.no_cpu:  # warm-up generations count: the board ran 64 + gens
+            exact, base = life_check(keep["life"], args.gens + 64)
+            out["bit_exact_vs_oracle"] = exact
+            out["cpu_baseline"] = base
+            if not exact:
+                print(json.dumps(out), flush=True)
+                sys.exit("life: the final board differs from the oracle's")
+    elif args.workload == "batch":
+        out = W.run_batch(eng, args.programs, args.groups, args.passes, args.seed)
+        if not args.no_cpu:  # the configuration's 10,000 programs, as many as fit the budget
+            out["cpu_baseline"] = cpu_batch_baseline(max(args.programs, 10000), args.seed,
+                                                     args.passes, args.cpu_seconds,
+                                                     args.cpu_threads)
+    else:
+        out = W.run_serve(eng, args.requests, args.rate, args.seed, keep=keep)
+        if not args.no_cpu:
+            out["cpu_baseline"] = cpu_serve_baseline(keep["requests"], args.cpu_seconds,
+                                                     args.cpu_threads)
+    out["device"] = eng.describe()
+    print(json.dumps(out), flush=True)
+
+
 def profiled_traffic():
     """HBM bytes per ts_enum launch from the committed counter summary (FETCH_SIZE + WRITE_SIZE
     runs, scaled by the copy-kernel factor), or None if it is absent."""
@@ -71,11 +179,24 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="afterwards copy a MIB-MiB device buffer (copy_probe): a kernel of known "
                          "bytes whose FETCH_SIZE/WRITE_SIZE readings scale the kernel's")
+    ap.add_argument("--workload", choices=["mul16", "life", "batch", "serve"], default="mul16")
+    ap.add_argument("--gens", type=int, default=1000, help="life: timed generations")
+    ap.add_argument("--programs", type=int, default=1000, help="batch: programs of the mix")
+    ap.add_argument("--groups", type=int, default=2048, help="batch: 512-machine groups each")
+    ap.add_argument("--passes", type=int, default=64, help="batch: most passes per group")
+    ap.add_argument("--requests", type=int, default=100, help="serve: requests of the stream")
+    ap.add_argument("--rate", type=float, default=200.0, help="serve: offered requests/s")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="batch, serve: bound on the CPU baseline's sample")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (each enumeration share "
-              "would run on its own GPU with a host-side merge); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU and has no multi-GPU path (each enumeration share would "
+              "run on its own GPU with a host-side merge); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "mul16":
+        run_workload(args)
+        return
 
     import tesseract_amd as t
     eng = t.Engine(0)

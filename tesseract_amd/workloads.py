@@ -1,16 +1,21 @@
-"""Secondary workloads of the engine, each printing one JSON line:
+"""Secondary workloads of the engine, each printing one JSON line of device measurements:
 
   python -m tesseract_amd.workloads batch  [--programs N] [--groups G] [--passes P] [--seed S]
   python -m tesseract_amd.workloads life   [--gens G]
   python -m tesseract_amd.workloads hash   (toyhash32 preimage search over 2^32)
   python -m tesseract_amd.workloads benes  (Benes permutation and bitonic sort of 512 bytes)
+  python -m tesseract_amd.workloads serve  [--requests N] [--rate R]
+
+`python bench.py --workload life|batch|serve` runs the same with their CPU baselines and the
+exactness checks against plain C beside them.
 
 batch: the program mix of the batching configuration. Program k draws two splitmix64 values from
 the stream seeded with S: cells = (16, 64, 256, 1024, 4096)[d0 % 5] and
 len = floor(10^(2 + 3 u)) with u = (d1 >> 11) / 2^53 (log-uniform in 10^2..10^5); its instructions
 come from random_program(cells, len, t1_program_seed(S, cells, k)) and its machines start from the
-t1_start stream of that seed. Every machine runs `passes` passes; the batch (persistent launches fed
-by a device queue) and the plain form (one launch per program) must give identical statistics.
+t1_start stream of that seed. A group of 512 machines runs until every machine found its cycle, at
+most `passes` passes; the batch (persistent launches fed by a device queue) and the plain form (one
+launch per program) must give identical statistics.
 """
 import argparse
 import ctypes as C
@@ -41,31 +46,7 @@ def batch_mix(n, seed):
     return progs, seeds
 
 
-def cpu_batch_baseline(n, seed, passes, seconds, threads=16):
-    """The reference's own t1_cycles (oracle/_ref, jit-compiled) on group 0 of the programs of the
-    same mix, on `threads` host threads for about `seconds`: a reported baseline, not the measured
-    path. Like the reference's tester, each call compiles the program for its 512 machines."""
-    from oracle import pyref
-    if not pyref.available():
-        return None
-    s = C.c_uint64(seed)
-    progs, seeds = [], []
-    for k in range(n):
-        d0 = t.lib().ts_splitmix64(C.byref(s))
-        d1 = t.lib().ts_splitmix64(C.byref(s))
-        cells = SIZES[d0 % 5]
-        length = int(math.floor(10 ** (2 + 3 * ((d1 >> 11) / float(1 << 53)))))
-        ps = t.t1_program_seed(seed, cells, k)
-        progs.append(pyref.make("random_program", cells, length, ps))
-        seeds.append(ps)
-    done, ops, secs = pyref.t1_sample(progs, seeds, passes, threads, seconds)
-    return {"value": ops / secs, "unit": "1-bit ops/s", "cores": threads, "kind": "reference",
-            "sample": "reference t1_cycles on group 0 (512 machines) of the first %d programs of "
-                      "the mix, %.1f s" % (done, secs)}
-
-
-def run_batch(engine, n, groups, passes, seed, plain_too=True, lds_cells=0, waves_per_cu=0,
-              cpu_seconds=0):
+def run_batch(engine, n, groups, passes, seed, plain_too=True, lds_cells=0, waves_per_cu=0):
     progs, seeds = batch_mix(n, seed)
     out = {"workload": "batch", "programs": n, "machines_per_program": groups * 512,
            "passes": passes, "instructions": sum(p.len for p in progs)}
@@ -87,8 +68,6 @@ def run_batch(engine, n, groups, passes, seed, plain_too=True, lds_cells=0, wave
         out["launches_plain"] = info_p["launches"]
         out["batched_over_plain"] = info_p["ms"] / info["ms"]
         out["same_results_alone_and_batched"] = stats == stats_p
-    if cpu_seconds:  # the configuration's 10,000 programs, as many as fit the time budget
-        out["cpu_baseline"] = cpu_batch_baseline(max(n, 10000), seed, passes, cpu_seconds)
     hist = np.zeros(64, np.uint64)
     for st in stats:
         hist += np.array(st["hist"], np.uint64)
@@ -97,7 +76,9 @@ def run_batch(engine, n, groups, passes, seed, plain_too=True, lds_cells=0, wave
     return out
 
 
-def run_life(engine, gens, check=True, tune=None):
+def run_life(engine, gens, tune=None, keep=None):
+    """Life on the 8192 x 8192 torus: 64 warm-up generations, then `gens` timed. keep, a dict,
+    receives the LifeCube (its start board and final state) for a caller's own checks."""
     from .life import LifeCube
     t0 = time.perf_counter()
     life = LifeCube(engine, tune=tune)  # 8192 x 8192 torus on 65,536 machines, 32 x 32 tiles
@@ -111,18 +92,8 @@ def run_life(engine, gens, check=True, tune=None):
            "compile_and_setup_s": setup, "ms": ms, "ms_per_generation": ms / gens,
            "cell_updates_per_s": 8192.0 * 8192 * gens / (ms * 1e-3),
            "ops_per_s": life.prog.len * 65536.0 * gens / (ms * 1e-3)}
-    if check:
-        from oracle import pyoracle as O  # the checker only: the run above is finished
-        start = life.start_bits()
-        t0 = time.perf_counter()
-        want = O.life_words(start, gens + 64)
-        cpu_s = time.perf_counter() - t0
-        out["bit_exact_vs_plain_c"] = bool(np.array_equal(life.board_bits(), want))
-        # the checker's bit-parallel plain C Life (64 cells per word, one thread) as the baseline
-        out["cpu_baseline"] = {"value": 8192.0 * 8192 * (gens + 64) / cpu_s, "unit": "cell updates/s",
-                               "cores": 1, "kind": "port",
-                               "sample": "the oracle's bit-parallel C Life, %d generations of the "
-                                         "same board, %.1f s" % (gens + 64, cpu_s)}
+    if keep is not None:
+        keep["life"] = life
     return out
 
 
@@ -156,32 +127,7 @@ def run_hash(engine, target=0x165C2A07):
             "ms": engine.last_ms, "ops_per_s": r["ops"] / (engine.last_ms * 1e-3)}
 
 
-def cpu_serve_baseline(reqs, seconds, threads=16):
-    """The reference's own enum_run (oracle/_ref: its netlist parser and jit, `threads` host
-    threads) on the requests in order until `seconds` have passed: a reported baseline."""
-    from oracle import pyref
-    if not pyref.available():
-        return None
-    t0 = time.perf_counter()
-    ops = done = 0
-    for a, b, c, r, n_in, faulty in reqs:
-        if time.perf_counter() - t0 > seconds:
-            break
-        ra, rb = pyref.Prog(), pyref.Prog()
-        e = C.create_string_buffer(512)
-        for prog, circ in ((ra, c), (rb, r)):
-            if not pyref.lib().net_parse(circ.bench().encode(), b"net", prog.ptr, e, len(e)):
-                raise ValueError(e.value.decode())
-        res = pyref.enum_run(t.EN_EQUIV, ra.ptr, rb.ptr, threads=threads)
-        ops += res["ops"]
-        done += 1
-    secs = time.perf_counter() - t0
-    return {"value": ops / secs, "unit": "1-bit ops/s", "cores": threads, "kind": "reference",
-            "sample": "reference enum_run (jit) on the first %d requests in order, %.1f s"
-                      % (done, secs)}
-
-
-def run_serve(engine, n, rate, seed, streams=4, chunk_words=1 << 24, check=True, cpu_seconds=0):
+def run_serve(engine, n, rate, seed, streams=4, chunk_words=1 << 24, check=True, keep=None):
     """A seeded Poisson stream of equiv requests (netlists.request) submitted at their arrival
     times; latency = done - arrival. Offered load is the stream's; served load is what finished."""
     import threading
@@ -233,8 +179,8 @@ def run_serve(engine, n, rate, seed, streams=4, chunk_words=1 << 24, check=True,
                 "n": int(m.sum()), "p50_ms": float(np.percentile(lat[m], 50) * 1e3),
                 "p99_ms": float(np.percentile(lat[m], 99) * 1e3)}
     out["latency_by_input_space"] = classes
-    if cpu_seconds:
-        out["cpu_baseline"] = cpu_serve_baseline(reqs, cpu_seconds)
+    if keep is not None:
+        keep["requests"] = reqs
     if check:
         bad = 0
         for (a, b, c, r, ni, faulty), res in zip(reqs, results):
@@ -261,17 +207,15 @@ def main():
     ap.add_argument("--cubes", type=int, default=4096)
     ap.add_argument("--lds-cells", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
-    ap.add_argument("--cpu-seconds", type=float, default=0,
-                    help="batch, serve: also time the reference's own code on host threads this long")
     a = ap.parse_args()
     eng = t.Engine(0)
     if a.workload == "batch":
         out = run_batch(eng, a.programs, a.groups, a.passes, a.seed, not a.no_plain, a.lds_cells,
-                        a.waves_per_cu, a.cpu_seconds)
+                        a.waves_per_cu)
     elif a.workload == "life":
         out = run_life(eng, a.gens)
     elif a.workload == "serve":
-        out = run_serve(eng, a.requests, a.rate, a.seed, a.streams, cpu_seconds=a.cpu_seconds)
+        out = run_serve(eng, a.requests, a.rate, a.seed, a.streams)
     elif a.workload == "benes":
         out = run_benes(eng, a.cubes, a.seed)
     else:

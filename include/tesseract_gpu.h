@@ -171,6 +171,19 @@ typedef struct {
 int  gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t n, const gpu_batch_tune *tune,
                    gpu_batch_stats *stats, gpu_batch_info *info, char *err, size_t errlen);
 
+/* Tester 1 on the device: replaces t1_run (src/tester.c:242-279). Program k of size index si is
+   random_program(cells[si], ts_t1_program_len(cfg, cells[si]), t1_program_seed(seed, cells[si], k))
+   run from t1_start of that seed on 512 machines, each machine's cycle length searched up to
+   max_passes passes (< 2^32) with the reference's Brent schedule; out[si] (nsizes entries) gets
+   the per-size summary the reference's workers add up (src/tester.c:182-226). Sizes take turns in
+   the reference's order, and the longest cycle's seed is the first program, in that order, that
+   reaches it (the reference's single-thread result). programs > 0: that many per size in one
+   batch (gpu_batch_run with one group per program); programs = 0: rounds of 64 per size until
+   `seconds` have passed, whole programs only, so unfinished stays 0. *seconds gets the wall time.
+   threads, idle and excluded are ignored. */
+int  gpu_t1_run(gpu_engine *e, const ts_t1_config *cfg, ts_t1_size *out, double *seconds, char *err,
+                size_t errlen);
+
 /* Serving: verify / equiv / search requests submitted over time and run with continuous batching.
    A request is copied, compiled on a pool of host threads (hipRTC), admitted to the device when its
    code fits the memory budget and fewer than max_active requests run, and enumerated in chunks of

@@ -145,6 +145,61 @@ void     ts_t1_start(const ts_program *p, uint64_t seed, unsigned groups, ts_lin
 #define TS_T1_BUCKETS 64
 int      ts_t1_bucket(uint64_t lam);
 
+/* ---- tester 1 summaries (reference src/tester.h:25-57) ----------------------------------------- */
+
+#define TS_T1_SIZES 32
+
+/* Layout-identical to the reference T1Size (src/tester.h:30-37): the results of one machine size. */
+typedef struct {
+    uint32_t cells;
+    size_t   len;
+    uint64_t programs, starts, passes;
+    uint64_t unfinished;             /* programs cut off by a time limit, not in hist */
+    uint64_t hist[TS_T1_BUCKETS];
+    uint64_t longest, longest_seed;  /* longest cycle found, and the seed of the program with it */
+} ts_t1_size;
+
+/* Layout-identical to the reference T1Config (src/tester.h:39-50). threads, idle and excluded
+   choose host processors and mean nothing to a GPU run. */
+typedef struct {
+    uint32_t             cells[TS_T1_SIZES];
+    int                  nsizes;
+    double               ratio;       /* instructions per cell */
+    uint64_t             programs;    /* per size; 0 = until `seconds` are up */
+    double               seconds;
+    uint64_t             max_passes;
+    uint64_t             seed;
+    int                  threads;
+    int                  idle;
+    const unsigned char *excluded;
+} ts_t1_config;
+
+/* round(cells * ratio), at least 1 (reference src/tester.c:176-180) */
+size_t ts_t1_program_len(const ts_t1_config *cfg, uint32_t cells);
+/* to += from; the longest cycle moves only if strictly longer (reference src/tester.c:228-240) */
+void   ts_t1_add_size(ts_t1_size *to, const ts_t1_size *from);
+/* The raw, machine-readable form of a run (reference t1_print_raw, src/tester.c:355-370): a
+   "T1RUN host threads seconds max_passes" line, then one "T1 ..." line per size. Returns a
+   malloc'd string (free it), NULL if out of memory. */
+char  *ts_t1_raw(const ts_t1_size *s, int n, const char *host, double seconds, int threads,
+                 uint64_t max_passes);
+/* Adds up raw outputs read from files (reference t1_merge, src/tester.c:372-431): sizes with the
+   same (cells, len) are summed, threads summed, seconds the largest, max_passes the last seen;
+   lines that are neither form are skipped. Returns the number of sizes, -1 if a file cannot be
+   read. s has room for TS_T1_SIZES. */
+int    ts_t1_merge(char **files, int nfiles, ts_t1_size *s, double *seconds, int *threads,
+                   uint64_t *max_passes);
+
+/* ---- raw enumeration results: shares run apart, merged (reference src/work.c:341-371,435-441) -- */
+
+/* "EN mode space tried hits seconds threads ops" and one "ENHIT x" (hex) per shown hit, the
+   reference's --raw output of verify / equiv / search. Returns a malloc'd string (free it). */
+char *ts_enum_raw(int mode, const ts_enum_result *r);
+/* Adds up such outputs read from files into r: tried, hits, ops and threads summed, seconds the
+   largest, space the last seen, hits kept as the 16 smallest. Returns 0 with a message if a file
+   cannot be opened. */
+int   ts_enum_merge(char **files, int nfiles, ts_enum_result *r, char *err, size_t errlen);
+
 /* ---- Beneš routing (reference src/benes.h) --------------------------------------------------- */
 
 #define TS_BENES_ROUNDS (2 * TS_CUBE_DIMS - 1)

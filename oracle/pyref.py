@@ -3,7 +3,9 @@ oracle/_ref/libalpha_ref.so. Test infrastructure only: it pins the oracle and gi
 baseline its "reference" kind. Absent (and every caller skips) where the build could not run.
 
 Structs mirrored here follow the reference headers: Program (src/isa.h:36-46), Enum
-(src/enum.h:17-24), EnumResult (src/enum.h:26-35), RefMachine (src/ref.h:11-20).
+(src/enum.h:17-24), EnumResult (src/enum.h:26-35), RefMachine (src/ref.h:11-20). t1_run, t1_merge
+and t1_print_raw take T1Config / T1Size pointers (src/tester.h:30-50); callers pass any
+layout-identical structure.
 """
 import ctypes as C
 import os
@@ -86,6 +88,12 @@ def lib():
         L.t1_cycles.argtypes = [vp, C.c_uint64, C.c_uint64, vp, vp, vp]
         L.t1_cycles.restype = C.c_uint64
         L.cpu_has_avx2.restype = C.c_int
+        L.t1_run.argtypes = [vp, vp, C.POINTER(C.c_int)]
+        L.t1_run.restype = C.c_double
+        L.t1_print_raw.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.c_uint64]
+        L.t1_merge.argtypes = [C.POINTER(C.c_char_p), C.c_int, vp, C.POINTER(C.c_double),
+                               C.POINTER(C.c_int), C.POINTER(C.c_uint64)]
+        L.t1_merge.restype = C.c_int
         _r = L
     return _r
 
@@ -175,6 +183,39 @@ def t1_sample(progs, seeds, max_passes, threads, seconds):
     for th in pool:
         th.join()
     return state["done"], state["ops"], time.perf_counter() - t0
+
+
+def t1_run(cfg_ptr, out_ptr):
+    """The reference's tester 1 on host threads (cfg.threads of them): (wall seconds, threads)."""
+    th = C.c_int()
+    wall = lib().t1_run(cfg_ptr, out_ptr, C.byref(th))
+    return wall, th.value
+
+
+def t1_print_raw(sizes_ptr, n, seconds, threads, max_passes):
+    """The text t1_print_raw writes to stdout (captured through file descriptor 1)."""
+    import tempfile
+    libc = C.CDLL(None)
+    with tempfile.TemporaryFile() as tmp:
+        saved = os.dup(1)
+        libc.fflush(None)
+        os.dup2(tmp.fileno(), 1)
+        try:
+            lib().t1_print_raw(sizes_ptr, n, seconds, threads, max_passes)
+            libc.fflush(None)
+        finally:
+            os.dup2(saved, 1)
+            os.close(saved)
+        tmp.seek(0)
+        return tmp.read().decode()
+
+
+def t1_merge(paths, out_ptr):
+    """The reference's merge of raw files: (sizes, seconds, threads, max_passes)."""
+    files = (C.c_char_p * len(paths))(*[os.fsencode(p) for p in paths])
+    sec, th, mp = C.c_double(), C.c_int(), C.c_uint64()
+    n = lib().t1_merge(files, len(paths), out_ptr, C.byref(sec), C.byref(th), C.byref(mp))
+    return n, sec.value, th.value, mp.value
 
 
 def cube_run(prog_ptr, ncells, mem, rr, passes, steps=0):

@@ -92,6 +92,21 @@ class GpuTapeStruct(C.Structure):
                 ("root", C.c_uint32), ("slots", C.c_uint32), ("and_nodes", C.c_uint64)]
 
 
+class T1SizeStruct(C.Structure):
+    """Layout-identical to the reference's T1Size (src/tester.h:30-37)."""
+    _fields_ = [("cells", C.c_uint32), ("len", C.c_size_t), ("programs", C.c_uint64),
+                ("starts", C.c_uint64), ("passes", C.c_uint64), ("unfinished", C.c_uint64),
+                ("hist", C.c_uint64 * 64), ("longest", C.c_uint64), ("longest_seed", C.c_uint64)]
+
+
+class T1ConfigStruct(C.Structure):
+    """Layout-identical to the reference's T1Config (src/tester.h:39-50)."""
+    _fields_ = [("cells", C.c_uint32 * 32), ("nsizes", C.c_int), ("ratio", C.c_double),
+                ("programs", C.c_uint64), ("seconds", C.c_double), ("max_passes", C.c_uint64),
+                ("seed", C.c_uint64), ("threads", C.c_int), ("idle", C.c_int),
+                ("excluded", C.c_void_p)]
+
+
 class GpuEnumStruct(C.Structure):
     _fields_ = [("mode", C.c_int), ("a", C.POINTER(ProgramStruct)), ("b", C.POINTER(ProgramStruct)),
                 ("spec", C.c_char_p), ("ticks", C.c_uint), ("part", C.c_int), ("part_end", C.c_int),
@@ -138,6 +153,15 @@ def lib():
             "ts_spec_value": (u64, [vp, u64]),
             "ts_spec_out_cell": (u32, [vp, i32]),
             "ts_benes_route": (None, [C.POINTER(C.c_uint16), vp]),
+            "ts_t1_program_len": (sz, [C.POINTER(T1ConfigStruct), u32]),
+            "ts_t1_raw": (vp, [C.POINTER(T1SizeStruct), i32, C.c_char_p, C.c_double, i32, u64]),
+            "ts_t1_merge": (i32, [C.POINTER(C.c_char_p), i32, C.POINTER(T1SizeStruct),
+                                  C.POINTER(C.c_double), C.POINTER(i32), C.POINTER(u64)]),
+            "ts_enum_raw": (vp, [i32, C.POINTER(EnumResultStruct)]),
+            "ts_enum_merge": (i32, [C.POINTER(C.c_char_p), i32, C.POINTER(EnumResultStruct),
+                                    C.c_char_p, sz]),
+            "gpu_t1_run": (i32, [vp, C.POINTER(T1ConfigStruct), C.POINTER(T1SizeStruct),
+                                 C.POINTER(C.c_double), C.c_char_p, sz]),
             "ts_benes_dim": (i32, [i32]),
             "ts_b_new": (vp, []),
             "ts_b_cell": (u32, [vp, C.c_char_p]),
@@ -456,6 +480,91 @@ def _take_string(ptr):
     return s
 
 
+# ---- tester 1 summaries and raw results ------------------------------------------------------
+
+def t1_config(cells=(8, 16, 32, 64, 128, 256), ratio=4.0, programs=64, max_passes=1 << 20, seed=1,
+              seconds=0.0):
+    """A T1Config with the reference's tester1 defaults (src/main.c:1071-1081)."""
+    c = T1ConfigStruct()
+    for i, n in enumerate(cells):
+        c.cells[i] = n
+    c.nsizes = len(cells)
+    c.ratio = ratio
+    c.programs = programs
+    c.seconds = seconds
+    c.max_passes = max_passes
+    c.seed = seed
+    return c
+
+
+def t1_program_len(cfg, cells):
+    return int(lib().ts_t1_program_len(C.byref(cfg), cells))
+
+
+def t1_size_dict(x):
+    return {"cells": x.cells, "len": x.len, "programs": x.programs, "starts": x.starts,
+            "passes": x.passes, "unfinished": x.unfinished, "hist": list(x.hist),
+            "longest": x.longest, "longest_seed": x.longest_seed}
+
+
+def _t1_sizes(sizes):
+    arr = (T1SizeStruct * max(1, len(sizes)))()
+    for i, d in enumerate(sizes):
+        for k, v in d.items():
+            if k == "hist":
+                for b, h in enumerate(v):
+                    arr[i].hist[b] = h
+            else:
+                setattr(arr[i], k, v)
+    return arr
+
+
+def t1_raw(sizes, host, seconds, threads, max_passes):
+    """ts_t1_raw: the reference's tester1 --raw text for per-size summaries (dicts)."""
+    arr = _t1_sizes(sizes)
+    ptr = lib().ts_t1_raw(arr, len(sizes), host.encode(), seconds, threads, max_passes)
+    if not ptr:
+        raise MemoryError
+    return _take_string(ptr)
+
+
+def t1_merge(paths):
+    """ts_t1_merge over raw files: (sizes, seconds, threads, max_passes)."""
+    files = (C.c_char_p * len(paths))(*[os.fsencode(p) for p in paths])
+    out = (T1SizeStruct * 32)()
+    sec, th, mp = C.c_double(), C.c_int(), C.c_uint64()
+    n = lib().ts_t1_merge(files, len(paths), out, C.byref(sec), C.byref(th), C.byref(mp))
+    if n < 0:
+        raise OSError("cannot read a file to merge")
+    return [t1_size_dict(out[i]) for i in range(n)], sec.value, th.value, mp.value
+
+
+def enum_raw(mode, result):
+    """ts_enum_raw: the reference's verify/equiv/search --raw text for a result dict."""
+    r = EnumResultStruct()
+    r.space, r.tried, r.hits = result["space"], result["tried"], result["hits"]
+    for i, x in enumerate(result["hit"]):
+        r.hit[i] = x
+    r.nhit = len(result["hit"])
+    r.seconds, r.ops, r.threads = result["seconds"], result["ops"], result.get("threads", 0)
+    ptr = lib().ts_enum_raw(mode, C.byref(r))
+    if not ptr:
+        raise MemoryError
+    return _take_string(ptr)
+
+
+def enum_merge(paths):
+    """ts_enum_merge over raw files: the summed result (with its threads)."""
+    files = (C.c_char_p * len(paths))(*[os.fsencode(p) for p in paths])
+    r = EnumResultStruct()
+    e = _err()
+    if not lib().ts_enum_merge(files, len(paths), C.byref(r), e, len(e)):
+        raise OSError(e.value.decode())
+    out = _result(r)
+    out["threads"] = r.threads
+    return out
+
+
 def source(prog, tune=None):
     """The HIP source gpu_compile would build for prog, and its code statistics (no device)."""
     out = C.c_void_p()
@@ -598,6 +707,15 @@ class Engine:
                 "passes": st.passes} for st in stats]
         return out, {"ms": info.ms, "ops": info.ops, "launches": info.launches,
                      "class_ms": list(info.class_ms), "class_ops": list(info.class_ops)}
+
+    def t1_run(self, cfg):
+        """gpu_t1_run: tester 1 on the device; (per-size summaries, seconds)."""
+        out = (T1SizeStruct * max(1, cfg.nsizes))()
+        sec = C.c_double()
+        e = _err()
+        if not lib().gpu_t1_run(self._e, C.byref(cfg), out, C.byref(sec), e, len(e)):
+            raise ValueError("gpu_t1_run: " + e.value.decode())
+        return [t1_size_dict(out[i]) for i in range(cfg.nsizes)], sec.value
 
     def compile(self, prog, tune=None, listing=False):
         return Code(self, prog, tune, listing)

@@ -15,8 +15,8 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libtesseract.so")
 
-HOST_C = ["program", "builder", "gen", "spec", "tester", "benes", "net"]
-ENGINE_CPP = ["lower", "codegen", "tape", "tape_kernel", "engine", "calibrate", "batch", "serve"]
+HOST_C = ["program", "builder", "gen", "spec", "tester", "benes", "net", "report"]
+ENGINE_CPP = ["lower", "codegen", "tape", "tape_kernel", "engine", "calibrate", "batch", "serve", "tester1"]
 ARCH = "gfx950"
 # Packed FP32 stays off in every HIP compile.
 NO_PACKED = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

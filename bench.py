@@ -19,6 +19,8 @@ and adds beside its device figures a CPU baseline and the check of its results:
          oracle's bit-parallel C Life from the same start; that C Life timed as the baseline
   batch  the random-program mix; the reference's t1_cycles on group 0 of the first programs
   serve  the seeded equiv request stream; the reference's enum_run on the first requests
+  tester1  tester 1 (gpu_t1_run) at the reference's defaults; the reference's own t1_run on the
+         same programs on host threads, its summary compared with the device's
 """
 import argparse
 import json
@@ -132,6 +134,43 @@ def life_check(life, gens):
     return exact, base
 
 
+def run_tester1(args):
+    """Tester 1 on the device and the reference's t1_run on host threads over the same programs:
+    the per-size summaries must agree (the longest cycle's seed aside, which depends on the order
+    the host threads finish in)."""
+    import ctypes as C
+    import tesseract_amd as t
+    eng = t.Engine(0)
+    cells = [int(c) for c in args.cells.split(",")]
+    cfg = t.t1_config(cells, args.ratio, args.programs, args.max_passes, args.seed)
+    eng.t1_run(t.t1_config(cells, args.ratio, 1, 1 << 10, args.seed))  # warm-up
+    got, gsec = eng.t1_run(cfg)
+    ops = sum(d["passes"] * d["len"] * 512 for d in got)  # src/tester.c:302-305
+    out = {"workload": "tester1", "cells": cells, "ratio": args.ratio,
+           "programs_per_size": args.programs, "max_passes": args.max_passes, "seed": args.seed,
+           "seconds": gsec, "ops": ops, "ops_per_s": ops / gsec,
+           "longest": [d["longest"] for d in got],
+           "none_found": [d["hist"][63] for d in got], "cpu_baseline": None}
+    if not args.no_cpu:
+        from oracle import pyref
+        if pyref.available():
+            cfg.threads = args.cpu_threads
+            ref = (t.T1SizeStruct * len(cells))()
+            rsec, th = pyref.t1_run(C.byref(cfg), ref)
+            ref = [t.t1_size_dict(x) for x in ref]
+            strip = lambda L: [{k: v for k, v in d.items() if k != "longest_seed"} for d in L]
+            out["same_summary_as_reference"] = strip(got) == strip(ref)
+            out["cpu_baseline"] = {"value": ops / rsec, "unit": "1-bit ops/s", "cores": th,
+                                   "kind": "reference",
+                                   "sample": "the reference's t1_run on the same %d programs, "
+                                             "%.2f s" % (len(cells) * args.programs, rsec)}
+            if not out["same_summary_as_reference"]:
+                print(json.dumps(out), flush=True)
+                sys.exit("tester1: the device summary differs from the reference's")
+    out["device"] = eng.describe()
+    print(json.dumps(out), flush=True)
+
+
 def run_workload(args):
     """One secondary workload, its checks and its CPU baseline, as one JSON line."""
     import tesseract_amd as t
@@ -187,9 +226,14 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="afterwards copy a MIB-MiB device buffer (copy_probe): a kernel of known "
                          "bytes whose FETCH_SIZE/WRITE_SIZE readings scale the kernel's")
-    ap.add_argument("--workload", choices=["mul16", "life", "batch", "serve"], default="mul16")
+    ap.add_argument("--workload", choices=["mul16", "life", "batch", "serve", "tester1"],
+                    default="mul16")
     ap.add_argument("--gens", type=int, default=1000, help="life: timed generations")
-    ap.add_argument("--programs", type=int, default=1000, help="batch: programs of the mix")
+    ap.add_argument("--programs", type=int, default=None,
+                    help="batch: programs of the mix (1000); tester1: programs per size (1024)")
+    ap.add_argument("--cells", default="8,16,32,64,128,256", help="tester1: machine sizes")
+    ap.add_argument("--ratio", type=float, default=4.0, help="tester1: instructions per cell")
+    ap.add_argument("--max-passes", type=int, default=1 << 20, help="tester1: cycle search limit")
     ap.add_argument("--groups", type=int, default=2048, help="batch: 512-machine groups each")
     ap.add_argument("--passes", type=int, default=64, help="batch: most passes per group")
     ap.add_argument("--requests", type=int, default=100, help="serve: requests of the stream")
@@ -202,7 +246,14 @@ def main():
         print("bench.py measures one GPU and has no multi-GPU path (each enumeration share would "
               "run on its own GPU with a host-side merge); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "tester1":
+        if args.programs is None:
+            args.programs = 1024
+        run_tester1(args)
+        return
     if args.workload != "mul16":
+        if args.programs is None:
+            args.programs = 1000
         run_workload(args)
         return
 

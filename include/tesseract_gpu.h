@@ -137,7 +137,8 @@ void gpu_enum_free(gpu_enum_code *c);
    pool (src/tester.c:182-279): programs are data for a persistent interpreter kernel fed by a device
    work queue, not compiled one by one. stats[i] is program i's histogram (bucket b: cycle length
    in (2^(b-1), 2^b], src/tester.c:42-48; bucket 63: none within max_passes). plain = 1 runs the same
-   kernel one launch per program, the baseline a batch is compared with. */
+   kernel one launch per program, the baseline a batch is compared with. A work item is one wave
+   of 64 words (2,048 machines); a program of one or two groups gets items of 16 or 32 words. */
 typedef struct {
     const ts_program *prog;
     uint64_t          seed;
@@ -154,7 +155,9 @@ typedef struct {
     unsigned groups;        /* 512-machine groups per program (default 2048 = 2^20 machines) */
     unsigned max_passes;    /* default 64 */
     unsigned plain;
-    unsigned lds_cells;     /* programs with ncells + 1 <= this keep their machines in LDS (default 130) */
+    unsigned lds_cells;     /* programs with ncells + 1 <= this keep their machines in LDS (default:
+                               as many bytes as 130 cells of 2,048 machines, more when few work
+                               items share a CU) */
     unsigned waves_per_cu;  /* persistent waves per CU per class, LDS permitting (default 16) */
 } gpu_batch_tune;
 

@@ -12,9 +12,12 @@
 // per-wave sink cell and a STO slot loading from it, and rr = M ? !(rr & v) : rr is a single
 // v_bitop3_b32 with the slot's wave-uniform mask M. A window is 16 words (one scalar load): per
 // slot the byte offset of the cell loaded (bit 31 = NAND) and of the cell stored.
-// A work item is 64 words (2,048 machines) of one program: one wave. The machines' cells live in
-// LDS (64 lanes x cells x 4 bytes, hare and tortoise) for small programs, or in a per-wave region
-// of HBM for large ones (coalesced: 256 contiguous bytes per cell). Start states come from the
+// A work item is 64 words (2,048 machines) of one program: one wave. A program of one or two
+// groups (the reference's tester runs one: 512 machines) gets items of 16 or 32 words, the wave's
+// other lanes idle, so that its cells take a quarter or half the room. The hare's cells live in
+// LDS (item words x cells x 4 bytes) for small programs, or in a per-wave region of HBM for large
+// ones (coalesced: one contiguous row per cell); the tortoise in HBM, or beside the hare in LDS
+// when so few items run that the LDS of a CU holds both for all of them. Start states come from the
 // reference's t1_start stream, generated in registers. Brent's schedule gives each machine's
 // cycle length; a 512-machine group stops once all its machines found theirs or after max_passes
 // passes (the reference's t1_cycles stops a group the same way, src/tester.c:131-144), a wave
@@ -56,16 +59,18 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z)
     return z ^ (z >> 31);
 }
 
-// Storage of one wave's machines: element (cell, lane) at byte cell * 256 + lane * 4 of the base.
+// Storage of one item's machines: element (cell, lane) at byte cell * row + lane * 4 of the base,
+// row = 4 x the item's words.
 struct Cells {
     uint8_t *base;
+    uint32_t row;
     __device__ __forceinline__ uint32_t ld(uint32_t cell, uint32_t lane) const
     {
-        return *(const uint32_t *)(base + cell * (kLanes * 4) + lane * 4);
+        return *(const uint32_t *)(base + cell * row + lane * 4);
     }
     __device__ __forceinline__ void st(uint32_t cell, uint32_t lane, uint32_t v) const
     {
-        *(uint32_t *)(base + cell * (kLanes * 4) + lane * 4) = v;
+        *(uint32_t *)(base + cell * row + lane * 4) = v;
     }
     __device__ __forceinline__ uint32_t ld_at(uint32_t byte_off, uint32_t lane4) const
     {
@@ -77,6 +82,26 @@ struct Cells {
     }
 };
 
+// One window: its 8 loads issue together, then the 8 slots run in order.
+__device__ __forceinline__ void run_window(const Cells &h, uint32_t lane4, const uint32_t (&w)[16],
+                                           uint32_t &rr)
+{
+    uint32_t v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        v[k] = h.ld_at(w[2 * k] & 0x7FFFFFFFu, lane4);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const uint32_t m = (uint32_t)((int32_t)w[2 * k] >> 31);  // all ones for a NAND
+        rr = __builtin_amdgcn_bitop3_b32(m, rr, v[k], 0x7Cu);   // m ? !(rr & v) : rr
+        h.st_at(w[2 * k + 1], lane4, rr);
+    }
+}
+
+// One pass: each window's record through the scalar cache as the window starts. A lone wave
+// (a search of few programs) spends about 27 cycles per instruction here whether its cells are in
+// LDS or HBM and whether the records are fetched ahead or not: the slots' dependent chain, not a
+// memory latency, sets the pace.
 __device__ __forceinline__ void run_pass(const Cells &h, uint32_t lane, const uint4 *rec,
                                          uint32_t nwin, uint32_t &rr)
 {
@@ -85,24 +110,18 @@ __device__ __forceinline__ void run_pass(const Cells &h, uint32_t lane, const ui
         const uint4 q0 = rec[4 * r], q1 = rec[4 * r + 1], q2 = rec[4 * r + 2], q3 = rec[4 * r + 3];
         const uint32_t w[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
                                 q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
-        uint32_t v[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++)
-            v[k] = h.ld_at(w[2 * k] & 0x7FFFFFFFu, lane4);
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const uint32_t m = (uint32_t)((int32_t)w[2 * k] >> 31);  // all ones for a NAND
-            rr = __builtin_amdgcn_bitop3_b32(m, rr, v[k], 0x7Cu);   // m ? !(rr & v) : rr
-            h.st_at(w[2 * k + 1], lane4, rr);
-        }
+        run_window(h, lane4, w, rr);
     }
 }
 
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+// sum over the wave's active lanes of popcount(m): one ballot per bit, so lanes that idle (items
+// narrower than the wave) take no part
+__device__ __forceinline__ uint32_t wave_popcount(uint32_t m)
 {
-    for (int o = 32; o >= 1; o >>= 1)
-        v += __shfl_xor(v, o, 64);
-    return v;
+    uint32_t n = 0;
+    for (int b = 0; b < 32; b++)
+        n += (uint32_t)__builtin_popcountll(__ballot((m >> b) & 1u));
+    return n;
 }
 
 __device__ __forceinline__ int bucket_of(uint64_t lam)
@@ -116,7 +135,7 @@ __device__ __forceinline__ int bucket_of(uint64_t lam)
 // lanes in `found` reached their cycle, of length lam
 __device__ __forceinline__ void record(uint64_t *st, uint32_t found, uint64_t lam, uint32_t lane)
 {
-    uint32_t n = wave_sum((uint32_t)__builtin_popcount(found));
+    uint32_t n = wave_popcount(found);
     if (n && lane == 0) {
         atomicAdd((unsigned long long *)&st[bucket_of(lam)], (unsigned long long)n);
         atomicAdd((unsigned long long *)&st[64], (unsigned long long)(n * lam));
@@ -124,7 +143,7 @@ __device__ __forceinline__ void record(uint64_t *st, uint32_t found, uint64_t la
     }
 }
 
-template <bool LDS>
+template <bool LDS, bool TORT_LDS>
 __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ progs,
                                                const uint32_t *__restrict__ item_prog,
                                                uint32_t item_base, uint32_t nitems,
@@ -133,13 +152,14 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
                                                uint32_t *__restrict__ queue,
                                                uint32_t *__restrict__ scratch, uint32_t region_cells,
                                                uint64_t *__restrict__ stats, uint32_t max_passes,
-                                               uint32_t prog_words)
+                                               uint32_t prog_words, uint32_t item_words)
 {
     extern __shared__ uint32_t lds[];
     const uint32_t lane = threadIdx.x;
+    const uint32_t row = item_words * 4;
     // the hare (every instruction touches it) in LDS or HBM; the tortoise (Brent's compare and
-    // copy only) always in this wave's HBM region
-    uint8_t *hbm = (uint8_t *)(scratch + (size_t)blockIdx.x * 2 * region_cells * kLanes);
+    // copy only) in this wave's HBM region or after the hare in LDS
+    uint8_t *hbm = (uint8_t *)(scratch + (size_t)blockIdx.x * 2 * region_cells * item_words);
     for (;;) {
         uint32_t item = 0;
         if (lane == 0)
@@ -147,94 +167,114 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
         item = __builtin_amdgcn_readfirstlane(__shfl(item, 0, 64));
         if (item >= nitems)
             break;
-        item += item_base;
-        const uint32_t pi = __builtin_amdgcn_readfirstlane(item_prog[item]);
-        const BatchProg P = progs[pi];
-        const uint32_t ncells = P.ncells;
-        Cells hare{LDS ? (uint8_t *)lds : hbm}, tort{hbm + (size_t)region_cells * kLanes * 4};
-        const uint4 *rec = recs + P.rec_off;
-        uint64_t *st = stats + (size_t)pi * kStats;
+        if (lane < item_words) {  // an item narrower than the wave: the other lanes idle
+            item += item_base;
+            const uint32_t pi = __builtin_amdgcn_readfirstlane(item_prog[item]);
+            const BatchProg P = progs[pi];
+            const uint32_t ncells = P.ncells;
+            Cells hare{LDS ? (uint8_t *)lds : hbm, row};
+            Cells tort{(TORT_LDS ? (uint8_t *)lds : hbm) + (size_t)region_cells * row, row};
+            const uint4 *rec = recs + P.rec_off;
+            uint64_t *st = stats + (size_t)pi * kStats;
 
-        // start state: group g = word / 16, u32 word j of its line; the stream continues group
-        // after group, cells 2.. then RR, 8 u64 words each (reference src/tester.c:83-91)
-        const uint32_t word = (item - P.first_item) * kLanes + lane;
-        // a program of fewer than 4 groups fills part of its last item: the other lanes run a
-        // throwaway machine and record nothing
-        const uint64_t g = word >> 4;
-        const uint32_t w64 = (word & 15) >> 1, hi = word & 1;
-        const uint64_t s0 = P.seed ^ 0x5DEECE66Dull, per_group = (uint64_t)(ncells - 1) * 8;
-        for (uint32_t c = 2; c < ncells; c++) {
-            uint64_t k = g * per_group + (uint64_t)(c - 2) * 8 + w64;
-            uint64_t v = mix64(s0 + (k + 1) * 0x9E3779B97F4A7C15ull);
-            hare.st(c, lane, hi ? (uint32_t)(v >> 32) : (uint32_t)v);
-        }
-        hare.st(0, lane, 0u);
-        hare.st(1, lane, 0xFFFFFFFFu);
-        uint32_t rr;
-        {
-            uint64_t k = g * per_group + (uint64_t)(ncells - 2) * 8 + w64;
-            uint64_t v = mix64(s0 + (k + 1) * 0x9E3779B97F4A7C15ull);
-            rr = hi ? (uint32_t)(v >> 32) : (uint32_t)v;
-        }
+            // start state: group g = word / 16, u32 word j of its line; the stream continues group
+            // after group, cells 2.. then RR, 8 u64 words each (reference src/tester.c:83-91)
+            const uint32_t word = (item - P.first_item) * item_words + lane;
+            // a program of fewer groups than its items hold fills part of its last item: the other
+            // lanes run a throwaway machine and record nothing
+            const uint64_t g = word >> 4;
+            const uint32_t w64 = (word & 15) >> 1, hi = word & 1;
+            const uint64_t s0 = P.seed ^ 0x5DEECE66Dull, per_group = (uint64_t)(ncells - 1) * 8;
+            for (uint32_t c = 2; c < ncells; c++) {
+                uint64_t k = g * per_group + (uint64_t)(c - 2) * 8 + w64;
+                uint64_t v = mix64(s0 + (k + 1) * 0x9E3779B97F4A7C15ull);
+                hare.st(c, lane, hi ? (uint32_t)(v >> 32) : (uint32_t)v);
+            }
+            hare.st(0, lane, 0u);
+            hare.st(1, lane, 0xFFFFFFFFu);
+            uint32_t rr;
+            {
+                uint64_t k = g * per_group + (uint64_t)(ncells - 2) * 8 + w64;
+                uint64_t v = mix64(s0 + (k + 1) * 0x9E3779B97F4A7C15ull);
+                rr = hi ? (uint32_t)(v >> 32) : (uint32_t)v;
+            }
 
-        // Brent: the tortoise jumps to the hare at powers of two (reference src/tester.c:96-164)
-        for (uint32_t c = 0; c < ncells; c++)
-            tort.st(c, lane, hare.ld(c, lane));
-        uint32_t trr = rr;
-        run_pass(hare, lane, rec, P.nrec, rr);
-        uint64_t lam = 1, power = 1, steps = 1;
-        uint32_t active = word < prog_words ? 0xFFFFFFFFu : 0u;
-        const uint32_t *smp = samples + P.sample_off;
-        // the wave's 4 groups (16 lanes each) that hold machines of the program, and those still
-        // searching; a group's passes are the steps at which its last machine found its cycle
-        const uint64_t real = __ballot(word < prog_words);
-        uint32_t open_groups = 0;
-        for (int k = 0; k < 4; k++)
-            open_groups |= ((real >> (16 * k)) & 0xFFFFull) ? 1u << k : 0u;
-        uint64_t passes = 0;
-        for (;;) {
-            uint32_t d = rr ^ trr;
-            for (uint32_t i = 0; i < P.nsample; i++) {
-                uint32_t c = smp[i];
-                d |= hare.ld(c, lane) ^ tort.ld(c, lane);
-            }
-            if (P.nsample < ncells - 2 && __any(active & ~d)) {
-                d = rr ^ trr;
-                for (uint32_t c = 2; c < ncells; c++)
-                    d |= hare.ld(c, lane) ^ tort.ld(c, lane);
-            }
-            const uint32_t found = active & ~d;
-            if (__any(found))
-                record(st, found, lam, lane);
-            active &= d;
-            const uint64_t searching = __ballot(active != 0);
-            for (int k = 0; k < 4; k++)
-                if ((open_groups >> k & 1) && !((searching >> (16 * k)) & 0xFFFFull)) {
-                    open_groups &= ~(1u << k);
-                    passes += steps;
-                }
-            if (!open_groups)
-                break;
-            if (steps >= max_passes) {
-                passes += steps * (uint64_t)__builtin_popcount(open_groups);
-                break;
-            }
-            if (power == lam) {
-                for (uint32_t c = 2; c < ncells; c++)
-                    tort.st(c, lane, hare.ld(c, lane));
-                trr = rr;
-                power *= 2;
-                lam = 0;
-            }
+            // Brent: the tortoise jumps to the hare at powers of two (reference src/tester.c:96-164)
+            for (uint32_t c = 0; c < ncells; c++)
+                tort.st(c, lane, hare.ld(c, lane));
+            uint32_t trr = rr;
             run_pass(hare, lane, rec, P.nrec, rr);
-            lam++;
-            steps++;
+            uint64_t lam = 1, power = 1, steps = 1;
+            uint32_t active = word < prog_words ? 0xFFFFFFFFu : 0u;
+            const uint32_t *smp = samples + P.sample_off;
+            // the wave's 4 groups (16 lanes each) that hold machines of the program, and those still
+            // searching; a group's passes are the steps at which its last machine found its cycle
+            const uint64_t real = __ballot(word < prog_words);
+            uint32_t open_groups = 0;
+            for (int k = 0; k < 4; k++)
+                open_groups |= ((real >> (16 * k)) & 0xFFFFull) ? 1u << k : 0u;
+            uint64_t passes = 0;
+            for (;;) {
+                // the sample list is padded to a multiple of 8 (repeats of its last cell), so eight
+                // loads of each side are in flight at once
+                uint32_t d = rr ^ trr;
+                for (uint32_t i = 0; i < P.nsample; i += 8) {
+                    uint32_t x[8];
+#pragma unroll
+                    for (int k = 0; k < 8; k++)
+                        x[k] = hare.ld(smp[i + k], lane) ^ tort.ld(smp[i + k], lane);
+#pragma unroll
+                    for (int k = 0; k < 8; k++)
+                        d |= x[k];
+                }
+                if (P.nsample < ncells - 2 && __any(active & ~d)) {
+                    d = rr ^ trr;
+                    uint32_t c = 2;
+                    for (; c + 8 <= ncells; c += 8) {
+                        uint32_t x[8];
+#pragma unroll
+                        for (int k = 0; k < 8; k++)
+                            x[k] = hare.ld(c + k, lane) ^ tort.ld(c + k, lane);
+#pragma unroll
+                        for (int k = 0; k < 8; k++)
+                            d |= x[k];
+                    }
+                    for (; c < ncells; c++)
+                        d |= hare.ld(c, lane) ^ tort.ld(c, lane);
+                }
+                const uint32_t found = active & ~d;
+                if (__any(found))
+                    record(st, found, lam, lane);
+                active &= d;
+                const uint64_t searching = __ballot(active != 0);
+                for (int k = 0; k < 4; k++)
+                    if ((open_groups >> k & 1) && !((searching >> (16 * k)) & 0xFFFFull)) {
+                        open_groups &= ~(1u << k);
+                        passes += steps;
+                    }
+                if (!open_groups)
+                    break;
+                if (steps >= max_passes) {
+                    passes += steps * (uint64_t)__builtin_popcount(open_groups);
+                    break;
+                }
+                if (power == lam) {
+                    for (uint32_t c = 2; c < ncells; c++)
+                        tort.st(c, lane, hare.ld(c, lane));
+                    trr = rr;
+                    power *= 2;
+                    lam = 0;
+                }
+                run_pass(hare, lane, rec, P.nrec, rr);
+                lam++;
+                steps++;
+            }
+            uint32_t left = wave_popcount(active);
+            if (left && lane == 0)
+                atomicAdd((unsigned long long *)&st[63], (unsigned long long)left);
+            if (lane == 0)
+                atomicAdd((unsigned long long *)&st[66], (unsigned long long)passes);
         }
-        uint32_t left = wave_sum((uint32_t)__builtin_popcount(active));
-        if (left && lane == 0)
-            atomicAdd((unsigned long long *)&st[63], (unsigned long long)left);
-        if (lane == 0)
-            atomicAdd((unsigned long long *)&st[66], (unsigned long long)passes);
         __syncthreads();  // one wave per block: orders this item's LDS use before the next
     }
 }
@@ -245,12 +285,12 @@ struct Encoded {
 };
 
 // Windows of up to 8 instructions in which no NAND reads a cell stored earlier in the window.
-// Offsets are bytes into a wave's cells (cell * 256). A NAND stores to the sink cell, a STO loads
+// Offsets are bytes into an item's cells (cell * row). A NAND stores to the sink cell, a STO loads
 // from it; STO to read-only cells (ignored by the machine) and padding store to it too.
-Encoded encode(const ts_program *p)
+Encoded encode(const ts_program *p, uint32_t row)
 {
     Encoded e;
-    const uint32_t sink = p->ncells * kLanes * 4;
+    const uint32_t sink = p->ncells * row;
     std::vector<uint32_t> win;
     std::vector<uint32_t> stored;
     auto flush = [&]() {
@@ -268,13 +308,13 @@ Encoded encode(const ts_program *p)
         if (in.op == TS_OP_NAND) {
             if (std::find(stored.begin(), stored.end(), in.addr) != stored.end())
                 flush();
-            win.push_back(in.addr * kLanes * 4 | 0x80000000u);
+            win.push_back(in.addr * row | 0x80000000u);
             win.push_back(sink);
         } else {
             bool ro = in.addr < TS_CELL_FIRST_FREE ||
                       (p->special && p->special[in.addr].kind != TS_CK_PLAIN);
             win.push_back(sink);
-            win.push_back(ro ? sink : in.addr * kLanes * 4);
+            win.push_back(ro ? sink : in.addr * row);
             if (!ro)
                 stored.push_back(in.addr);
         }
@@ -344,7 +384,21 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     }
     BCHK(hipSetDevice(e->device));
     const uint32_t prog_words = tune.groups * 16;
-    const uint32_t items_per_prog = (prog_words + kLanes - 1) / kLanes;
+    // items as wide as the program needs, up to the wave: 16 words for one group, 32 for two
+    const uint32_t item_words = tune.groups == 1 ? 16 : tune.groups == 2 ? 32 : kLanes;
+    const uint32_t row = item_words * 4;  // bytes of one cell of an item
+    const uint32_t items_per_prog = (prog_words + item_words - 1) / item_words;
+    const size_t lds_max = e->lds_per_block;
+    if (!tune_in || !tune_in->lds_cells) {
+        // as many bytes per item as 130 cells of a full wave; more when so few items run that
+        // each CU holds only a few (a search of few large programs keeps its hare in LDS)
+        const uint64_t items_total = (uint64_t)n * items_per_prog;
+        const uint64_t per_cu_needed = std::max<uint64_t>(1, (items_total + e->cus - 1) / e->cus);
+        const uint64_t fit = lds_max / per_cu_needed / row;  // cells, the sink included
+        tune.lds_cells = (unsigned)std::max<uint64_t>(130ull * kLanes / item_words,
+                                                      fit ? fit - 1 : 0);
+    }
+    tune.lds_cells = (unsigned)std::min<uint64_t>(tune.lds_cells, lds_max / row - 1);
 
     // host encoding, by residency class: 0 = LDS (<= 32 cells with the sink), 1 = LDS (<= lds_cells),
     // 2 = HBM; each class is one persistent launch sized by what a CU can hold
@@ -360,7 +414,7 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     uint64_t ops = 0;
     for (size_t i = 0; i < n; i++) {
         const ts_program *p = items[i].prog;
-        Encoded enc = encode(p);
+        Encoded enc = encode(p, row);
         BatchProg &bp = progs[i];
         bp.rec_off = (uint32_t)(recs.size() / 4);
         bp.nrec = enc.nwin;
@@ -371,6 +425,8 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
         bp.sample_off = (uint32_t)samples.size();
         for (uint32_t k = 0; k < bp.nsample; k++)
             samples.push_back(TS_CELL_FIRST_FREE + (uint32_t)((uint64_t)k * writable / bp.nsample));
+        while ((samples.size() - bp.sample_off) % 8)  // the kernel compares 8 at a time
+            samples.push_back(samples.back());
         bp.seed = items[i].seed;
         int cls = class_of(p);
         bp.first_item = (uint32_t)item_of[cls].size();
@@ -391,21 +447,42 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
         say(err, errlen, "%s", "out of device memory for the batch");
         return 0;
     }
-    // per class: persistent waves per CU (LDS bounds the LDS classes) and its HBM regions
+    // per class: persistent waves per CU (LDS bounds the LDS classes; the HBM regions of all
+    // waves stay within 16 GiB) and its HBM regions (hare and tortoise per wave)
     unsigned per_cu[3];
+    bool tort_lds[3] = {false, false, false};
     size_t scratch_off[4] = {0, 0, 0, 0};
+    constexpr size_t kRegionBudget = (size_t)16 << 30;
     for (int c = 0; c < 3; c++) {
         per_cu[c] = tune.waves_per_cu;
-        if (c < 2 && maxcells[c])
+        if (c < 2 && maxcells[c]) {
             per_cu[c] = (unsigned)std::max<size_t>(
-                1, std::min<size_t>(per_cu[c], (160u << 10) / ((size_t)maxcells[c] * kLanes * 4)));
-        scratch_off[c + 1] = scratch_off[c] + (size_t)e->cus * per_cu[c] * 2 * maxcells[c] * kLanes;
+                1, std::min<size_t>(per_cu[c], lds_max / ((size_t)maxcells[c] * row)));
+            // the tortoise joins the hare in LDS when every item of the class still finds a
+            // wave slot (Brent compares the whole state each pass once the sample matches)
+            const size_t both = std::min<size_t>(tune.waves_per_cu,
+                                                 lds_max / (2 * (size_t)maxcells[c] * row));
+            if (both && item_of[c].size() <= (size_t)e->cus * both) {
+                tort_lds[c] = true;
+                per_cu[c] = (unsigned)both;
+            }
+        }
+        const size_t region = (size_t)2 * maxcells[c] * row;
+        if (region)
+            per_cu[c] = (unsigned)std::max<size_t>(
+                1, std::min<size_t>(per_cu[c], kRegionBudget / ((size_t)e->cus * region)));
+        scratch_off[c + 1] = scratch_off[c] + (size_t)e->cus * per_cu[c] * region / 4;
     }
     if (!d_scratch.alloc(scratch_off[3])) {
         say(err, errlen, "%s", "out of device memory for the machines' HBM regions");
         return 0;
     }
     hipStream_t s = e->stream;
+    // LDS beyond the default 64 KiB per workgroup for the LDS classes
+    BCHK(hipFuncSetAttribute((const void *)ts_batch<true, false>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
+    BCHK(hipFuncSetAttribute((const void *)ts_batch<true, true>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
     BCHK(hipMemcpyAsync(d_progs.p, progs.data(), n * sizeof(BatchProg), hipMemcpyHostToDevice, s));
     BCHK(hipMemcpyAsync(d_recs.p, recs.data(), recs.size() * 4, hipMemcpyHostToDevice, s));
     BCHK(hipMemcpyAsync(d_samples.p, samples.data(), samples.size() * 4, hipMemcpyHostToDevice, s));
@@ -422,15 +499,20 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
         const uint32_t *ip = d_items[cls].p;
         unsigned grid = std::min<unsigned>((unsigned)e->cus * per_cu[cls], count);
         uint32_t *sc = d_scratch.p + scratch_off[cls];
-        if (cls < 2) {
-            const size_t lds_bytes = (size_t)maxcells[cls] * kLanes * 4;
-            hipLaunchKernelGGL(ts_batch<true>, dim3(grid), dim3(kLanes), lds_bytes, st, d_progs.p,
-                               ip, first, count, d_recs.p, d_samples.p, queue, sc, maxcells[cls],
-                               d_stats.p, tune.max_passes, prog_words);
+        if (cls < 2 && tort_lds[cls]) {
+            const size_t lds_bytes = 2 * (size_t)maxcells[cls] * row;
+            hipLaunchKernelGGL((ts_batch<true, true>), dim3(grid), dim3(kLanes), lds_bytes, st,
+                               d_progs.p, ip, first, count, d_recs.p, d_samples.p, queue, sc,
+                               maxcells[cls], d_stats.p, tune.max_passes, prog_words, item_words);
+        } else if (cls < 2) {
+            const size_t lds_bytes = (size_t)maxcells[cls] * row;
+            hipLaunchKernelGGL((ts_batch<true, false>), dim3(grid), dim3(kLanes), lds_bytes, st,
+                               d_progs.p, ip, first, count, d_recs.p, d_samples.p, queue, sc,
+                               maxcells[cls], d_stats.p, tune.max_passes, prog_words, item_words);
         } else {
-            hipLaunchKernelGGL(ts_batch<false>, dim3(grid), dim3(kLanes), 0, st, d_progs.p, ip,
-                               first, count, d_recs.p, d_samples.p, queue, sc, maxcells[cls],
-                               d_stats.p, tune.max_passes, prog_words);
+            hipLaunchKernelGGL((ts_batch<false, false>), dim3(grid), dim3(kLanes), 0, st, d_progs.p,
+                               ip, first, count, d_recs.p, d_samples.p, queue, sc, maxcells[cls],
+                               d_stats.p, tune.max_passes, prog_words, item_words);
         }
         launches++;
         return hipGetLastError() == hipSuccess;

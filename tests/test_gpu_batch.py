@@ -58,6 +58,33 @@ def test_batch_matches_oracle(engine, plain):
     assert info["ops"] == sum(p.len * g["passes"] * 512 for (p, _), g in zip(progs, got))
 
 
+@pytest.mark.parametrize("groups", [1, 2, 3])
+@pytest.mark.parametrize("lds_cells", [0, 20])
+def test_batch_narrow_items(engine, groups, lds_cells):
+    # one or two groups run as items of 16 or 32 words (the wave's other lanes idle), three as a
+    # 64-word item with a throwaway group; lds_cells 20 sends every program of more than 18 cells
+    # to HBM, 0 lets few items keep even the largest in LDS
+    progs = programs()
+    print("batch groups=%d lds_cells=%d" % (groups, lds_cells), flush=True)
+    for plain in (False, True):
+        got, _ = engine.batch_run([p for p, _ in progs], [s for _, s in progs], groups=groups,
+                                  max_passes=96, plain=plain, lds_cells=lds_cells)
+        for (p, s), g in zip(progs, got):
+            assert g == expected(p, s, groups, 96), (p.ncells, p.len, plain)
+
+
+@pytest.mark.parametrize("waves_per_cu", [1, 16])
+def test_batch_tortoise_placement(engine, waves_per_cu):
+    # 128 groups = 32 items per program, 352 in all: at one wave per CU they outnumber the wave
+    # slots, so the LDS classes keep the tortoise in HBM; at 16 both fit in LDS
+    progs = programs()
+    print("batch groups=128 waves_per_cu=%d" % waves_per_cu, flush=True)
+    got, _ = engine.batch_run([p for p, _ in progs], [s for _, s in progs], groups=128,
+                              max_passes=24, waves_per_cu=waves_per_cu)
+    for (p, s), g in zip(progs, got):
+        assert g == expected(p, s, 128, 24), (p.ncells, p.len)
+
+
 def test_batch_long_cycles(engine):
     # more passes than the default: buckets beyond 64 and "none" both occur
     progs = [(t.random_program(4, 30, 5), 5), (t.random_program(12, 40, 6), 6)]

@@ -21,6 +21,8 @@ and adds beside its device figures a CPU baseline and the check of its results:
   serve  the seeded equiv request stream; the reference's enum_run on the first requests
   tester1  tester 1 (gpu_t1_run) at the reference's defaults; the reference's own t1_run on the
          same programs on host threads, its summary compared with the device's
+  polymul  the NTT polynomial product on many cubes; every cube's product checked against the
+         oracle's schoolbook product; the reference's jit on one cube, one host thread
 """
 import argparse
 import json
@@ -171,6 +173,25 @@ def run_tester1(args):
     print(json.dumps(out), flush=True)
 
 
+def cpu_polymul_baseline(keep, seconds):
+    """The reference's jit (oracle/_ref) running gen_polymul on cube 0, one host thread, passes
+    for about `seconds`: products per second (one product per pass)."""
+    from oracle import pyref
+    if not pyref.available():
+        return None
+    pm = keep["polymul"]
+    mem, _ = pm.lines(keep["a"][:1], keep["b"][:1])
+    theirs = pyref.make("gen_polymul")
+    cells = {c: mem[0, c] for c in pm.a + pm.b + list(pm.twiddles)}
+    _, t = pyref.jit_run_cells(theirs.ptr, cells, 100, [])
+    passes = max(100, int(seconds / (t / 100)))
+    got, t = pyref.jit_run_cells(theirs.ptr, cells, passes, pm.c)
+    return {"value": passes / t, "unit": "products/s", "cores": 1, "kind": "reference",
+            "ops_per_s": theirs.s.len * 512.0 * passes / t,
+            "sample": "the reference's jit_run of gen_polymul on one cube, %d passes, %.1f s"
+                      % (passes, t)}
+
+
 def run_workload(args):
     """One secondary workload, its checks and its CPU baseline, as one JSON line."""
     import tesseract_amd as t
@@ -192,6 +213,16 @@ def run_workload(args):
             out["cpu_baseline"] = cpu_batch_baseline(max(args.programs, 10000), args.seed,
                                                      args.passes, args.cpu_seconds,
                                                      args.cpu_threads)
+    elif args.workload == "polymul":
+        out = W.run_polymul(eng, args.cubes, args.seed, keep=keep)
+        if not args.no_cpu:
+            from oracle import pyoracle as O
+            exact = bool((keep["c"] == O.polymul(keep["a"], keep["b"])).all())
+            out["bit_exact_vs_oracle"] = exact
+            out["cpu_baseline"] = cpu_polymul_baseline(keep, args.cpu_seconds)
+            if not exact:
+                print(json.dumps(out), flush=True)
+                sys.exit("polymul: a product differs from the oracle's")
     else:
         out = W.run_serve(eng, args.requests, args.rate, args.seed, keep=keep)
         if not args.no_cpu:
@@ -226,8 +257,9 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="afterwards copy a MIB-MiB device buffer (copy_probe): a kernel of known "
                          "bytes whose FETCH_SIZE/WRITE_SIZE readings scale the kernel's")
-    ap.add_argument("--workload", choices=["mul16", "life", "batch", "serve", "tester1"],
+    ap.add_argument("--workload", choices=["mul16", "life", "batch", "serve", "tester1", "polymul"],
                     default="mul16")
+    ap.add_argument("--cubes", type=int, default=4096, help="polymul: independent products")
     ap.add_argument("--gens", type=int, default=1000, help="life: timed generations")
     ap.add_argument("--programs", type=int, default=None,
                     help="batch: programs of the mix (1000); tester1: programs per size (1024)")

@@ -78,6 +78,13 @@ void     ts_gen_adder(ts_program *p, int bits);
    smallest value. Output v0..v7. */
 void     ts_gen_perm8(ts_program *p);
 void     ts_gen_sort8(ts_program *p);
+/* c = a * b in Z_12289[x] / (x^512 + 1) by the number-theoretic transform on the 512-machine
+   cube (reference gen_polymul, src/gen.c:634-669; 220,758 instructions): machine k holds
+   coefficient k of a (a0..a13) and b (b0..b13) and the twiddles of ts_ntt_twiddles (inputs
+   zfD_0..13, ziD_0..13 for rounds D = 0..8); after one pass c0..c13 hold coefficient k of c. */
+void     ts_gen_polymul(ts_program *p);
+void     ts_ntt_twiddles(uint16_t zf[TS_CUBE_DIMS][TS_CUBE_SIZE],
+                         uint16_t zi[TS_CUBE_DIMS][TS_CUBE_SIZE]);
 
 /* ---- specifications (reference src/spec.h) --------------------------------------------------- */
 

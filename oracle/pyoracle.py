@@ -135,6 +135,28 @@ def benes_apply(ctrl, v):
     return x
 
 
+NTT_Q = 12289
+
+
+def polymul(a, b):
+    """c = a * b in Z_12289[x] / (x^512 + 1), coefficient k at index k of the last axis: the
+    schoolbook product folded with x^512 = -1, as the reference's polymul_scalar
+    (src/gen.c:671-684). a, b: (..., 512) integers in 0 .. 12288. Pinned in tests against that
+    function and against the ring's identities (1 is the unit, x shifts negacyclically)."""
+    a = np.asarray(a, np.int64)
+    b = np.asarray(b, np.int64)
+    lead = np.broadcast_shapes(a.shape[:-1], b.shape[:-1])
+    a = np.broadcast_to(a, lead + (512,)).reshape(-1, 512)
+    b = np.broadcast_to(b, lead + (512,)).reshape(-1, 512)
+    out = np.empty(a.shape, np.int64)
+    for i in range(a.shape[0]):
+        full = np.convolve(a[i], b[i])  # 1,023 terms, each below 512 q^2 < 2^63
+        low = full[:512].copy()
+        low[:511] -= full[512:]
+        out[i] = low % NTT_Q
+    return out.reshape(lead + (512,)).astype(np.uint16)
+
+
 def toyhash(x, bits, rounds):
     return int(lib().oracle_toyhash(x, bits, rounds))
 

@@ -88,6 +88,16 @@ def lib():
         L.t1_cycles.argtypes = [vp, C.c_uint64, C.c_uint64, vp, vp, vp]
         L.t1_cycles.restype = C.c_uint64
         L.cpu_has_avx2.restype = C.c_int
+        L.gen_polymul.argtypes = [vp]
+        L.ntt_twiddles.argtypes = [vp, vp]
+        L.polymul_scalar.argtypes = [vp, vp, vp]
+        L.lane_init.argtypes = [vp, vp]
+        L.lane_init.restype = C.c_int
+        L.lane_free.argtypes = [vp]
+        L.jit_compile.argtypes = [vp, C.c_int, vp, vp, C.c_char_p, C.c_size_t]
+        L.jit_compile.restype = C.c_int
+        L.jit_free.argtypes = [vp]
+        L.jit_run.argtypes = [vp, vp, C.c_uint64]
         L.t1_run.argtypes = [vp, vp, C.POINTER(C.c_int)]
         L.t1_run.restype = C.c_double
         L.t1_print_raw.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.c_uint64]
@@ -216,6 +226,46 @@ def t1_merge(paths, out_ptr):
     sec, th, mp = C.c_double(), C.c_int(), C.c_uint64()
     n = lib().t1_merge(files, len(paths), out_ptr, C.byref(sec), C.byref(th), C.byref(mp))
     return n, sec.value, th.value, mp.value
+
+
+def _aligned(nbytes, align=64):
+    raw = C.create_string_buffer(nbytes + align)
+    off = (-C.addressof(raw)) % align
+    return raw, C.addressof(raw) + off
+
+
+def jit_run_cells(prog_ptr, cells, passes, read):
+    """The reference's jit on one 512-machine group: a LaneMachine at power-on (lane_init), the
+    lines of `cells` (cell -> 8 u64) written into its memory, `passes` passes by jit_run, timed.
+    Returns ({cell: 8 u64 line} for the cells in `read`, seconds of jit_run). LaneMachine is
+    src/lanes.h:38-50 (64-byte aligned, its memory pointer at offset 8); JitCode src/jit.h:25-36."""
+    L = lib()
+    keep_m, m = _aligned(1024)
+    keep_j, j = _aligned(512)
+    if not L.lane_init(m, prog_ptr):
+        raise MemoryError
+    err = C.create_string_buffer(512)
+    try:
+        if not L.jit_compile(prog_ptr, 0, j, None, err, len(err)):
+            raise ValueError(err.value.decode())
+        mem = C.c_void_p.from_address(m + 8).value
+        for cell, line in cells.items():
+            line = np.ascontiguousarray(line, dtype=np.uint64)
+            C.memmove(mem + 64 * cell, line.ctypes.data, 64)
+        import time
+        t0 = time.perf_counter()
+        L.jit_run(j, m, passes)
+        secs = time.perf_counter() - t0
+        out = {}
+        for cell in read:
+            line = np.zeros(8, np.uint64)
+            C.memmove(line.ctypes.data, mem + 64 * cell, 64)
+            out[cell] = line
+        L.jit_free(j)
+        return out, secs
+    finally:
+        L.lane_free(m)
+        del keep_m, keep_j
 
 
 def cube_run(prog_ptr, ncells, mem, rr, passes, steps=0):

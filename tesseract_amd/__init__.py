@@ -141,6 +141,8 @@ def lib():
             "ts_gen_xorconv": (None, [P]),
             "ts_gen_perm8": (None, [P]),
             "ts_gen_sort8": (None, [P]),
+            "ts_gen_polymul": (None, [P]),
+            "ts_ntt_twiddles": (None, [vp, vp]),
             "ts_random_program": (None, [P, u32, sz, u64]),
             "ts_t1_program_seed": (u64, [u64, u32, u64]),
             "ts_t1_start": (None, [P, u64, C.c_uint, vp, vp]),
@@ -364,6 +366,18 @@ def gen_perm8():
 
 def gen_sort8():
     return _make(lib().ts_gen_sort8)[0]
+
+
+def gen_polymul():
+    return _make(lib().ts_gen_polymul)[0]
+
+
+def ntt_twiddles():
+    """(zf, zi): (9, 512) u16 arrays, the twiddle inputs of gen_polymul per round and machine."""
+    zf = np.zeros((CUBE_DIMS, CUBE_SIZE), np.uint16)
+    zi = np.zeros((CUBE_DIMS, CUBE_SIZE), np.uint16)
+    lib().ts_ntt_twiddles(zf.ctypes.data, zi.ctypes.data)
+    return zf, zi
 
 
 def random_program(cells, length, seed):

@@ -517,3 +517,236 @@ void ts_gen_sort8(ts_program *p)
         ts_b_output(b, v[i]);
     ts_b_finish(b, p);
 }
+
+/* ---- polynomial product mod (x^512 + 1, 12289) by the number-theoretic transform ------------
+   (reference src/gen.c:456-669). Machine k of a 512-machine cube holds coefficient k; numbers mod
+   q = 12289 are 14-bit vectors. The butterflies of round d pair machines k and k ^ 2^d through
+   links of dimension d; the twiddle of each machine and round is an input. The gate order below
+   is the reference's, so the program is the reference's instruction for instruction. */
+
+enum { NTT_BITS = 14, NTT_Q = 12289, NO_CARRY_OUT = -1 };
+
+/* out = x + y + cin over n bits, the carry into cout unless cout < 0 */
+static void adder_n(ts_builder *b, const uint32_t *out, const uint32_t *x, const uint32_t *y,
+                    uint32_t cin, int n, int64_t cout)
+{
+    uint32_t c = ts_b_cell(b, "add_c");
+    ts_b_gate(b, TS_G_BUF, c, &cin, 1);
+    for (int i = 0; i < n; i++)
+        ts_b_full_adder(b, out[i], c, x[i], y[i], c);
+    if (cout >= 0) {
+        uint32_t co = (uint32_t)cout;
+        ts_b_gate(b, TS_G_BUF, co, &c, 1);
+    }
+}
+
+/* out = sel ? x1 : x0, bit by bit through x0 ^ ((x0 ^ x1) & sel) */
+static void select_n(ts_builder *b, const uint32_t *out, uint32_t sel, const uint32_t *x1,
+                     const uint32_t *x0, int n)
+{
+    uint32_t t = ts_b_cell(b, "mux_t");
+    for (int i = 0; i < n; i++) {
+        ts_b_gate2(b, TS_G_XOR, t, x0[i], x1[i]);
+        ts_b_gate2(b, TS_G_AND, t, t, sel);
+        ts_b_gate2(b, TS_G_XOR, out[i], x0[i], t);
+    }
+}
+
+/* r = x y 2^-14 mod q (Montgomery reduction), every value in 0 .. q - 1. With t = x y,
+   m = t q^-1 mod 2^14 = t (1 + 2^12) mod 2^14 (q = 3 2^12 + 1), and t - m q is divisible by
+   2^14: r = t / 2^14 - (m q) / 2^14, where (m q) / 2^14 = (m + m / 2^12 + 2 m) / 4 rounded down
+   (the low 12 bits of m q are those of m), plus q when that difference is negative. */
+static void montgomery(ts_builder *b, const uint32_t *r, const uint32_t *x, const uint32_t *y)
+{
+    const uint32_t zero = TS_CELL_ZERO, one = TS_CELL_ONE;
+    uint32_t k = ts_b_cell(b, "mm_k");
+    uint32_t t[2 * NTT_BITS], m[NTT_BITS], s[16], h[15], u[16], v[16], th[15], ns[15], qm[NTT_BITS];
+    for (int i = 0; i < 2 * NTT_BITS; i++)
+        t[i] = named(b, "mm_t%d", i, 0);
+    for (int i = 0; i < 16; i++)
+        s[i] = named(b, "mm_s%d", i, 0);
+    for (int i = 0; i < 15; i++)
+        h[i] = named(b, "mm_h%d", i, 0);
+    multiply(b, t, x, y, NTT_BITS);
+    memcpy(m, t, 12 * sizeof *m);
+    m[12] = ts_b_cell(b, "mm_m12");
+    m[13] = ts_b_cell(b, "mm_m13");
+    ts_b_gate2(b, TS_G_AND, k, t[12], t[0]);  /* bits 12, 13 of t + (t << 12) */
+    ts_b_gate2(b, TS_G_XOR, m[12], t[12], t[0]);
+    ts_b_gate2(b, TS_G_XOR, m[13], t[13], t[1]);
+    ts_b_gate2(b, TS_G_XOR, m[13], m[13], k);
+    for (int i = 0; i < 16; i++) {  /* s = m + (m >> 12) */
+        u[i] = i < NTT_BITS ? m[i] : zero;
+        v[i] = i < 2 ? m[12 + i] : zero;
+    }
+    adder_n(b, s, u, v, zero, 16, NO_CARRY_OUT);
+    for (int i = 0; i < 16; i++)  /* s += 2 m */
+        v[i] = (i >= 1 && i <= NTT_BITS) ? m[i - 1] : zero;
+    adder_n(b, s, s, v, zero, 16, NO_CARRY_OUT);
+    for (int i = 0; i < 15; i++) {  /* h = (t >> 14) + ~(s >> 2) + 1, 15 bits */
+        th[i] = i < NTT_BITS ? t[NTT_BITS + i] : zero;
+        if (i < NTT_BITS) {
+            ns[i] = named(b, "mm_n%d", i, 0);
+            ts_b_gate(b, TS_G_NOT, ns[i], &s[2 + i], 1);
+        } else {
+            ns[i] = one;
+        }
+    }
+    adder_n(b, h, th, ns, one, 15, NO_CARRY_OUT);
+    for (int i = 0; i < NTT_BITS; i++)  /* r = h + (h < 0 ? q : 0) */
+        qm[i] = (NTT_Q >> i) & 1 ? h[14] : zero;
+    adder_n(b, r, h, qm, zero, NTT_BITS, NO_CARRY_OUT);
+}
+
+/* out = sel ? x - y : x + y mod q */
+static void add_or_sub_q(ts_builder *b, const uint32_t *out, uint32_t sel, const uint32_t *x,
+                         const uint32_t *y)
+{
+    const uint32_t zero = TS_CELL_ZERO, one = TS_CELL_ONE;
+    uint32_t carry = ts_b_cell(b, "as_cy");
+    uint32_t s[15], d[15], c[15], xx[15], w[15];
+    for (int i = 0; i < 15; i++) {
+        s[i] = named(b, "as_s%d", i, 0);
+        d[i] = named(b, "as_d%d", i, 0);
+        xx[i] = i < NTT_BITS ? x[i] : zero;
+        c[i] = (NTT_Q >> i) & 1 ? sel : zero;
+    }
+    adder_n(b, s, xx, c, zero, 15, NO_CARRY_OUT);  /* x, plus q when subtracting */
+    for (int i = 0; i < NTT_BITS; i++) {
+        w[i] = named(b, "as_v%d", i, 0);
+        ts_b_gate2(b, TS_G_XOR, w[i], y[i], sel);
+    }
+    w[NTT_BITS] = sel;
+    adder_n(b, s, s, w, sel, 15, NO_CARRY_OUT);  /* + y, or + ~y + 1: below 2 q */
+    for (int i = 0; i < 15; i++)
+        c[i] = ((32768 - NTT_Q) >> i) & 1 ? one : zero;
+    adder_n(b, d, s, c, zero, 15, carry);  /* s - q; the carry says s >= q */
+    select_n(b, out, carry, d, s, NTT_BITS);
+}
+
+/* a <- its negacyclic transform: rounds d = 8 .. 0 of Cooley-Tukey butterflies between machines
+   u and u + 2^d, (a_u, a_v) -> (a_u + z a_v, a_u - z a_v), z from input z[d] */
+static void ntt_forward(ts_builder *b, const uint32_t *a, uint32_t z[TS_CUBE_DIMS][NTT_BITS],
+                        const char *tag)
+{
+    uint32_t t[NTT_BITS], x[NTT_BITS], y[NTT_BITS], la[NTT_BITS], lt[NTT_BITS];
+    char name[64];
+    for (int i = 0; i < NTT_BITS; i++) {
+        t[i] = named(b, "nt_t%d", i, 0);
+        x[i] = named(b, "nt_x%d", i, 0);
+        y[i] = named(b, "nt_y%d", i, 0);
+    }
+    for (int d = TS_CUBE_DIMS - 1; d >= 0; d--) {
+        uint32_t id = named(b, "ID%d", d, 0);
+        montgomery(b, t, a, z[d]);  /* z a, used where bit d is set */
+        for (int i = 0; i < NTT_BITS; i++) {
+            snprintf(name, sizeof name, "%s_a%d_%d", tag, d, i);
+            la[i] = ts_b_link(b, name, a[i], d);
+            snprintf(name, sizeof name, "%s_t%d_%d", tag, d, i);
+            lt[i] = ts_b_link(b, name, t[i], d);
+        }
+        select_n(b, x, id, la, a, NTT_BITS);  /* a_u */
+        select_n(b, y, id, t, lt, NTT_BITS);  /* z a_v */
+        add_or_sub_q(b, a, id, x, y);
+    }
+}
+
+/* the way back without the factor 1/512: rounds d = 0 .. 8 of Gentleman-Sande butterflies,
+   (a_u, a_v) -> (a_u + a_v, (a_u - a_v) w), w from input w[d] (1 in machine u) */
+static void ntt_inverse(ts_builder *b, const uint32_t *a, uint32_t w[TS_CUBE_DIMS][NTT_BITS],
+                        const char *tag)
+{
+    uint32_t x[NTT_BITS], y[NTT_BITS], la[NTT_BITS];
+    char name[64];
+    for (int i = 0; i < NTT_BITS; i++) {
+        x[i] = named(b, "nt_x%d", i, 0);
+        y[i] = named(b, "nt_y%d", i, 0);
+    }
+    for (int d = 0; d < TS_CUBE_DIMS; d++) {
+        uint32_t id = named(b, "ID%d", d, 0);
+        for (int i = 0; i < NTT_BITS; i++) {
+            snprintf(name, sizeof name, "%s_a%d_%d", tag, d, i);
+            la[i] = ts_b_link(b, name, a[i], d);
+        }
+        select_n(b, x, id, la, a, NTT_BITS);  /* a_u */
+        select_n(b, y, id, a, la, NTT_BITS);  /* a_v */
+        add_or_sub_q(b, a, id, x, y);
+        montgomery(b, a, a, w[d]);
+    }
+}
+
+static uint32_t pow_mod_q(uint32_t x, uint32_t e)
+{
+    uint64_t r = 1, p = x % NTT_Q;
+    while (e) {
+        if (e & 1)
+            r = r * p % NTT_Q;
+        p = p * p % NTT_Q;
+        e >>= 1;
+    }
+    return (uint32_t)r;
+}
+
+static unsigned reverse9(unsigned k)
+{
+    unsigned r = 0;
+    for (int i = 0; i < TS_CUBE_DIMS; i++)
+        if (k >> i & 1)
+            r |= 1u << (TS_CUBE_DIMS - 1 - i);
+    return r;
+}
+
+/* The twiddle inputs of machine k in round d, in Montgomery form (times 2^14 mod q): forward
+   zf = psi^rev9(2^(8-d) + k / 2^(d+1)), psi a primitive 1024th root of 1; inverse zi = zf^-1 on
+   the machines with bit d set, 1 elsewhere (reference src/gen.c:617-632). */
+void ts_ntt_twiddles(uint16_t zf[TS_CUBE_DIMS][TS_CUBE_SIZE], uint16_t zi[TS_CUBE_DIMS][TS_CUBE_SIZE])
+{
+    const uint32_t mont = (1u << NTT_BITS) % NTT_Q;
+    uint32_t g = 2;  /* the least generator: q - 1 = 2^12 * 3 */
+    while (pow_mod_q(g, (NTT_Q - 1) / 2) == 1 || pow_mod_q(g, (NTT_Q - 1) / 3) == 1)
+        g++;
+    const uint32_t psi = pow_mod_q(g, (NTT_Q - 1) / 1024);
+    for (int d = 0; d < TS_CUBE_DIMS; d++)
+        for (unsigned k = 0; k < TS_CUBE_SIZE; k++) {
+            uint32_t z = pow_mod_q(psi, reverse9((256u >> d) + (k >> (d + 1))));
+            zf[d][k] = (uint16_t)((uint64_t)z * mont % NTT_Q);
+            zi[d][k] = (uint16_t)((k >> d) & 1 ? (uint64_t)pow_mod_q(z, NTT_Q - 2) * mont % NTT_Q
+                                               : mont);
+        }
+}
+
+void ts_gen_polymul(ts_program *p)
+{
+    ts_builder *b = ts_b_new();
+    uint32_t a[NTT_BITS], bv[NTT_BITS], c[NTT_BITS], wa[NTT_BITS], wb[NTT_BITS], f[NTT_BITS];
+    uint32_t zf[TS_CUBE_DIMS][NTT_BITS], zi[TS_CUBE_DIMS][NTT_BITS];
+    for (int i = 0; i < NTT_BITS; i++)
+        ts_b_input(b, a[i] = named(b, "a%d", i, 0));
+    for (int i = 0; i < NTT_BITS; i++)
+        ts_b_input(b, bv[i] = named(b, "b%d", i, 0));
+    for (int d = 0; d < TS_CUBE_DIMS; d++)
+        for (int i = 0; i < NTT_BITS; i++)
+            ts_b_input(b, zf[d][i] = named(b, "zf%d_%d", d, i));
+    for (int d = 0; d < TS_CUBE_DIMS; d++)
+        for (int i = 0; i < NTT_BITS; i++)
+            ts_b_input(b, zi[d][i] = named(b, "zi%d_%d", d, i));
+    /* 2^28 / 512 mod q: two Montgomery factors 2^-14 and the transform's 512 undone at the end */
+    const uint32_t fc = (uint32_t)((uint64_t)pow_mod_q((1u << NTT_BITS) % NTT_Q, 2) *
+                                   pow_mod_q(TS_CUBE_SIZE, NTT_Q - 2) % NTT_Q);
+    for (int i = 0; i < NTT_BITS; i++) {
+        c[i] = named(b, "c%d", i, 0);
+        wa[i] = named(b, "wa%d", i, 0);
+        ts_b_gate(b, TS_G_BUF, wa[i], &a[i], 1);
+        wb[i] = named(b, "wb%d", i, 0);
+        ts_b_gate(b, TS_G_BUF, wb[i], &bv[i], 1);
+        f[i] = (fc >> i) & 1 ? TS_CELL_ONE : TS_CELL_ZERO;
+    }
+    ntt_forward(b, wa, zf, "fa");
+    ntt_forward(b, wb, zf, "fb");
+    montgomery(b, wa, wa, wb);  /* pointwise product */
+    ntt_inverse(b, wa, zi, "ia");
+    montgomery(b, c, wa, f);
+    for (int i = 0; i < NTT_BITS; i++)
+        ts_b_output(b, c[i]);
+    ts_b_finish(b, p);
+}

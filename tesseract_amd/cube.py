@@ -1,10 +1,12 @@
-"""Data movement on the 512-machine cube: a Benes permutation (PERM8) and a bitonic sort (SORT8) of
-one byte per machine, the front end of the reference's cube demos (src/cubedemo.c:352-377 check
-them against benes_apply and qsort). Many independent cubes run in one state: cube i is group i.
+"""Programs on the 512-machine cube, the front end of the reference's cube demos
+(src/cubedemo.c:253-377): a Benes permutation (PERM8) and a bitonic sort (SORT8) of one byte per
+machine, checked against benes_apply and qsort there, and the polynomial product by the
+number-theoretic transform, checked against polymul_scalar. Many independent cubes run in one
+state: cube i is group i.
 """
 import numpy as np
 
-from . import BENES_ROUNDS, benes_route, gen_perm8, gen_sort8
+from . import BENES_ROUNDS, CUBE_DIMS, benes_route, gen_perm8, gen_polymul, gen_sort8, ntt_twiddles
 
 
 def _bits_to_lines(bits):
@@ -64,3 +66,57 @@ class Sort8(CubeByteProgram):
 
     def run(self, values):
         return self._run(np.atleast_2d(values))
+
+
+class PolyMul:
+    """c = a * b in Z_12289[x] / (x^512 + 1) (gen_polymul, 220,758 instructions): machine k of
+    cube i holds coefficient k of a[i] and b[i] and its twiddles; one pass leaves c[i] in it
+    (the reference's demo runs one pass, src/cubedemo.c:253-281). Further passes recompute the
+    same product: the inputs are only read."""
+
+    BITS = 14
+
+    def __init__(self, engine, tune=None):
+        self.prog = gen_polymul()
+        self._cells()
+        self.code = engine.compile(self.prog, tune)
+
+    def _cells(self):
+        p = self.prog
+        self.a = [p.cell("a%d" % i) for i in range(self.BITS)]
+        self.b = [p.cell("b%d" % i) for i in range(self.BITS)]
+        self.c = [p.cell("c%d" % i) for i in range(self.BITS)]
+        zf, zi = ntt_twiddles()
+        self.twiddles = {}  # cell -> its line, the same in every cube
+        for d in range(CUBE_DIMS):
+            for i in range(self.BITS):
+                self.twiddles[p.cell("zf%d_%d" % (d, i))] = _bits_to_lines(((zf[d] >> i) & 1)[None])[0]
+                self.twiddles[p.cell("zi%d_%d" % (d, i))] = _bits_to_lines(((zi[d] >> i) & 1)[None])[0]
+
+    def lines(self, a, b):
+        """The state of len(a) cubes: (cubes, ncells, 8) u64 lines and (cubes, 8) RR lines."""
+        a = np.atleast_2d(np.asarray(a, np.uint16))
+        b = np.atleast_2d(np.asarray(b, np.uint16))
+        cubes = a.shape[0]
+        mem = np.zeros((cubes, self.prog.ncells, 8), np.uint64)
+        for i in range(self.BITS):
+            mem[:, self.a[i], :] = _bits_to_lines((a >> i) & 1)
+            mem[:, self.b[i], :] = _bits_to_lines((b >> i) & 1)
+        for cell, line in self.twiddles.items():
+            mem[:, cell, :] = line
+        return mem, np.zeros((cubes, 8), np.uint64)
+
+    def read(self, mem):
+        out = np.zeros((mem.shape[0], 512), np.uint16)
+        for i in range(self.BITS):
+            out |= (_lines_to_bits(mem[:, self.c[i], :]).astype(np.uint16) << i)
+        return out
+
+    def run(self, a, b, passes=1):
+        """(c, device ms for `passes` passes of every cube)."""
+        mem, rr = self.lines(a, b)
+        st = self.code.state(mem.shape[0])
+        st.put(mem, rr)
+        ms = st.run(passes)
+        out, _ = st.get()
+        return self.read(out), ms

@@ -5,9 +5,10 @@
   python -m tesseract_amd.workloads hash   (toyhash32 preimage search over 2^32)
   python -m tesseract_amd.workloads benes  (Benes permutation and bitonic sort of 512 bytes)
   python -m tesseract_amd.workloads serve  [--requests N] [--rate R]
+  python -m tesseract_amd.workloads polymul [--cubes C] (NTT product mod x^512 + 1, 12289)
 
-`python bench.py --workload life|batch|serve` runs the same with their CPU baselines and the
-exactness checks against plain C beside them.
+`python bench.py --workload life|batch|serve|polymul` runs the same with their CPU baselines and
+the exactness checks against plain C beside them.
 
 batch: the program mix of the batching configuration. Program k draws two splitmix64 values from
 the stream seeded with S: cells = (16, 64, 256, 1024, 4096)[d0 % 5] and
@@ -117,6 +118,31 @@ def run_benes(engine, cubes, seed):
             "perm_bit_exact": ok_p, "sort_bit_exact": ok_s}
 
 
+def run_polymul(engine, cubes, seed, passes=20, keep=None):
+    """The NTT polynomial product (gen_polymul, 220,758 instructions) on `cubes` independent
+    512-machine cubes of seeded random polynomials: one pass, then `passes` passes timed (each
+    recomputes every cube's product). keep, a dict, receives a, b and c for a caller's checks."""
+    from .cube import PolyMul
+    rng = np.random.default_rng(seed)
+    a = rng.integers(0, 12289, (cubes, 512)).astype(np.uint16)
+    b = rng.integers(0, 12289, (cubes, 512)).astype(np.uint16)
+    t0 = time.perf_counter()
+    pm = PolyMul(engine)
+    setup = time.perf_counter() - t0
+    info = pm.code.stats()
+    c1, _ = pm.run(a, b, 1)
+    c, ms = pm.run(a, b, passes)
+    out = {"workload": "polymul", "cubes": cubes, "instructions": pm.prog.len,
+           "bitop3_per_word": info["bitop3"], "resident": bool(info["resident"]),
+           "compile_s": setup, "passes": passes, "ms": ms, "us_per_pass": ms * 1e3 / passes,
+           "products_per_s": cubes * passes / (ms * 1e-3),
+           "ops_per_s": pm.prog.len * 512.0 * cubes * passes / (ms * 1e-3),
+           "same_product_every_pass": bool((c == c1).all())}
+    if keep is not None:
+        keep.update(a=a, b=b, c=c, polymul=pm)
+    return out
+
+
 def run_hash(engine, target=0x165C2A07):
     prog = t.gen_hash(32, 8)
     code = t.EnumCode(engine, t.EN_SEARCH, prog, spec="h = %d" % target)
@@ -194,7 +220,7 @@ def run_serve(engine, n, rate, seed, streams=4, chunk_words=1 << 24, check=True,
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("workload", choices=["batch", "life", "hash", "benes", "serve"])
+    ap.add_argument("workload", choices=["batch", "life", "hash", "benes", "serve", "polymul"])
     ap.add_argument("--requests", type=int, default=200)
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--streams", type=int, default=4)
@@ -218,6 +244,8 @@ def main():
         out = run_serve(eng, a.requests, a.rate, a.seed, a.streams)
     elif a.workload == "benes":
         out = run_benes(eng, a.cubes, a.seed)
+    elif a.workload == "polymul":
+        out = run_polymul(eng, a.cubes, a.seed)
     else:
         out = run_hash(eng)
     out["device"] = eng.describe()

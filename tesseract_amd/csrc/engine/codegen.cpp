@@ -9,6 +9,7 @@
 #include <array>
 #include <functional>
 #include <map>
+#include <unordered_map>
 #include <cstdio>
 #include <cstring>
 #include <sstream>
@@ -340,17 +341,17 @@ std::string hex32(uint32_t v)
 // ds_swizzle's xor mode inside 32 lanes, ds_bpermute across the halves.
 std::string xl_expr(const std::string &v, uint32_t x)
 {
-    char b[96];
     static const unsigned quad[4] = {0, 0xB1, 0x4E, 0x1B};
-    if (x < 4)
-        snprintf(b, sizeof b, "(u32)__builtin_amdgcn_mov_dpp((int)(%s), 0x%X, 0xF, 0xF, false)",
-                 v.c_str(), quad[x]);
-    else if (x < 32)
-        snprintf(b, sizeof b, "(u32)__builtin_amdgcn_ds_swizzle((int)(%s), 0x%X)", v.c_str(),
-                 (x << 10) | 0x1Fu);
-    else
-        snprintf(b, sizeof b, "ts_xl_far(%s, %uu)", v.c_str(), x);
-    return b;
+    char k[32];
+    if (x < 4) {
+        snprintf(k, sizeof k, "0x%X", quad[x]);
+        return "(u32)__builtin_amdgcn_mov_dpp((int)(" + v + "), " + k + ", 0xF, 0xF, false)";
+    }
+    if (x < 32) {
+        snprintf(k, sizeof k, "0x%X", (x << 10) | 0x1Fu);
+        return "(u32)__builtin_amdgcn_ds_swizzle((int)(" + v + "), " + k + ")";
+    }
+    return "ts_xl_far(" + v + ", " + std::to_string(x) + "u)";
 }
 
 // Common device helpers, emitted once per module.
@@ -380,6 +381,8 @@ struct Emitter {
     std::ostringstream &o;
     std::string prefix;  // variable name prefix
     size_t luts = 0, values = 0;
+    // lines written right after a node's definition (stores of outputs as soon as they exist)
+    const std::unordered_map<uint32_t, std::vector<std::string>> *after = nullptr;
 
     Emitter(const Aig &g_, const Mapping &m_, std::ostringstream &o_, std::string p)
         : g(g_), m(m_), o(o_), prefix(std::move(p)) {}
@@ -440,6 +443,12 @@ struct Emitter {
                 o << "  u32 " << var((uint32_t)n) << " = " << leaf((uint32_t)n, nd) << ";\n";
             }
             values++;
+            if (after) {
+                auto it = after->find((uint32_t)n);
+                if (it != after->end())
+                    for (const std::string &line : it->second)
+                        o << line;
+            }
         }
     }
 };
@@ -454,6 +463,15 @@ void codegen_run(const Lowered &lw, const CodegenOptions &opt, KernelSource &ks)
     o << kPrelude;
     const unsigned cube_wl = lw.dims - 5;  // words per cube = 2^cube_wl
     const char *ptr_of[3] = {"cur", "prev", "init"};
+    // the launch bounds, and the waves per SIMD the register allocation must leave room for. A
+    // resident program's pass is one long block of straight-line code: left alone the compiler
+    // spends up to all 512 registers on it (one wave per SIMD); held to 256 it spills a little and
+    // runs two waves per SIMD, 1.5x the throughput on the NTT polynomial product.
+    const unsigned waves = opt.waves ? opt.waves : lw.resident ? 2u : 0u;
+    const std::string bounds =
+        "__launch_bounds__(" + std::to_string(opt.block) + ")" +
+        (waves > 1 ? " __attribute__((amdgpu_waves_per_eu(" + std::to_string(waves) + ")))"
+                   : std::string());
 
     auto id_expr = [&](uint32_t d) -> std::string {
         static const uint32_t pat[5] = {0xAAAAAAAAu, 0xCCCCCCCCu, 0xF0F0F0F0u, 0xFF00FF00u,
@@ -470,33 +488,41 @@ void codegen_run(const Lowered &lw, const CodegenOptions &opt, KernelSource &ks)
             roots.push_back(out.v);
         Mapping m;
         map_luts(s.g, roots, opt.lut3, m);
-        // state variables: every version -1 that is read, every written cell
+        // state variables carried from pass to pass: the cells whose value at the start of a pass
+        // is read (version -1). A cell the pass writes before any read of it is no loop-carried
+        // variable: its value of the last pass is stored as the last pass makes it, so no register
+        // holds it across the loop.
         std::vector<uint8_t> is_state(lw.ncells + 1, 0);
         for (const Version &v : lw.versions)
             is_state[v.cell] = 1;
-        for (const SegOut &out : s.outs)
-            is_state[out.cell] = 1;
-        o << "extern \"C\" __global__ void __launch_bounds__(" << opt.block
-          << ") ts_pass(const u32* __restrict__ prev, u32* __restrict__ cur, const u32* "
+        o << "extern \"C\" __global__ void " << bounds
+          << " ts_pass(const u32* __restrict__ prev, u32* __restrict__ cur, const u32* "
              "__restrict__ init, u32 nw, u32 passes) {\n"
              "  const u32 w = blockIdx.x * "
           << opt.block << "u + threadIdx.x;\n  if (w >= nw) return;\n"
           << "  const u32 cw = w & " << ((1u << cube_wl) - 1) << "u;\n  (void)cw;\n";
+        o << "  if (passes == 0u) {  // the state as it was\n";
+        for (const SegOut &out : s.outs)
+            o << "    cur[(size_t)" << lw.final_slot[out.cell] << "u * nw + w] = prev[(size_t)"
+              << lw.final_slot[out.cell] << "u * nw + w];\n";
+        o << "    return;\n  }\n";
+        // carried cells are loaded before the loop; a cell no pass writes is loaded where the
+        // pass reads it (the loads are invariant: the compiler keeps or repeats them as registers
+        // allow, instead of holding every such cell from the start)
         for (uint32_t c = 0; c <= lw.ncells; c++) {
             if (!is_state[c])
                 continue;
             int32_t fs = lw.final_slot[c];
             if (fs >= 0)
                 o << "  u32 s" << c << " = prev[(size_t)" << fs << "u * nw + w];\n";
-            else
-                o << "  u32 s" << c << " = init[(size_t)" << lw.init_slot[c] << "u * nw + w];\n";
         }
-        o << "  for (u32 pass = 0; pass < passes; pass++) {\n";
-        Emitter e(s.g, m, o, "v");
-        e.emit([&](uint32_t, const Node &nd) -> std::string {
+        auto leaf = [&](uint32_t, const Node &nd) -> std::string {
             if (nd.kind == N_LOAD) {
                 const Version &ver = lw.versions[nd.a];
-                std::string x = "s" + std::to_string(ver.cell);
+                std::string x = lw.final_slot[ver.cell] >= 0
+                                    ? "s" + std::to_string(ver.cell)
+                                    : "init[(size_t)" + std::to_string(lw.init_slot[ver.cell]) +
+                                          "u * nw + w]";
                 uint32_t inword = nd.b & 31u, lanes = nd.b >> 5;
                 if (inword)
                     x = "ts_pw(" + x + ", " + std::to_string(inword) + "u)";
@@ -507,17 +533,40 @@ void codegen_run(const Lowered &lw, const CodegenOptions &opt, KernelSource &ks)
             if (nd.kind == N_IDBIT)
                 return id_expr(nd.a);
             return "0u";
-        });
-        // all new values first, then the state update (every read above saw the old state)
-        for (size_t i = 0; i < s.outs.size(); i++)
-            o << "    u32 n" << i << " = " << e.val(s.outs[i].v) << ";\n";
-        for (size_t i = 0; i < s.outs.size(); i++)
-            o << "    s" << s.outs[i].cell << " = n" << i << ";\n";
-        o << "  }\n";
-        for (const SegOut &out : s.outs)
-            o << "  cur[(size_t)" << lw.final_slot[out.cell] << "u * nw + w] = s" << out.cell
-              << ";\n";
-        o << "}\n";
+        };
+        // Every pass but the last updates only the carried cells: what else the pass computes is
+        // overwritten by the next pass before anything reads it, and the compiler drops it. The
+        // last pass runs apart, storing each cell's value the moment it exists, so that no value
+        // waits in a register for the end of the pass.
+        o << "  for (u32 pass = 1; pass < passes; pass++) {\n";
+        {
+            Emitter e(s.g, m, o, "v");
+            e.emit(leaf);
+            // all new values first, then the state update (every read above saw the old state)
+            for (size_t i = 0; i < s.outs.size(); i++)
+                if (is_state[s.outs[i].cell])
+                    o << "    u32 n" << i << " = " << e.val(s.outs[i].v) << ";\n";
+            for (size_t i = 0; i < s.outs.size(); i++)
+                if (is_state[s.outs[i].cell])
+                    o << "    s" << s.outs[i].cell << " = n" << i << ";\n";
+        }
+        o << "  }\n  {  // the last pass\n";
+        Emitter e(s.g, m, o, "v");
+        std::unordered_map<uint32_t, std::vector<std::string>> after;
+        std::vector<std::string> constant_outs;
+        for (const SegOut &out : s.outs) {
+            std::string line = "    cur[(size_t)" + std::to_string(lw.final_slot[out.cell]) +
+                               "u * nw + w] = " + e.val(out.v) + ";\n";
+            if (s.g.nodes[out.v >> 1].kind == N_CONST)
+                constant_outs.push_back(line);
+            else
+                after[out.v >> 1].push_back(line);
+        }
+        e.after = &after;
+        e.emit(leaf);
+        for (const std::string &line : constant_outs)
+            o << line;
+        o << "  }\n}\n";
         ks.names.push_back("ts_pass");
         ks.bitop3 += e.luts;
         ks.values += e.values;
@@ -530,7 +579,7 @@ void codegen_run(const Lowered &lw, const CodegenOptions &opt, KernelSource &ks)
         const Segment &s = lw.segs[si];
         std::string name = "ts_seg" + std::to_string(si);
         ks.names.push_back(name);
-        o << "extern \"C\" __global__ void __launch_bounds__(" << opt.block << ") " << name
+        o << "extern \"C\" __global__ void " << bounds << " " << name
           << "(const u32* __restrict__ prev, u32* __restrict__ cur, const u32* __restrict__ "
              "init, u32 nw) {\n"
              "  const u32 t = blockIdx.x * "

@@ -258,6 +258,7 @@ int generate(const ts_program *p, const gpu_tune &tune, Lowered &lw, KernelSourc
     CodegenOptions co;
     co.block = tune.block;
     co.lut3 = tune.lut3 == 1;
+    co.waves = std::min(tune.waves, 8u);
     codegen_run(lw, co, ks);
     return 1;
 }

@@ -124,6 +124,7 @@ struct KernelSource {
 struct CodegenOptions {
     unsigned block = 256;
     bool     lut3 = true;  // map 3-input cuts to v_bitop3_b32
+    unsigned waves = 0;    // run kernels: minimum waves per SIMD (0 = the compiler's choice)
 };
 
 // Streamed form: one kernel per segment, state in HBM slots.

@@ -71,8 +71,8 @@ class Sort8(CubeByteProgram):
 class PolyMul:
     """c = a * b in Z_12289[x] / (x^512 + 1) (gen_polymul, 220,758 instructions): machine k of
     cube i holds coefficient k of a[i] and b[i] and its twiddles; one pass leaves c[i] in it
-    (the reference's demo runs one pass, src/cubedemo.c:253-281). Further passes recompute the
-    same product: the inputs are only read."""
+    (the reference's demo runs one pass, src/cubedemo.c:253-281). Further passes give the same
+    product: the inputs are only read."""
 
     BITS = 14
 
@@ -118,5 +118,16 @@ class PolyMul:
         st = self.code.state(mem.shape[0])
         st.put(mem, rr)
         ms = st.run(passes)
+        out, _ = st.get()
+        return self.read(out), ms
+
+    def run_each(self, a, b, runs):
+        """`runs` calls of one pass each on one state (every call computes every product in
+        full; within one call of several passes the engine computes the product only in the
+        last pass, since nothing reads it before). (c, device ms of each call)."""
+        mem, rr = self.lines(a, b)
+        st = self.code.state(mem.shape[0])
+        st.put(mem, rr)
+        ms = [st.run(1) for _ in range(runs)]
         out, _ = st.get()
         return self.read(out), ms

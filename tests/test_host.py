@@ -227,6 +227,23 @@ def test_generated_kernels_compile_for_gfx950():
     progs = [t.gen_adder(8), t.gen_rank(12), t.gen_perm8(), t.random_program(30, 300, 1)]
     life, _ = t.gen_life_cube(4, 4, 6, 6)
     progs.append(life)
+    # cube programs whose links read cells no pass writes (loads inside cross-lane moves) and
+    # cells written before any read (stored as the last pass makes them)
+    rng = np.random.default_rng(11)
+    for dims in (9, 12):
+        b = t.Builder()
+        plain = [b.cell("c%d" % i) for i in range(10)]
+        ids = [b.cell("ID%d" % d) for d in range(dims)]
+        links = [b.link("l%d" % k, (plain + ids)[int(rng.integers(0, 10 + dims))], k % dims)
+                 for k in range(3 * dims)]
+        for k in range(3 * dims):  # links of links: a move inside the word and across lanes
+            links.append(b.link("m%d" % k, links[k], (k + 5) % dims))
+        for _ in range(200):
+            if rng.random() < 0.55:
+                b.nand(([0, 1] + plain + ids + links)[int(rng.integers(0, 12 + 7 * dims))])
+            else:
+                b.sto(plain[int(rng.integers(0, 6))])
+        progs.append(b.finish())
     for p in progs:
         src, info = t.source(p)
         t.rtc_check(src)

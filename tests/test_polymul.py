@@ -145,3 +145,13 @@ def test_polymul_gpu_repeated_passes(polymul):
         b = rng.integers(0, Q, (cubes, 512)).astype(np.uint16)
         c, _ = polymul.run(a, b, passes=3)
         assert (c == O.polymul(a, b)).all()
+
+
+@pytest.mark.gpu
+def test_polymul_gpu_calls_on_one_state(polymul):
+    # one-pass calls one after another on the same state: each computes the product in full
+    rng = np.random.default_rng(11)
+    a = rng.integers(0, Q, (5, 512)).astype(np.uint16)
+    b = rng.integers(0, Q, (5, 512)).astype(np.uint16)
+    c, ms = polymul.run_each(a, b, 4)
+    assert (c == O.polymul(a, b)).all() and len(ms) == 4

@@ -506,23 +506,22 @@ void codegen_run(const Lowered &lw, const CodegenOptions &opt, KernelSource &ks)
             o << "    cur[(size_t)" << lw.final_slot[out.cell] << "u * nw + w] = prev[(size_t)"
               << lw.final_slot[out.cell] << "u * nw + w];\n";
         o << "    return;\n  }\n";
-        // carried cells are loaded before the loop; a cell no pass writes is loaded where the
-        // pass reads it (the loads are invariant: the compiler keeps or repeats them as registers
-        // allow, instead of holding every such cell from the start)
+        // every cell read at the start of a pass is loaded before the loop, so that all the
+        // loads are in flight at once (issued where they are read, each round of a short program
+        // waited for its own)
         for (uint32_t c = 0; c <= lw.ncells; c++) {
             if (!is_state[c])
                 continue;
             int32_t fs = lw.final_slot[c];
             if (fs >= 0)
                 o << "  u32 s" << c << " = prev[(size_t)" << fs << "u * nw + w];\n";
+            else
+                o << "  u32 s" << c << " = init[(size_t)" << lw.init_slot[c] << "u * nw + w];\n";
         }
         auto leaf = [&](uint32_t, const Node &nd) -> std::string {
             if (nd.kind == N_LOAD) {
                 const Version &ver = lw.versions[nd.a];
-                std::string x = lw.final_slot[ver.cell] >= 0
-                                    ? "s" + std::to_string(ver.cell)
-                                    : "init[(size_t)" + std::to_string(lw.init_slot[ver.cell]) +
-                                          "u * nw + w]";
+                std::string x = "s" + std::to_string(ver.cell);
                 uint32_t inword = nd.b & 31u, lanes = nd.b >> 5;
                 if (inword)
                     x = "ts_pw(" + x + ", " + std::to_string(inword) + "u)";

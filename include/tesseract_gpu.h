@@ -33,7 +33,7 @@ typedef struct {
     unsigned block;        /* threads per workgroup (multiple of 64) */
     unsigned slice_nodes;  /* AIG nodes per slice for segmented programs (0 = by cube size) */
     unsigned lut3;         /* 1 = map to v_bitop3_b32 (default), 2 = plain AND/NOT */
-    unsigned grid;         /* workgroups for enumeration kernels (0 = default) */
+    unsigned grid;         /* workgroups for enumeration kernels (0 = default: 256 per CU) */
     unsigned graph_passes; /* segmented programs: passes per captured HIP graph (default 32,
                               1 = launch every segment directly) */
     unsigned tape;         /* enumerations: 1 = run the precompiled tape interpreter instead of

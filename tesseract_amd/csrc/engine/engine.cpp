@@ -82,8 +82,8 @@ gpu_tune defaults(const gpu_engine *e, const gpu_tune *t)
         d.block = 256;
     if (!d.lut3)
         d.lut3 = 1;
-    if (!d.grid)
-        d.grid = (unsigned)std::max(1, e->cus) * 8;
+    if (!d.grid)  // a few words per thread: 8 per CU measured 5.7% slower on mul16 (tail waves)
+        d.grid = (unsigned)std::max(1, e->cus) * 256;
     if (!d.graph_passes)
         d.graph_passes = 32;
     return d;

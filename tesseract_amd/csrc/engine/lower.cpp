@@ -2,6 +2,7 @@
 #include "lower.hpp"
 
 #include <algorithm>
+#include <unordered_map>
 #include <cstdlib>
 #include <cstdio>
 
@@ -175,16 +176,82 @@ struct PassBuilder {
             else {
                 Lit base = value(c);
                 if ((mask >> lw.wave_dims) && g().derived(base)) {
-                    cut();
-                    base = value(c);
+                    // a value computed in this segment, read from a machine in another wave:
+                    // computed again here from that machine's start-of-segment values if its cone
+                    // is small, else the segment ends (the value crosses as a slot)
+                    if (!remat(base, mask, v)) {
+                        cut();
+                        v = g().Perm(value(c), mask);
+                    }
+                } else {
+                    v = g().Perm(base, mask);
                 }
-                v = g().Perm(base, mask);
             }
             Lit r = value(RR);
             write(RR, lit_not(g().And(r, v)));
         }
         close_segment();
         return true;
+    }
+
+    // Cones of at most this many computed nodes are recomputed rather than cut for.
+    static constexpr size_t kRematNodes = 128;
+
+    // x as machine k ^ mask computes it, rebuilt from loads and ids of that machine: for a cone
+    // of at most kRematNodes AND/PERM nodes. Reads across dims the wave sees stay PERM nodes.
+    std::vector<uint32_t> seen_at;  // node -> the remat call that last visited it
+    uint32_t remat_calls = 0;
+
+    bool remat(Lit x, uint32_t mask, Lit &out)
+    {
+        if (seen_at.size() < g().nodes.size())
+            seen_at.resize(g().nodes.size() * 2, 0);
+        const uint32_t stamp = ++remat_calls;
+        std::vector<uint32_t> stack{x >> 1};
+        size_t count = 0;
+        while (!stack.empty()) {
+            uint32_t n = stack.back();
+            stack.pop_back();
+            if (seen_at[n] == stamp)
+                continue;
+            seen_at[n] = stamp;
+            const Node nd = g().nodes[n];
+            if (nd.kind == N_AND) {
+                stack.push_back(nd.a >> 1);
+                stack.push_back(nd.b >> 1);
+            } else if (nd.kind == N_PERM) {
+                stack.push_back(nd.a >> 1);
+            } else {
+                continue;
+            }
+            if (++count > kRematNodes)
+                return false;
+        }
+        std::unordered_map<uint64_t, Lit> memo;
+        out = moved(x, mask, memo);
+        return true;
+    }
+
+    Lit moved(Lit x, uint32_t mask, std::unordered_map<uint64_t, Lit> &memo)
+    {
+        if (!(mask >> lw.wave_dims))
+            return g().Perm(x, mask);  // inside the wave: a cross-lane move of the value
+        const uint32_t n = x >> 1;
+        const Lit c = x & 1u;
+        const uint64_t key = (uint64_t)n << 32 | mask;
+        auto it = memo.find(key);
+        if (it != memo.end())
+            return it->second ^ c;
+        const Node nd = g().nodes[n];
+        Lit r;
+        if (nd.kind == N_AND)
+            r = g().And(moved(nd.a, mask, memo), moved(nd.b, mask, memo));
+        else if (nd.kind == N_PERM)
+            r = moved(nd.a, nd.b ^ mask, memo);
+        else
+            r = g().Perm(n << 1, mask);  // loads, ids, constants move directly
+        memo.emplace(key, r);
+        return r ^ c;
     }
 
     bool ts_readonly(uint32_t c) const

@@ -184,9 +184,10 @@ int  gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t n, const g
    max_passes passes (< 2^32) with the reference's Brent schedule; out[si] (nsizes entries) gets
    the per-size summary the reference's workers add up (src/tester.c:182-226). Sizes take turns in
    the reference's order, and the longest cycle's seed is the first program, in that order, that
-   reaches it (the reference's single-thread result). programs > 0: that many per size in one
-   batch (gpu_batch_run with one group per program); programs = 0: rounds of 64 per size until
-   `seconds` have passed, whole programs only, so unfinished stays 0. *seconds gets the wall time.
+   reaches it (the reference's single-thread result). programs > 0: that many per size, in
+   batches (gpu_batch_run with one group per program) of at most 4,096 per size; programs = 0:
+   rounds of 64 per size until `seconds` have passed, whole programs only, so unfinished stays 0.
+   *seconds gets the wall time.
    threads, idle and excluded are ignored. */
 int  gpu_t1_run(gpu_engine *e, const ts_t1_config *cfg, ts_t1_size *out, double *seconds, char *err,
                 size_t errlen);

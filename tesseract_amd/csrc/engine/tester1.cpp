@@ -1,6 +1,7 @@
 // Tester 1 on the device (reference src/tester.c:166-279): the random programs of each machine
 // size, run as one batch of the program-as-data interpreter (one 512-machine group per program),
 // summed per size in the reference's program order.
+#include <algorithm>
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
@@ -33,7 +34,8 @@ struct Round {
     }
 };
 
-constexpr uint64_t kRoundPerSize = 64;  // programs per size and round when running for a time
+constexpr uint64_t kRoundPerSize = 64;     // programs per size and round when running for a time
+constexpr uint64_t kMostPerSize = 4096;    // programs per size in one batch otherwise
 
 }  // namespace
 
@@ -70,11 +72,14 @@ extern "C" int gpu_t1_run(gpu_engine *e, const ts_t1_config *cfg, ts_t1_size *ou
     auto elapsed = [&] {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     };
-    const uint64_t per_size = cfg->programs ? cfg->programs : kRoundPerSize;
     gpu_batch_tune tune{};
     tune.groups = 1;
     tune.max_passes = (unsigned)cfg->max_passes;
-    for (uint64_t k0 = 0;; k0 += per_size) {
+    // rounds of programs k0 .. k0 + per_size - 1 of every size: the reference's order, a round at
+    // a time (the summaries add up in that order whatever the rounds)
+    for (uint64_t k0 = 0;;) {
+        const uint64_t per_size = cfg->programs ? std::min(kMostPerSize, cfg->programs - k0)
+                                                : kRoundPerSize;
         Round r;
         const size_t n = (size_t)(per_size * (uint64_t)cfg->nsizes);
         r.progs.resize(n);
@@ -105,7 +110,8 @@ extern "C" int gpu_t1_run(gpu_engine *e, const ts_t1_config *cfg, ts_t1_size *ou
             x.longest_seed = r.seeds[j];
             ts_t1_add_size(&out[r.size_of[j]], &x);
         }
-        if (cfg->programs || elapsed() >= cfg->seconds)
+        k0 += per_size;
+        if (cfg->programs ? k0 >= cfg->programs : elapsed() >= cfg->seconds)
             break;
     }
     if (seconds)

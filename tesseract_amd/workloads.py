@@ -6,6 +6,7 @@
   python -m tesseract_amd.workloads benes  (Benes permutation and bitonic sort of 512 bytes)
   python -m tesseract_amd.workloads serve  [--requests N] [--rate R]
   python -m tesseract_amd.workloads polymul [--cubes C] (NTT product mod x^512 + 1, 12289)
+  python -m tesseract_amd.workloads tape   (one equiv on the tape interpreter and compiled)
 
 `python bench.py --workload life|batch|serve|polymul` runs the same with their CPU baselines and
 the exactness checks against plain C beside them.
@@ -145,6 +146,31 @@ def run_polymul(engine, cubes, seed, runs=20, keep=None):
     return out
 
 
+def run_tape(engine, inputs=26, gates=2000, seed=3, runs=5):
+    """The tape tier on its own: an equiv of a seeded random netlist of `gates` gates and `inputs`
+    inputs against its NAND-level rewrite (netlists.circuit / rewrite), enumerated on the
+    precompiled tape interpreter (no hipRTC) and, for comparison, as a compiled kernel."""
+    from . import netlists as N
+    rng = np.random.default_rng(seed)
+    c = N.circuit(inputs, 8, gates, rng)
+    r = N.rewrite(c, rng)
+    a, b = t.net_parse(c.bench()), t.net_parse(r.bench())
+    tape = t.enum_tape(t.EN_EQUIV, a, b)
+    out = {"workload": "tape", "inputs": inputs, "gates": gates, "and_nodes": tape["and_nodes"],
+           "groups": int(tape["ops"].shape[0]), "slots": tape["slots"]}
+    for tier, tune in (("tape", {"tape": 1}), ("compiled", None)):
+        code = t.EnumCode(engine, t.EN_EQUIV, a, b, tune=tune)
+        code.run()
+        ms = []
+        for _ in range(runs):
+            res = code.run()
+            ms.append(engine.last_ms)
+        out[tier + "_ms"] = min(ms)
+        out[tier + "_and_per_s"] = tape["and_nodes"] * float(res["tried"]) / (min(ms) * 1e-3)
+        out[tier + "_hits"] = res["hits"]
+    return out
+
+
 def run_hash(engine, target=0x165C2A07):
     prog = t.gen_hash(32, 8)
     code = t.EnumCode(engine, t.EN_SEARCH, prog, spec="h = %d" % target)
@@ -222,7 +248,8 @@ def run_serve(engine, n, rate, seed, streams=4, chunk_words=1 << 24, check=True,
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("workload", choices=["batch", "life", "hash", "benes", "serve", "polymul"])
+    ap.add_argument("workload", choices=["batch", "life", "hash", "benes", "serve", "polymul",
+                                         "tape"])
     ap.add_argument("--requests", type=int, default=200)
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--streams", type=int, default=4)
@@ -248,6 +275,8 @@ def main():
         out = run_benes(eng, a.cubes, a.seed)
     elif a.workload == "polymul":
         out = run_polymul(eng, a.cubes, a.seed)
+    elif a.workload == "tape":
+        out = run_tape(eng)
     else:
         out = run_hash(eng)
     out["device"] = eng.describe()

@@ -38,10 +38,10 @@ typedef struct {
                               1 = launch every segment directly) */
     unsigned tape;         /* enumerations: 1 = run the precompiled tape interpreter instead of
                               compiling a kernel (equiv and search with one tick; no hipRTC) */
-    unsigned waves;        /* run kernels: waves per SIMD the register allocation must allow
-                              (0 = default: 2 for a resident program, which caps it at 256
-                              VGPRs, the compiler's choice for segments; 1 = the compiler's
-                              choice everywhere) */
+    unsigned waves;        /* waves per SIMD the register allocation must allow (0 = default:
+                              2 for a resident program, which caps it at 256 VGPRs, the
+                              compiler's choice for segments and enumerations; 1 = the
+                              compiler's choice everywhere) */
 } gpu_tune;
 
 typedef struct {

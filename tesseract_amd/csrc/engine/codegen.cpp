@@ -857,8 +857,10 @@ bool codegen_enum(const ts_program *a, const ts_program *b, const int *in_map, c
     o << "TS_DEV u32 ts_shl32(u32 a, u32 s) { return s < 32u ? a << s : 0u; }\n"
          "TS_DEV u64 ts_shl64(u64 a, u64 s) { return s < 64ull ? a << s : 0ull; }\n"
          "TS_DEV u64 ts_shr64(u64 a, u64 s) { return s < 64ull ? a >> s : 0ull; }\n";
-    o << "extern \"C\" __global__ void __launch_bounds__(" << opt.block
-      << ") ts_enum(u64* __restrict__ out, u64* __restrict__ list, u64 cap, u64 i0, u64 i1, "
+    o << "extern \"C\" __global__ void __launch_bounds__(" << opt.block << ")"
+      << (opt.waves > 1 ? " __attribute__((amdgpu_waves_per_eu(" + std::to_string(opt.waves) + ")))"
+                        : std::string())
+      << " ts_enum(u64* __restrict__ out, u64* __restrict__ list, u64 cap, u64 i0, u64 i1, "
          "u64 space, u32 parts, u32 part, u32 nshare) {\n"
          "  for (u64 i = i0 + (u64)blockIdx.x * "
       << opt.block << "u + threadIdx.x; i < i1; i += (u64)gridDim.x * " << opt.block

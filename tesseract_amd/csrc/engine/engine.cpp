@@ -849,6 +849,7 @@ int ts_internal::enum_prepare(const gpu_engine *e, const gpu_enum *en, gpu_tune 
     CodegenOptions co;
     co.block = tune.block;
     co.lut3 = tune.lut3 == 1;
+    co.waves = std::min(tune.waves, 8u);
     std::string msg;
     if (!codegen_enum(en->a, en->b, plan.in_map.data(), plan.out_map.data(), plan.el, co, ks, msg,
                       plan.spec)) {

@@ -200,7 +200,8 @@ int  gpu_t1_run(gpu_engine *e, const ts_t1_config *cfg, ts_t1_size *out, double 
    within a chunk and their work overlaps on the device. Each result is exactly what gpu_enum_run
    returns for the request alone. Compiled modules stay resident for repeated circuits. Small
    equiv/search requests skip the compile: their tape (gpu_enum_tape) runs on the precompiled
-   interpreter kernel. */
+   interpreter kernel; larger ones, up to about 0.3 s of tape (tape_race_ms), start there too and
+   move to their compiled kernel once it is ready. */
 typedef struct gpu_server gpu_server;
 typedef struct {
     unsigned streams;          /* device slots (default 4) */
@@ -209,13 +210,18 @@ typedef struct {
     uint64_t chunk_words;      /* words (32 assignments each) per launch (default 2^24) */
     unsigned max_active;       /* admitted requests at once (default 64) */
     double   tape_max_ms;      /* equiv/search requests whose estimated device time on the tape
-                                  interpreter is at most this run there with no compile
-                                  (default 0 = 50 ms; negative = always compile) */
+                                  interpreter is at most this run there with no compile; larger
+                                  ones, up to tape_race_ms, start there and finish compiled
+                                  (default 0 = 50 ms; negative = compile only, no tape) */
+    double   tape_race_ms;     /* the estimated tape time up to which a request worth a compile
+                                  also starts on the tape meanwhile (default 0 = 300 ms, about a
+                                  compile; negative = never) */
 } gpu_serve_tune;
 typedef struct {
     double   submitted_s, compiled_s, admitted_s, started_s, done_s;  /* steady clock, seconds */
     uint64_t launches;
-    uint64_t tier;             /* 0 = compiled kernel, 1 = tape interpreter */
+    uint64_t tier;             /* 0 = compiled kernel, 1 = tape interpreter, 2 = started on the
+                                  tape while its kernel compiled, finished compiled */
 } gpu_request_times;
 /* state: 0 queued, 1 compiling, 2 waiting for admission, 3 running, 4 done, 5 failed */
 int  gpu_server_open(gpu_engine *e, const gpu_serve_tune *tune, gpu_server **out, char *err,

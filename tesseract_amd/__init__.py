@@ -77,7 +77,8 @@ class GpuBatchInfo(C.Structure):
 
 class GpuServeTune(C.Structure):
     _fields_ = [("streams", C.c_uint), ("compile_threads", C.c_uint), ("memory_budget", C.c_uint64),
-                ("chunk_words", C.c_uint64), ("max_active", C.c_uint), ("tape_max_ms", C.c_double)]
+                ("chunk_words", C.c_uint64), ("max_active", C.c_uint), ("tape_max_ms", C.c_double),
+                ("tape_race_ms", C.c_double)]
 
 
 class GpuRequestTimes(C.Structure):
@@ -774,10 +775,10 @@ class Server:
     """gpu_server: continuous-batching request loop on one engine."""
 
     def __init__(self, engine, streams=0, compile_threads=0, memory_budget=0, chunk_words=0,
-                 max_active=0, tape_max_ms=0.0):
+                 max_active=0, tape_max_ms=0.0, tape_race_ms=0.0):
         self.engine = engine
         tune = GpuServeTune(streams, compile_threads, memory_budget, chunk_words, max_active,
-                            tape_max_ms)
+                            tape_max_ms, tape_race_ms)
         h = C.c_void_p()
         e = _err()
         if not lib().gpu_server_open(engine._e, C.byref(tune), C.byref(h), e, len(e)):

@@ -12,11 +12,15 @@
 // come from the earliest chunks (a chunk whose hit list overflowed is listed again by halves).
 // Compiled modules stay resident, keyed by their source, for requests that repeat a circuit.
 //
-// Two tiers: hipRTC takes 0.1-1.5 s for a kernel of 500-35,000 AND nodes, while the tape
+// Two tiers: hipRTC takes 0.3-3 s for a kernel of 500-35,000 AND nodes, while the tape
 // interpreter (tape_kernel.hip) starts at once but runs about 100 times slower than a compiled
 // kernel. An equiv/search request whose estimated tape time is at most tape_max_ms runs on the
-// tape; the rest are compiled. Tape buffers are recycled through a pool, never freed while the
-// server runs (hipFree would wait for the whole device).
+// tape; one of up to 0.3 s of tape starts there too, in chunks of about 2 ms, while its kernel
+// compiles, and its remaining chunks run compiled once the module is in (both tiers list the same
+// hits for the same words, so which tier ran a chunk changes nothing in the result). Larger
+// requests, and verify requests, which have no tape, wait for their compile. Chunks are cut as
+// they are launched, each to the size its tier wants. Tape buffers are recycled through a pool,
+// never freed while the server runs (hipFree would wait for the whole device).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -78,6 +82,11 @@ struct Module {
 // CU): measured 2.4e13-2.5e13 on MI355X for netlist pairs of 3,000-35,000 nodes; smaller files
 // run faster, so the estimate errs on the slow side.
 constexpr double kTapeAndPerSecond = 2.0e13;
+// default estimated tape time below which a request worth a compile also starts on the tape
+// meanwhile: about a compile's time, so that the tape may finish first; a larger tape share is
+// mostly spent work (a compiled kernel does the same words about 80 times faster) that other
+// requests wait for
+constexpr double kTapeRaceMs = 300;
 
 struct TapeBuffer {
     uint32_t *ptr = nullptr;
@@ -87,6 +96,9 @@ struct TapeBuffer {
 struct Request {
     uint64_t id = 0;
     bool tape = false;      // tier: tape interpreter instead of a compiled module
+    bool upgrade = false;   // on the tape while its kernel compiles
+    bool upgrade_done = false, upgrade_ok = false;  // the compile finished (under the lock)
+    uint64_t tape_nodes = 0;
     ts::Tape host_tape;     // until admission
     TapeDev td{};
     TapeBuffer tbuf{};
@@ -98,10 +110,12 @@ struct Request {
     std::string source, error;
     std::vector<char> code;
     gpu_tune tune{};
+    gpu_tune upgrade_tune{};
     ShareInfo si{};
     Module *mod = nullptr;
-    std::vector<Chunk> chunks;
-    size_t next_chunk = 0, chunks_done = 0;
+    std::vector<Chunk> chunks;  // cut as launched, in increasing words
+    size_t chunks_done = 0;
+    uint64_t pos = 0;           // the next word to hand out
     uint64_t words_left = 0;
     ts_enum_result result{};
     int state = ST_QUEUED;
@@ -189,17 +203,21 @@ void plan_loop(gpu_server *s)
                 ShareInfo si = shares((unsigned)r->en.a->ninputs, r->en.part, r->en.part_end,
                                       r->en.parts);
                 double ms = (double)si.tried * (double)tp.and_nodes / kTapeAndPerSecond * 1e3;
-                if (ms <= s->tune.tape_max_ms) {
+                // up to tape_max_ms: the tape alone; up to tape_race_ms: the tape while the kernel
+                // compiles (it may finish first); beyond, the compile alone
+                if (ms <= std::max(s->tune.tape_max_ms, s->tune.tape_race_ms)) {
+                    r->tape_nodes = tp.and_nodes;
                     r->host_tape = std::move(tp);
                     r->tune = tn;
                     r->tape = true;
+                    r->upgrade = ms > s->tune.tape_max_ms;
                 }
             }
         }
         std::lock_guard<std::mutex> lk(s->mu);
-        if (r->tape) {
+        if (r->tape)
             prepared(s, r, true, "");
-        } else {
+        if (!r->tape || r->upgrade) {
             s->to_compile.push_back(r);
             s->cv_compile.notify_one();
         }
@@ -223,18 +241,35 @@ void compile_loop(gpu_server *s)
         }
         char err[1024] = "";
         ts::KernelSource ks;
-        bool ok = enum_prepare(s->e, &r->en, r->tune, ks, err, sizeof err) != 0;
+        gpu_tune tn = r->upgrade ? gpu_tune{} : r->tune;
+        bool ok = enum_prepare(s->e, &r->en, tn, ks, err, sizeof err) != 0;
+        std::string source;
+        std::vector<char> code;
         if (ok) {
-            r->source = ks.text;
+            source = ks.text;
             bool cached;
             {
                 std::lock_guard<std::mutex> lk(s->mu);
-                cached = s->modules.count(r->source) != 0;
+                cached = s->modules.count(source) != 0;
             }
             if (!cached)
-                ok = rtc_compile(r->source, s->e->arch, r->code, nullptr, err, sizeof err) != 0;
+                ok = rtc_compile(source, s->e->arch, code, nullptr, err, sizeof err) != 0;
         }
         std::lock_guard<std::mutex> lk(s->mu);
+        if (r->upgrade) {  // running on the tape: the scheduler switches it over if still running
+            if (r->state == ST_RUNNING || r->state == ST_WAITING) {
+                r->source = std::move(source);
+                r->code = std::move(code);
+                r->upgrade_tune = tn;
+                r->upgrade_ok = ok;
+                r->upgrade_done = true;
+                s->cv_sched.notify_all();
+            }
+            continue;
+        }
+        r->source = std::move(source);
+        r->code = std::move(code);
+        r->tune = tn;
         prepared(s, r, ok, err);
     }
 }
@@ -326,13 +361,6 @@ bool admit(gpu_server *s, Request *r)
     r->result.tried = r->si.tried;
     r->result.ops = r->si.passes_run * r->en.ticks * r->len_total * 512;
     r->result.threads = 1;
-    const uint64_t step = std::max<uint64_t>(1024, s->tune.chunk_words);
-    for (uint64_t i = 0; i < r->si.nact_words; i += step) {
-        Chunk c;
-        c.i0 = i;
-        c.i1 = std::min(r->si.nact_words, i + step);
-        r->chunks.push_back(c);
-    }
     r->words_left = r->si.nact_words;
     r->state = ST_RUNNING;
     r->times.admitted_s = now_s();
@@ -354,6 +382,48 @@ void evict(gpu_server *s, size_t need)
         s->module_bytes -= victim->second.bytes;
         s->modules.erase(victim);
     }
+}
+
+// the next chunk of a request, cut for the tier that will run it: the configured size compiled
+// (and on the tape for a request that stays there), about 2 ms of tape while a compile is pending,
+// so that a request arriving meanwhile finds a slot within 2 ms (a chunk is not preempted)
+size_t cut_chunk(gpu_server *s, Request *r)
+{
+    uint64_t step = std::max<uint64_t>(1024, s->tune.chunk_words);
+    if (r->tape && r->upgrade && r->tape_nodes) {
+        const double words = 2e-3 * kTapeAndPerSecond / (32.0 * (double)r->tape_nodes);
+        step = std::min<uint64_t>(step, std::max<uint64_t>(1024, (uint64_t)words));
+    }
+    Chunk c;
+    c.i0 = r->pos;
+    c.i1 = std::min(r->si.nact_words, r->pos + step);
+    r->pos = c.i1;
+    r->chunks.push_back(c);
+    return r->chunks.size() - 1;
+}
+
+// a request on the tape whose kernel has compiled: later chunks run compiled. A failed compile,
+// or no room for the module, leaves it on the tape (its result is the same).
+void upgrade(gpu_server *s, Request *r)
+{
+    r->upgrade_done = false;
+    if (!r->upgrade_ok || r->pos >= r->si.nact_words)
+        return;
+    size_t need = s->modules.count(r->source) ? 0 : r->code.size();
+    evict(s, need);
+    if (need && s->module_bytes + need > s->tune.memory_budget)
+        return;
+    gpu_tune keep = r->tune;
+    r->tune = r->upgrade_tune;
+    std::string err = r->error;
+    if (!admit_module(s, r)) {
+        r->tune = keep;
+        r->error = err;
+        return;
+    }
+    r->tape = false;
+    r->upgrade = false;
+    r->times.tier = 2;
 }
 
 bool launch_chunk(gpu_server *s, Slot &sl, Request *r, size_t ci)
@@ -507,7 +577,7 @@ void schedule_loop(gpu_server *s)
         // finished requests
         for (size_t k = 0; k < s->active.size();) {
             Request *r = s->active[k];
-            if (r->chunks_done == r->chunks.size() && r->next_chunk == r->chunks.size()) {
+            if (r->chunks_done == r->chunks.size() && r->pos == r->si.nact_words) {
                 bool ok = finalize(s, r);
                 if (r->mod)
                     r->mod->users--;
@@ -527,23 +597,36 @@ void schedule_loop(gpu_server *s)
                 k++;
             }
         }
-        // dispatch: each idle slot takes the next chunk of the request with the least work left
+        // compiles that finished for requests running on the tape
+        for (Request *r : s->active)
+            if (r->upgrade_done) {
+                upgrade(s, r);
+                busy = true;
+            }
+        // dispatch: each idle slot takes the next chunk of the request with the least work left;
+        // requests on the tape while their kernel compiles hold at most half the slots, the rest
+        // staying for arrivals
+        unsigned speculative = 0;
+        for (Slot &sl : s->slots)
+            speculative += sl.req && sl.req->tape && sl.req->upgrade;
         for (Slot &sl : s->slots) {
             if (sl.req)
                 continue;
             Request *best = nullptr;
+            const bool room = 2 * (speculative + 1) <= s->slots.size();
             for (Request *r : s->active)
-                if (r->next_chunk < r->chunks.size() && (!best || r->words_left < best->words_left))
+                if (r->pos < r->si.nact_words && (room || !(r->tape && r->upgrade)) &&
+                    (!best || r->words_left < best->words_left))
                     best = r;
             if (!best)
                 break;
-            if (!launch_chunk(s, sl, best, best->next_chunk)) {
+            speculative += best->tape && best->upgrade;
+            if (!launch_chunk(s, sl, best, cut_chunk(s, best))) {
                 s->fatal = "launch failed";
                 s->stop = true;
                 s->cv_done.notify_all();
                 return;
             }
-            best->next_chunk++;
             busy = true;
         }
         bool in_flight = false;
@@ -583,6 +666,8 @@ extern "C" int gpu_server_open(gpu_engine *e, const gpu_serve_tune *tune, gpu_se
         s->tune.max_active = 64;
     if (s->tune.tape_max_ms == 0)
         s->tune.tape_max_ms = 50;
+    if (s->tune.tape_race_ms == 0)
+        s->tune.tape_race_ms = kTapeRaceMs;
     if (hipSetDevice(e->device) != hipSuccess) {
         put_err(err, errlen, "hipSetDevice failed");
         return 0;

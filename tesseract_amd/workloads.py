@@ -181,7 +181,8 @@ def run_hash(engine, target=0x165C2A07):
             "ms": engine.last_ms, "ops_per_s": r["ops"] / (engine.last_ms * 1e-3)}
 
 
-def run_serve(engine, n, rate, seed, streams=4, chunk_words=1 << 24, check=True, keep=None):
+def run_serve(engine, n, rate, seed, streams=4, chunk_words=1 << 24, check=True, keep=None,
+              tape_race_ms=0.0):
     """A seeded Poisson stream of equiv requests (netlists.request) submitted at their arrival
     times; latency = done - arrival. Offered load is the stream's; served load is what finished."""
     import threading
@@ -196,7 +197,7 @@ def run_serve(engine, n, rate, seed, streams=4, chunk_words=1 << 24, check=True,
         if (k + 1) % 50 == 0:
             print("serve: %d requests generated (%.1f s)" % (k + 1, time.perf_counter() - t_gen),
                   file=sys.stderr, flush=True)
-    srv = t.Server(engine, streams=streams, chunk_words=chunk_words)
+    srv = t.Server(engine, streams=streams, chunk_words=chunk_words, tape_race_ms=tape_race_ms)
     tickets = [None] * n
     t0 = time.perf_counter() + 0.05
 
@@ -224,7 +225,8 @@ def run_serve(engine, n, rate, seed, streams=4, chunk_words=1 << 24, check=True,
            "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
            "compile_p50_ms": float(np.median([tm["compiled_s"] - tm["submitted_s"] for tm in times]) * 1e3),
            "faulty": int(sum(q[5] for q in reqs)), "mismatching": int(sum(r["hits"] > 0 for r in results)),
-           "tape_tier_requests": int(sum(tm["tier"] == 1 for tm in times))}
+           "tape_tier_requests": int(sum(tm["tier"] == 1 for tm in times)),
+           "tape_then_compiled_requests": int(sum(tm["tier"] == 2 for tm in times))}
     classes = {}
     for lo, hi in ((8, 16), (16, 24), (24, 30), (30, 37)):
         m = (n_in >= lo) & (n_in < hi)
@@ -262,6 +264,9 @@ def main():
     ap.add_argument("--cubes", type=int, default=4096)
     ap.add_argument("--lds-cells", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--tape-race-ms", type=float, default=0.0,
+                    help="serve: tape time up to which a compiled request races on the tape "
+                         "(0 = the server's default, negative = never)")
     a = ap.parse_args()
     eng = t.Engine(0)
     if a.workload == "batch":
@@ -270,7 +275,7 @@ def main():
     elif a.workload == "life":
         out = run_life(eng, a.gens)
     elif a.workload == "serve":
-        out = run_serve(eng, a.requests, a.rate, a.seed, a.streams)
+        out = run_serve(eng, a.requests, a.rate, a.seed, a.streams, tape_race_ms=a.tape_race_ms)
     elif a.workload == "benes":
         out = run_benes(eng, a.cubes, a.seed)
     elif a.workload == "polymul":

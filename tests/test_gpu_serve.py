@@ -60,9 +60,10 @@ def test_server_mix_equals_alone(engine):
     assert all(tm["done_s"] <= big_times["done_s"] for tm in inside)
     adder_res = results.pop(6)
     assert adder_res["hits"] == 65536 and adder_res["hit"] == [2 * h + 1 for h in range(16)]
-    # both tiers served: small requests on the tape interpreter, the big one and verify compiled
+    # both tiers served: small requests on the tape interpreter, the big one (far more than 2 s
+    # of tape) and verify compiled
     assert big_times["tier"] == 0 and times[6]["tier"] == 0
-    assert sum(tm["tier"] for tm in times) >= 10
+    assert sum(tm["tier"] == 1 for tm in times) >= 10
     key = lambda r: (r["hits"], r["hit"], r["tried"], r["ops"], r["space"])
     for (a, b, c, r, faulty), got in zip(reqs, results):
         alone = engine.enum_run(t.EN_EQUIV, a, b)
@@ -101,3 +102,26 @@ def test_server_tiers_agree(engine):
         assert tiers == [1 if limit > 0 else 0] * len(reqs)
     key = lambda r: (r["hits"], r["hit"], r["tried"], r["ops"], r["space"])
     assert [key(r) for r in out[1e6]] == [key(r) for r in out[-1.0]]
+
+
+def test_server_tape_then_compiled(engine):
+    # every equiv request is worth a compile (tape_max_ms tiny) and races its compile on the tape
+    # (tape_race_ms large): each starts on the tape in short chunks and moves to its kernel when the
+    # compile is in (2^32-2^33 assignments take the tape longer than a compile); the results equal
+    # each request run alone
+    reqs = []
+    for k in range(6):
+        c, r, faulty = small_request(13, k, [32, 33, 32][k % 3], [800, 1500][k % 2])
+        reqs.append((t.net_parse(c.bench()), t.net_parse(r.bench()), c, r, faulty))
+    srv = t.Server(engine, streams=4, tape_max_ms=1e-6, tape_race_ms=1e6)
+    tks = [srv.submit(t.EN_EQUIV, a, b) for a, b, _, _, _ in reqs]
+    got = [srv.wait(tk) for tk in tks]
+    tiers = [srv.times(tk)["tier"] for tk in tks]
+    srv.close()
+    print("tiers", tiers, flush=True)
+    assert set(tiers) <= {1, 2} and 2 in tiers
+    key = lambda r: (r["hits"], r["hit"], r["tried"], r["ops"], r["space"])
+    for (a, b, c, r, faulty), res in zip(reqs, got):
+        assert key(res) == key(engine.enum_run(t.EN_EQUIV, a, b))
+        for x in res["hit"]:
+            assert c.evaluate(x) != r.evaluate(x)

@@ -65,7 +65,7 @@ def test_linkfree_many_groups(engine):
     check(engine, prog, groups=64, runs=(4,))
 
 
-@pytest.mark.parametrize("dims", [9, 10, 11, 12])
+@pytest.mark.parametrize("dims", [9, 10, 11, 12, 13, 14, 15, 16])
 def test_random_cube(engine, dims):
     prog = random_cube_program(24, 400, dims, seed=100 + dims)
     print("check random cube dims", dims, flush=True)
@@ -116,7 +116,7 @@ def test_graph_replays_match_eager(engine):
     assert all(np.array_equal(a, b) for a, b in zip(st1.get(), st.get()))
 
 
-@pytest.mark.parametrize("dims,seed", [(9, 41), (12, 42)])
+@pytest.mark.parametrize("dims,seed", [(9, 41), (12, 42), (14, 43), (16, 44)])
 def test_run_steps_against_oracle(engine, dims, seed):
     # partial passes: single instructions, a pass finished from the middle, whole passes from
     # pc 0 and a run wrapping from the middle of one pass into the middle of another, each

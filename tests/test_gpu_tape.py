@@ -81,3 +81,18 @@ def test_tape_enum_code_reruns(engine):
     first = code.run()
     assert _same(first, code.run())
     assert _same(first, engine.enum_run(t.EN_EQUIV, a, b))
+
+
+@pytest.mark.parametrize("n_in", [5, 7, 9, 10, 11, 13])
+def test_tape_partial_rounds(engine, n_in):
+    # 1 to 256 words: the last round of a wave has fewer lanes with a word than the 16 that hold
+    # a group's descriptors, and every such lane must still run the tape
+    rng = np.random.default_rng(300 + n_in)
+    c = N.circuit(n_in, 3, 150, rng)
+    r = N.fault(N.rewrite(c, rng), rng)
+    a, b = t.net_parse(c.bench()), t.net_parse(r.bench())
+    print("partial rounds, inputs", n_in, flush=True)
+    got = engine.enum_run(t.EN_EQUIV, a, b, tune=TAPE)
+    o = O.enum(t.EN_EQUIV, a, b, b_in=b.inputs, b_out=b.outputs)
+    assert (got["hits"], got["hit"], got["tried"]) == (o["hits"], o["hit"], o["tried"])
+    assert _same(got, engine.enum_run(t.EN_EQUIV, a, b))

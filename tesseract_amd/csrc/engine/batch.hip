@@ -10,8 +10,12 @@
 // same window, so a window's 8 loads issue together before any of its instructions executes. The
 // window runs without a branch: every slot is one load and one store, a NAND slot storing to a
 // per-wave sink cell and a STO slot loading from it, and rr = M ? !(rr & v) : rr is a single
-// v_bitop3_b32 with the slot's wave-uniform mask M. A window is 16 words (one scalar load): per
-// slot the byte offset of the cell loaded (bit 31 = NAND) and of the cell stored.
+// v_bitop3_b32 with the slot's wave-uniform mask M. A window of W slots is 2W words (scalar
+// loads): the byte offsets of the W cells loaded (bit 31 = NAND), then of the W cells stored, so
+// the loads issue once the first half is in while the second is still on its way.
+// W is 8 or 16 per program, whichever the host's cost model (a fixed wait per window plus a cost
+// per slot, padding included) prefers: a window pays one memory latency for all its loads, so long
+// programs with few hazards take 16 slots and hazard-dense ones 8.
 // A work item is 64 words (2,048 machines) of one program: one wave. A program of one or two
 // groups (the reference's tester runs one: 512 machines) gets items of 16 or 32 words, the wave's
 // other lanes idle, so that its cells take a quarter or half the room. The hare's cells live in
@@ -37,14 +41,16 @@
 
 namespace {
 
-constexpr unsigned kWin = 8;          // instructions per window (two 16-byte records)
+constexpr unsigned kWinSmall = 8;     // slots per window: 8 or 16
+constexpr unsigned kWinLarge = 16;
 constexpr unsigned kLanes = 64;       // words per work item (one wave)
 constexpr unsigned kStats = 67;       // hist[64], lam_sum, longest, passes (sum over groups)
 constexpr unsigned kSample = 64;      // cells compared before a full comparison (reference: 64)
 
 struct BatchProg {
-    uint32_t rec_off;    // first window record
-    uint32_t nrec;       // windows per pass (16 words each)
+    uint32_t rec_off;    // first window record (16-byte units)
+    uint32_t nrec;       // windows per pass (2 x win words each)
+    uint32_t win;        // slots per window (8 or 16)
     uint32_t ncells;     // cells incl. ZERO and ONE (the sink is cell ncells)
     uint32_t nsample;    // sampled cells (<= 64)
     uint32_t sample_off; // into the sample list
@@ -82,36 +88,52 @@ struct Cells {
     }
 };
 
-// One window: its 8 loads issue together, then the 8 slots run in order.
-__device__ __forceinline__ void run_window(const Cells &h, uint32_t lane4, const uint32_t (&w)[16],
+// One window: its W loads issue together, then the W slots run in order.
+template <unsigned W>
+__device__ __forceinline__ void run_window(const Cells &h, uint32_t lane4, const uint32_t (&w)[2 * W],
                                            uint32_t &rr)
 {
-    uint32_t v[8];
+    uint32_t v[W];
 #pragma unroll
-    for (int k = 0; k < 8; k++)
-        v[k] = h.ld_at(w[2 * k] & 0x7FFFFFFFu, lane4);
+    for (unsigned k = 0; k < W; k++)
+        v[k] = h.ld_at(w[k] & 0x7FFFFFFFu, lane4);
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const uint32_t m = (uint32_t)((int32_t)w[2 * k] >> 31);  // all ones for a NAND
-        rr = __builtin_amdgcn_bitop3_b32(m, rr, v[k], 0x7Cu);   // m ? !(rr & v) : rr
-        h.st_at(w[2 * k + 1], lane4, rr);
+    for (unsigned k = 0; k < W; k++) {
+        const uint32_t m = (uint32_t)((int32_t)w[k] >> 31);  // all ones for a NAND
+        rr = __builtin_amdgcn_bitop3_b32(m, rr, v[k], 0x7Cu); // m ? !(rr & v) : rr
+        h.st_at(w[W + k], lane4, rr);
+    }
+}
+
+template <unsigned W>
+__device__ __forceinline__ void run_windows(const Cells &h, uint32_t lane4, const uint4 *rec,
+                                            uint32_t nwin, uint32_t &rr)
+{
+    constexpr unsigned Q = 2 * W / 4;  // 16-byte records per window
+    for (uint32_t r = 0; r < nwin; r++) {
+        uint32_t w[2 * W];
+#pragma unroll
+        for (unsigned q = 0; q < Q; q++) {
+            const uint4 x = rec[Q * r + q];
+            w[4 * q] = x.x;
+            w[4 * q + 1] = x.y;
+            w[4 * q + 2] = x.z;
+            w[4 * q + 3] = x.w;
+        }
+        run_window<W>(h, lane4, w, rr);
     }
 }
 
 // One pass: each window's record through the scalar cache as the window starts. A lone wave
-// (a search of few programs) spends about 27 cycles per instruction here whether its cells are in
-// LDS or HBM and whether the records are fetched ahead or not: the slots' dependent chain, not a
-// memory latency, sets the pace.
+// (a search of few programs) waits about one memory latency per window (its loads follow the
+// previous window's stores) and then runs the window's dependent chain of W slots.
 __device__ __forceinline__ void run_pass(const Cells &h, uint32_t lane, const uint4 *rec,
-                                         uint32_t nwin, uint32_t &rr)
+                                         uint32_t nwin, uint32_t win, uint32_t &rr)
 {
-    const uint32_t lane4 = lane * 4;
-    for (uint32_t r = 0; r < nwin; r++) {
-        const uint4 q0 = rec[4 * r], q1 = rec[4 * r + 1], q2 = rec[4 * r + 2], q3 = rec[4 * r + 3];
-        const uint32_t w[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
-                                q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
-        run_window(h, lane4, w, rr);
-    }
+    if (win == kWinLarge)
+        run_windows<kWinLarge>(h, lane * 4, rec, nwin, rr);
+    else
+        run_windows<kWinSmall>(h, lane * 4, rec, nwin, rr);
 }
 
 // sum over the wave's active lanes of popcount(m): one ballot per bit, so lanes that idle (items
@@ -203,7 +225,7 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
             for (uint32_t c = 0; c < ncells; c++)
                 tort.st(c, lane, hare.ld(c, lane));
             uint32_t trr = rr;
-            run_pass(hare, lane, rec, P.nrec, rr);
+            run_pass(hare, lane, rec, P.nrec, P.win, rr);
             uint64_t lam = 1, power = 1, steps = 1;
             uint32_t active = word < prog_words ? 0xFFFFFFFFu : 0u;
             const uint32_t *smp = samples + P.sample_off;
@@ -265,7 +287,7 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
                     power *= 2;
                     lam = 0;
                 }
-                run_pass(hare, lane, rec, P.nrec, rr);
+                run_pass(hare, lane, rec, P.nrec, P.win, rr);
                 lam++;
                 steps++;
             }
@@ -280,27 +302,29 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
 }
 
 struct Encoded {
-    std::vector<uint32_t> words;  // windows, 16 words each: (load offset | NAND << 31, store offset) x 8
-    uint32_t nwin = 0;
+    std::vector<uint32_t> words;  // windows of 2 x win words: win load offsets (| NAND << 31), then win store offsets
+    uint32_t nwin = 0, win = 0;
 };
 
-// Windows of up to 8 instructions in which no NAND reads a cell stored earlier in the window.
+// Windows of up to `win` instructions in which no NAND reads a cell stored earlier in the window.
 // Offsets are bytes into an item's cells (cell * row). A NAND stores to the sink cell, a STO loads
 // from it; STO to read-only cells (ignored by the machine) and padding store to it too.
-Encoded encode(const ts_program *p, uint32_t row)
+Encoded encode(const ts_program *p, uint32_t row, unsigned win_slots)
 {
     Encoded e;
+    e.win = win_slots;
     const uint32_t sink = p->ncells * row;
-    std::vector<uint32_t> win;
-    std::vector<uint32_t> stored;
+    std::vector<uint32_t> lds_, sts, stored;  // the open window's load and store offsets
     auto flush = [&]() {
-        while (win.size() < 2 * kWin) {
-            win.push_back(sink);
-            win.push_back(sink);
+        while (lds_.size() < win_slots) {
+            lds_.push_back(sink);
+            sts.push_back(sink);
         }
-        e.words.insert(e.words.end(), win.begin(), win.end());
+        e.words.insert(e.words.end(), lds_.begin(), lds_.end());
+        e.words.insert(e.words.end(), sts.begin(), sts.end());
         e.nwin++;
-        win.clear();
+        lds_.clear();
+        sts.clear();
         stored.clear();
     };
     for (size_t i = 0; i < p->len; i++) {
@@ -308,22 +332,34 @@ Encoded encode(const ts_program *p, uint32_t row)
         if (in.op == TS_OP_NAND) {
             if (std::find(stored.begin(), stored.end(), in.addr) != stored.end())
                 flush();
-            win.push_back(in.addr * row | 0x80000000u);
-            win.push_back(sink);
+            lds_.push_back(in.addr * row | 0x80000000u);
+            sts.push_back(sink);
         } else {
             bool ro = in.addr < TS_CELL_FIRST_FREE ||
                       (p->special && p->special[in.addr].kind != TS_CK_PLAIN);
-            win.push_back(sink);
-            win.push_back(ro ? sink : in.addr * row);
+            lds_.push_back(sink);
+            sts.push_back(ro ? sink : in.addr * row);
             if (!ro)
                 stored.push_back(in.addr);
         }
-        if (win.size() == 2 * kWin)
+        if (lds_.size() == win_slots)
             flush();
     }
-    if (!win.empty())
+    if (!lds_.empty())
         flush();
     return e;
+}
+
+// The window size with the smaller estimated pass time: each window waits `wait` cycles for its
+// loads (about a memory latency when nothing else runs on the CU meanwhile, little when other
+// waves fill it), then spends kSlotCycles per slot, padding slots included.
+Encoded pick_encoding(const ts_program *p, uint32_t row, double wait)
+{
+    constexpr double kSlotCycles = 8.0;
+    Encoded small = encode(p, row, kWinSmall), large = encode(p, row, kWinLarge);
+    const double t_small = small.nwin * (wait + kWinSmall * kSlotCycles);
+    const double t_large = large.nwin * (wait + kWinLarge * kSlotCycles);
+    return t_large < t_small ? large : small;
 }
 
 void say(char *err, size_t errlen, const char *fmt, const char *what)
@@ -412,12 +448,21 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     std::vector<uint32_t> item_of[3];
     uint32_t maxcells[3] = {0, 0, 0};
     uint64_t ops = 0;
+    // a window's wait for its loads: HBM latency in the HBM class; in LDS, hidden when the
+    // class has more than two items per CU, otherwise the lone wave's LDS latency
+    uint64_t class_items[3] = {0, 0, 0};
+    for (size_t i = 0; i < n; i++)
+        class_items[class_of(items[i].prog)] += items_per_prog;
+    double wait[3];
+    for (int c = 0; c < 3; c++)
+        wait[c] = c == 2 || class_items[c] <= 2ull * e->cus ? 150.0 : 20.0;
     for (size_t i = 0; i < n; i++) {
         const ts_program *p = items[i].prog;
-        Encoded enc = encode(p, row);
+        Encoded enc = pick_encoding(p, row, wait[class_of(p)]);
         BatchProg &bp = progs[i];
         bp.rec_off = (uint32_t)(recs.size() / 4);
         bp.nrec = enc.nwin;
+        bp.win = enc.win;
         recs.insert(recs.end(), enc.words.begin(), enc.words.end());
         bp.ncells = p->ncells;
         uint32_t writable = p->ncells - TS_CELL_FIRST_FREE;

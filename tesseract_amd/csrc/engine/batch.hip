@@ -6,11 +6,12 @@
 // Programs are data, not compiled code: a hipRTC compile per random program would cost more than
 // running it. Each instruction touches one 4-byte word per 32 machines, so the interpreter must
 // keep many accesses in flight and spend almost nothing on control. Each program is re-encoded on
-// the host into *windows* of 8 instructions in which no NAND reads a cell stored earlier in the
-// same window, so a window's 8 loads issue together before any of its instructions executes. The
-// window runs without a branch: every slot is one load and one store, a NAND slot storing to a
-// per-wave sink cell and a STO slot loading from it, and rr = M ? !(rr & v) : rr is a single
-// v_bitop3_b32 with the slot's wave-uniform mask M. A window of W slots is 2W words (scalar
+// the host into *windows* of slots in which no NAND reads a cell stored earlier in the same
+// window, so a window's loads issue together before any of its slots executes. A slot is a NAND
+// and the STO that follows it, either of which may be missing (a missing NAND loads a per-wave
+// sink cell, a missing STO stores to it). The window runs without a branch: every slot is one load
+// and one store, and rr = M ? !(rr & v) : rr is a single v_bitop3_b32 with the slot's
+// wave-uniform mask M. A window of W slots is 2W words (scalar
 // loads): the byte offsets of the W cells loaded (bit 31 = NAND), then of the W cells stored, so
 // the loads issue once the first half is in while the second is still on its way.
 // W is 8 or 16 per program, whichever the host's cost model (a fixed wait per window plus a cost
@@ -306,15 +307,19 @@ struct Encoded {
     uint32_t nwin = 0, win = 0;
 };
 
-// Windows of up to `win` instructions in which no NAND reads a cell stored earlier in the window.
-// Offsets are bytes into an item's cells (cell * row). A NAND stores to the sink cell, a STO loads
-// from it; STO to read-only cells (ignored by the machine) and padding store to it too.
+// Windows of up to `win` slots in which no NAND reads a cell stored earlier in the window.
+// Offsets are bytes into an item's cells (cell * row). A slot is a NAND (or nothing: a load of the
+// sink cell) followed by a STO (or nothing: a store to the sink): a STO right after a NAND shares
+// its slot, since the slot stores rr after the NAND, so a random program of NANDs and STOs in
+// equal parts takes about three slots per four instructions. A STO to a read-only cell (ignored
+// by the machine) takes no slot; padding loads and stores the sink.
 Encoded encode(const ts_program *p, uint32_t row, unsigned win_slots)
 {
     Encoded e;
     e.win = win_slots;
     const uint32_t sink = p->ncells * row;
     std::vector<uint32_t> lds_, sts, stored;  // the open window's load and store offsets
+    bool open_nand = false;                   // the last slot is a NAND that stores nothing yet
     auto flush = [&]() {
         while (lds_.size() < win_slots) {
             lds_.push_back(sink);
@@ -326,24 +331,31 @@ Encoded encode(const ts_program *p, uint32_t row, unsigned win_slots)
         lds_.clear();
         sts.clear();
         stored.clear();
+        open_nand = false;
     };
     for (size_t i = 0; i < p->len; i++) {
         const ts_instr in = p->code[i];
         if (in.op == TS_OP_NAND) {
-            if (std::find(stored.begin(), stored.end(), in.addr) != stored.end())
+            if (lds_.size() == win_slots ||
+                std::find(stored.begin(), stored.end(), in.addr) != stored.end())
                 flush();
             lds_.push_back(in.addr * row | 0x80000000u);
             sts.push_back(sink);
-        } else {
-            bool ro = in.addr < TS_CELL_FIRST_FREE ||
-                      (p->special && p->special[in.addr].kind != TS_CK_PLAIN);
-            lds_.push_back(sink);
-            sts.push_back(ro ? sink : in.addr * row);
-            if (!ro)
-                stored.push_back(in.addr);
+            open_nand = true;
+            continue;
         }
-        if (lds_.size() == win_slots)
-            flush();
+        if (in.addr < TS_CELL_FIRST_FREE || (p->special && p->special[in.addr].kind != TS_CK_PLAIN))
+            continue;
+        if (open_nand) {
+            sts.back() = in.addr * row;
+            open_nand = false;
+        } else {
+            if (lds_.size() == win_slots)
+                flush();
+            lds_.push_back(sink);
+            sts.push_back(in.addr * row);
+        }
+        stored.push_back(in.addr);
     }
     if (!lds_.empty())
         flush();

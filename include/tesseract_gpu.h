@@ -163,6 +163,10 @@ typedef struct {
                                as many bytes as 130 cells of 2,048 machines, more when few work
                                items share a CU) */
     unsigned waves_per_cu;  /* persistent waves per CU per class, LDS permitting (default 16) */
+    unsigned lds_large_cells; /* programs too large for lds_cells with ncells + 1 <= this keep
+                               their machines in LDS in a class of their own, down to two waves per
+                               CU (default and upper bound: what two items per CU allow; a value
+                               <= lds_cells turns the class off) */
 } gpu_batch_tune;
 
 typedef struct {
@@ -170,9 +174,9 @@ typedef struct {
     uint64_t ops;           /* 1-bit ops: instructions x passes x machines, summed */
     uint64_t launches;
     unsigned lds_cells;
-    double   class_ms[3];   /* batched: from the start to the end of each residency class's launch
-                               (LDS small, LDS, HBM); the three run concurrently */
-    uint64_t class_ops[3];
+    double   class_ms[4];   /* batched: from the start to the end of each residency class's launch
+                               (LDS small, LDS, LDS large, HBM); they run concurrently */
+    uint64_t class_ops[4];
 } gpu_batch_info;
 
 int  gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t n, const gpu_batch_tune *tune,

@@ -67,12 +67,12 @@ class GpuBatchStats(C.Structure):
 
 class GpuBatchTune(C.Structure):
     _fields_ = [("groups", C.c_uint), ("max_passes", C.c_uint), ("plain", C.c_uint),
-                ("lds_cells", C.c_uint), ("waves_per_cu", C.c_uint)]
+                ("lds_cells", C.c_uint), ("waves_per_cu", C.c_uint), ("lds_large_cells", C.c_uint)]
 
 
 class GpuBatchInfo(C.Structure):
     _fields_ = [("ms", C.c_double), ("ops", C.c_uint64), ("launches", C.c_uint64),
-                ("lds_cells", C.c_uint), ("class_ms", C.c_double * 3), ("class_ops", C.c_uint64 * 3)]
+                ("lds_cells", C.c_uint), ("class_ms", C.c_double * 4), ("class_ops", C.c_uint64 * 4)]
 
 
 class GpuServeTune(C.Structure):
@@ -705,14 +705,14 @@ class Engine:
         return float(lib().gpu_last_ms(self._e))
 
     def batch_run(self, progs, seeds, groups=2048, max_passes=64, plain=False, lds_cells=0,
-                  waves_per_cu=0):
+                  waves_per_cu=0, lds_large_cells=0):
         """gpu_batch_run: per program a dict(hist, lam_sum, longest, passes); plus the run info."""
         n = len(progs)
         items = (GpuBatchItem * n)()
         for i, (p, sd) in enumerate(zip(progs, seeds)):
             items[i].prog = p.ptr
             items[i].seed = sd
-        tune = GpuBatchTune(groups, max_passes, int(plain), lds_cells, waves_per_cu)
+        tune = GpuBatchTune(groups, max_passes, int(plain), lds_cells, waves_per_cu, lds_large_cells)
         stats = (GpuBatchStats * n)()
         info = GpuBatchInfo()
         e = _err()

@@ -47,6 +47,8 @@ constexpr unsigned kWinLarge = 16;
 constexpr unsigned kLanes = 64;       // words per work item (one wave)
 constexpr unsigned kStats = 67;       // hist[64], lam_sum, longest, passes (sum over groups)
 constexpr unsigned kSample = 64;      // cells compared before a full comparison (reference: 64)
+constexpr int kClasses = 4;           // residency classes: LDS small, LDS, LDS large, HBM
+constexpr int kHbm = 3;
 
 struct BatchProg {
     uint32_t rec_off;    // first window record (16-byte units)
@@ -435,42 +437,79 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     // items as wide as the program needs, up to the wave: 16 words for one group, 32 for two
     const uint32_t item_words = tune.groups == 1 ? 16 : tune.groups == 2 ? 32 : kLanes;
     const uint32_t row = item_words * 4;  // bytes of one cell of an item
-    const uint32_t items_per_prog = (prog_words + item_words - 1) / item_words;
     const size_t lds_max = e->lds_per_block;
     if (!tune_in || !tune_in->lds_cells) {
         // as many bytes per item as 130 cells of a full wave; more when so few items run that
         // each CU holds only a few (a search of few large programs keeps its hare in LDS)
-        const uint64_t items_total = (uint64_t)n * items_per_prog;
+        const uint64_t items_total = (uint64_t)n * ((prog_words + item_words - 1) / item_words);
         const uint64_t per_cu_needed = std::max<uint64_t>(1, (items_total + e->cus - 1) / e->cus);
         const uint64_t fit = lds_max / per_cu_needed / row;  // cells, the sink included
         tune.lds_cells = (unsigned)std::max<uint64_t>(130ull * kLanes / item_words,
                                                       fit ? fit - 1 : 0);
     }
     tune.lds_cells = (unsigned)std::min<uint64_t>(tune.lds_cells, lds_max / row - 1);
+    // the large LDS class: programs of which two items still fit a CU's LDS
+    const uint32_t two_fit = (uint32_t)(lds_max / (2 * (size_t)row));
+    if (!tune.lds_large_cells)
+        tune.lds_large_cells = two_fit;
+    tune.lds_large_cells = std::min<uint32_t>(tune.lds_large_cells, two_fit);
 
-    // host encoding, by residency class: 0 = LDS (<= 32 cells with the sink), 1 = LDS (<= lds_cells),
-    // 2 = HBM; each class is one persistent launch sized by what a CU can hold
-    const uint32_t limit[3] = {std::min<uint32_t>(32, tune.lds_cells), tune.lds_cells, 0xFFFFFFFFu};
+    // residency classes: 0 = LDS (<= 32 cells with the sink), 1 = LDS (<= lds_cells), 2 = LDS
+    // (<= lds_large_cells, as few as two waves per CU), 3 = HBM; each class is one persistent
+    // launch sized by what a CU can hold
+    const uint32_t limit[kClasses] = {std::min<uint32_t>(32, tune.lds_cells), tune.lds_cells,
+                                      tune.lds_large_cells, 0xFFFFFFFFu};
     auto class_of = [&](const ts_program *p) {
         uint32_t need = p->ncells + 1;
-        return need <= limit[0] ? 0 : need <= limit[1] ? 1 : 2;
+        return need <= limit[0] ? 0 : need <= limit[1] ? 1 : need <= limit[2] ? 2 : 3;
     };
+    const uint32_t per_prog = (prog_words + item_words - 1) / item_words;  // items per program
     std::vector<BatchProg> progs(n);
     std::vector<uint32_t> recs, samples;
-    std::vector<uint32_t> item_of[3];
-    uint32_t maxcells[3] = {0, 0, 0};
+    std::vector<uint32_t> item_of[kClasses];
+    uint32_t maxcells[kClasses] = {0, 0, 0, 0};
+    uint64_t class_items[kClasses] = {0, 0, 0, 0};
     uint64_t ops = 0;
-    // a window's wait for its loads: HBM latency in the HBM class; in LDS, hidden when the
-    // class has more than two items per CU, otherwise the lone wave's LDS latency
-    uint64_t class_items[3] = {0, 0, 0};
-    for (size_t i = 0; i < n; i++)
-        class_items[class_of(items[i].prog)] += items_per_prog;
-    double wait[3];
-    for (int c = 0; c < 3; c++)
-        wait[c] = c == 2 || class_items[c] <= 2ull * e->cus ? 150.0 : 20.0;
+    for (size_t i = 0; i < n; i++) {
+        const int c = class_of(items[i].prog);
+        class_items[c] += per_prog;
+        maxcells[c] = std::max(maxcells[c], items[i].prog->ncells + 1);
+    }
+    // per class: persistent waves per CU (LDS bounds the LDS classes; the HBM regions of all
+    // waves stay within 16 GiB) and its HBM regions (hare and tortoise per wave)
+    unsigned per_cu[kClasses];
+    bool tort_lds[kClasses] = {false, false, false, false};
+    size_t scratch_off[kClasses + 1] = {0, 0, 0, 0, 0};
+    constexpr size_t kRegionBudget = (size_t)16 << 30;
+    for (int c = 0; c < kClasses; c++) {
+        per_cu[c] = tune.waves_per_cu;
+        if (c != kHbm && maxcells[c]) {
+            per_cu[c] = (unsigned)std::max<size_t>(
+                1, std::min<size_t>(per_cu[c], lds_max / ((size_t)maxcells[c] * row)));
+            // the tortoise joins the hare in LDS when every item of the class still finds a
+            // wave slot (Brent compares the whole state each pass once the sample matches)
+            const size_t both = std::min<size_t>(tune.waves_per_cu,
+                                                 lds_max / (2 * (size_t)maxcells[c] * row));
+            if (both && class_items[c] <= (size_t)e->cus * both) {
+                tort_lds[c] = true;
+                per_cu[c] = (unsigned)both;
+            }
+        }
+        const size_t region = (size_t)2 * maxcells[c] * row;
+        if (region)
+            per_cu[c] = (unsigned)std::max<size_t>(
+                1, std::min<size_t>(per_cu[c], kRegionBudget / ((size_t)e->cus * region)));
+        scratch_off[c + 1] = scratch_off[c] + (size_t)e->cus * per_cu[c] * region / 4;
+    }
+    // a window's wait for its loads: HBM latency in the HBM class; in LDS, hidden when the class
+    // runs more than two waves per CU, otherwise the lone wave's LDS latency
+    double wait[kClasses];
+    for (int c = 0; c < kClasses; c++)
+        wait[c] = c == kHbm || per_cu[c] <= 2 || class_items[c] <= 2ull * e->cus ? 150.0 : 20.0;
     for (size_t i = 0; i < n; i++) {
         const ts_program *p = items[i].prog;
-        Encoded enc = pick_encoding(p, row, wait[class_of(p)]);
+        const int cls = class_of(p);
+        Encoded enc = pick_encoding(p, row, wait[cls]);
         BatchProg &bp = progs[i];
         bp.rec_off = (uint32_t)(recs.size() / 4);
         bp.nrec = enc.nwin;
@@ -485,52 +524,24 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
         while ((samples.size() - bp.sample_off) % 8)  // the kernel compares 8 at a time
             samples.push_back(samples.back());
         bp.seed = items[i].seed;
-        int cls = class_of(p);
         bp.first_item = (uint32_t)item_of[cls].size();
-        for (uint32_t k = 0; k < items_per_prog; k++)
+        for (uint32_t k = 0; k < per_prog; k++)
             item_of[cls].push_back((uint32_t)i);
-        maxcells[cls] = std::max(maxcells[cls], p->ncells + 1);
         ops += (uint64_t)p->len * tune.max_passes * tune.groups * 512ull;
     }
     DevBuf<BatchProg> d_progs;
     DevBuf<uint4> d_recs;
-    DevBuf<uint32_t> d_samples, d_items[3], d_queue, d_scratch;
+    DevBuf<uint32_t> d_samples, d_items[kClasses], d_queue, d_scratch;
     DevBuf<uint64_t> d_stats;
     bool ok = d_progs.alloc(n) && d_recs.alloc(recs.size() / 4) && d_samples.alloc(samples.size()) &&
-              d_queue.alloc(3 + n) && d_stats.alloc(n * kStats);
-    for (int c = 0; c < 3; c++)
+              d_queue.alloc(kClasses + n) && d_stats.alloc(n * kStats);
+    for (int c = 0; c < kClasses; c++)
         ok = ok && d_items[c].alloc(item_of[c].size());
     if (!ok) {
         say(err, errlen, "%s", "out of device memory for the batch");
         return 0;
     }
-    // per class: persistent waves per CU (LDS bounds the LDS classes; the HBM regions of all
-    // waves stay within 16 GiB) and its HBM regions (hare and tortoise per wave)
-    unsigned per_cu[3];
-    bool tort_lds[3] = {false, false, false};
-    size_t scratch_off[4] = {0, 0, 0, 0};
-    constexpr size_t kRegionBudget = (size_t)16 << 30;
-    for (int c = 0; c < 3; c++) {
-        per_cu[c] = tune.waves_per_cu;
-        if (c < 2 && maxcells[c]) {
-            per_cu[c] = (unsigned)std::max<size_t>(
-                1, std::min<size_t>(per_cu[c], lds_max / ((size_t)maxcells[c] * row)));
-            // the tortoise joins the hare in LDS when every item of the class still finds a
-            // wave slot (Brent compares the whole state each pass once the sample matches)
-            const size_t both = std::min<size_t>(tune.waves_per_cu,
-                                                 lds_max / (2 * (size_t)maxcells[c] * row));
-            if (both && item_of[c].size() <= (size_t)e->cus * both) {
-                tort_lds[c] = true;
-                per_cu[c] = (unsigned)both;
-            }
-        }
-        const size_t region = (size_t)2 * maxcells[c] * row;
-        if (region)
-            per_cu[c] = (unsigned)std::max<size_t>(
-                1, std::min<size_t>(per_cu[c], kRegionBudget / ((size_t)e->cus * region)));
-        scratch_off[c + 1] = scratch_off[c] + (size_t)e->cus * per_cu[c] * region / 4;
-    }
-    if (!d_scratch.alloc(scratch_off[3])) {
+    if (!d_scratch.alloc(scratch_off[kClasses])) {
         say(err, errlen, "%s", "out of device memory for the machines' HBM regions");
         return 0;
     }
@@ -543,11 +554,11 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     BCHK(hipMemcpyAsync(d_progs.p, progs.data(), n * sizeof(BatchProg), hipMemcpyHostToDevice, s));
     BCHK(hipMemcpyAsync(d_recs.p, recs.data(), recs.size() * 4, hipMemcpyHostToDevice, s));
     BCHK(hipMemcpyAsync(d_samples.p, samples.data(), samples.size() * 4, hipMemcpyHostToDevice, s));
-    for (int c = 0; c < 3; c++)
+    for (int c = 0; c < kClasses; c++)
         if (!item_of[c].empty())
             BCHK(hipMemcpyAsync(d_items[c].p, item_of[c].data(), item_of[c].size() * 4,
                                 hipMemcpyHostToDevice, s));
-    BCHK(hipMemsetAsync(d_queue.p, 0, (3 + n) * 4, s));
+    BCHK(hipMemsetAsync(d_queue.p, 0, (kClasses + n) * 4, s));
     BCHK(hipMemsetAsync(d_stats.p, 0, n * kStats * 8, s));
 
     uint64_t launches = 0;
@@ -556,12 +567,12 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
         const uint32_t *ip = d_items[cls].p;
         unsigned grid = std::min<unsigned>((unsigned)e->cus * per_cu[cls], count);
         uint32_t *sc = d_scratch.p + scratch_off[cls];
-        if (cls < 2 && tort_lds[cls]) {
+        if (cls != kHbm && tort_lds[cls]) {
             const size_t lds_bytes = 2 * (size_t)maxcells[cls] * row;
             hipLaunchKernelGGL((ts_batch<true, true>), dim3(grid), dim3(kLanes), lds_bytes, st,
                                d_progs.p, ip, first, count, d_recs.p, d_samples.p, queue, sc,
                                maxcells[cls], d_stats.p, tune.max_passes, prog_words, item_words);
-        } else if (cls < 2) {
+        } else if (cls != kHbm) {
             const size_t lds_bytes = (size_t)maxcells[cls] * row;
             hipLaunchKernelGGL((ts_batch<true, false>), dim3(grid), dim3(kLanes), lds_bytes, st,
                                d_progs.p, ip, first, count, d_recs.p, d_samples.p, queue, sc,
@@ -574,19 +585,23 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
         launches++;
         return hipGetLastError() == hipSuccess;
     };
-    // batched: the three classes run concurrently, LDS-bound next to HBM-bound work, on side
-    // streams that join the engine's stream; plain: one launch per program on the engine's stream
-    hipStream_t side[3] = {nullptr, nullptr, nullptr};
-    hipEvent_t start_ev = nullptr, done[3] = {nullptr, nullptr, nullptr};
+    // batched: the classes run concurrently, LDS-bound next to HBM-bound work. The HBM class runs
+    // on the engine's stream, the large LDS class on a side stream and the two small LDS classes
+    // (short) one after the other on another, so that no two long classes share one of the
+    // process's four hardware queues (a fifth stream would share one, and its kernels would wait
+    // behind that queue's work); the engine's stream joins the side streams after every launch.
+    // Plain: one launch per program on the engine's stream.
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t start_ev = nullptr, done[kClasses] = {nullptr, nullptr, nullptr, nullptr};
     struct Cleanup {
         hipStream_t *s;
         hipEvent_t *st, *d;
         ~Cleanup()
         {
-            for (int k = 0; k < 3; k++) {
+            for (int k = 0; k < 2; k++)
                 if (s[k]) (void)hipStreamDestroy(s[k]);
+            for (int k = 0; k < kClasses; k++)
                 if (d[k]) (void)hipEventDestroy(d[k]);
-            }
             if (*st) (void)hipEventDestroy(*st);
         }
     } cleanup{side, &start_ev, done};
@@ -594,24 +609,36 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     if (!tune.plain) {
         BCHK(hipEventCreateWithFlags(&start_ev, hipEventDisableTiming));
         BCHK(hipEventRecord(start_ev, s));
-        for (int cls = 0; cls < 3; cls++) {
+        const int order[kClasses] = {kHbm, 2, 1, 0};
+        for (int cls : order) {
             if (item_of[cls].empty())
                 continue;
-            BCHK(hipStreamCreateWithFlags(&side[cls], hipStreamNonBlocking));
+            hipStream_t st = s;
+            if (cls != kHbm) {
+                hipStream_t &sd = side[cls == 2 ? 0 : 1];
+                if (!sd) {
+                    BCHK(hipStreamCreateWithFlags(&sd, hipStreamNonBlocking));
+                    BCHK(hipStreamWaitEvent(sd, start_ev, 0));
+                }
+                st = sd;
+            }
             BCHK(hipEventCreate(&done[cls]));
-            BCHK(hipStreamWaitEvent(side[cls], start_ev, 0));
-            if (!launch_class(cls, side[cls], d_queue.p + cls, 0, (uint32_t)item_of[cls].size())) {
+            if (!launch_class(cls, st, d_queue.p + cls, 0, (uint32_t)item_of[cls].size())) {
                 say(err, errlen, "%s", "batch launch failed");
                 return 0;
             }
-            BCHK(hipEventRecord(done[cls], side[cls]));
-            BCHK(hipStreamWaitEvent(s, done[cls], 0));
+            BCHK(hipEventRecord(done[cls], st));
         }
+        for (int k = 0; k < 2; k++)
+            if (side[k]) {
+                hipEvent_t last = k == 0 ? done[2] : (done[0] ? done[0] : done[1]);
+                BCHK(hipStreamWaitEvent(s, last, 0));
+            }
     } else {
         // one launch per program, in program order (the baseline the batch is compared with)
         for (size_t i = 0; i < n; i++) {
             int cls = class_of(items[i].prog);
-            if (!launch_class(cls, s, d_queue.p + 3 + i, progs[i].first_item, items_per_prog)) {
+            if (!launch_class(cls, s, d_queue.p + kClasses + i, progs[i].first_item, per_prog)) {
                 say(err, errlen, "%s", "batch launch failed");
                 return 0;
             }
@@ -635,7 +662,7 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     for (size_t i = 0; i < n; i++)
         ops += st[i * kStats + 66] * items[i].prog->len * 512ull;
     if (info && !tune.plain)
-        for (int k = 0; k < 3; k++) {
+        for (int k = 0; k < kClasses; k++) {
             float cms = 0;
             if (done[k])
                 BCHK(hipEventElapsedTime(&cms, e->ev0, done[k]));

@@ -48,13 +48,15 @@ def batch_mix(n, seed):
     return progs, seeds
 
 
-def run_batch(engine, n, groups, passes, seed, plain_too=True, lds_cells=0, waves_per_cu=0):
+def run_batch(engine, n, groups, passes, seed, plain_too=True, lds_cells=0, waves_per_cu=0,
+              lds_large_cells=0):
     progs, seeds = batch_mix(n, seed)
     out = {"workload": "batch", "programs": n, "machines_per_program": groups * 512,
            "passes": passes, "instructions": sum(p.len for p in progs)}
     t0 = time.perf_counter()
     stats, info = engine.batch_run(progs, seeds, groups=groups, max_passes=passes,
-                                   lds_cells=lds_cells, waves_per_cu=waves_per_cu)
+                                   lds_cells=lds_cells, waves_per_cu=waves_per_cu,
+                                   lds_large_cells=lds_large_cells)
     out["batched_wall_s"] = time.perf_counter() - t0
     out["batched_ms"] = info["ms"]
     out["ops"] = info["ops"]
@@ -64,7 +66,8 @@ def run_batch(engine, n, groups, passes, seed, plain_too=True, lds_cells=0, wave
     out["class_ops_per_s"] = [o / (m * 1e-3) if m else 0 for o, m in zip(info["class_ops"], info["class_ms"])]
     if plain_too:
         stats_p, info_p = engine.batch_run(progs, seeds, groups=groups, max_passes=passes, plain=True,
-                                           lds_cells=lds_cells, waves_per_cu=waves_per_cu)
+                                           lds_cells=lds_cells, waves_per_cu=waves_per_cu,
+                                           lds_large_cells=lds_large_cells)
         out["plain_ms"] = info_p["ms"]
         out["plain_ops_per_s"] = info_p["ops"] / (info_p["ms"] * 1e-3)
         out["launches_plain"] = info_p["launches"]
@@ -264,6 +267,9 @@ def main():
     ap.add_argument("--cubes", type=int, default=4096)
     ap.add_argument("--lds-cells", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--lds-large-cells", type=int, default=0,
+                    help="batch: largest program (cells + 1) of the large LDS class "
+                         "(0 = what two items per CU allow, 1 = none)")
     ap.add_argument("--tape-race-ms", type=float, default=0.0,
                     help="serve: tape time up to which a compiled request races on the tape "
                          "(0 = the server's default, negative = never)")
@@ -271,7 +277,7 @@ def main():
     eng = t.Engine(0)
     if a.workload == "batch":
         out = run_batch(eng, a.programs, a.groups, a.passes, a.seed, not a.no_plain, a.lds_cells,
-                        a.waves_per_cu)
+                        a.waves_per_cu, a.lds_large_cells)
     elif a.workload == "life":
         out = run_life(eng, a.gens)
     elif a.workload == "serve":

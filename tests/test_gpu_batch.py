@@ -73,6 +73,27 @@ def test_batch_narrow_items(engine, groups, lds_cells):
             assert g == expected(p, s, groups, 96), (p.ncells, p.len, plain)
 
 
+@pytest.mark.parametrize("lds_large_cells", [0, 1])
+def test_batch_large_lds_class(engine, lds_large_cells):
+    # programs too large for lds_cells: the 300-cell one by default in the large LDS class (two
+    # items per CU), with lds_large_cells 1 in HBM like the larger ones; the same statistics either
+    # way, batched and one launch each
+    rng = np.random.default_rng(77)
+    progs = []
+    for k, cells in enumerate([300, 700, 1100]):
+        seed = t.t1_program_seed(5, cells, k)
+        progs.append((t.random_program(cells, int(rng.integers(400, 1500)), seed), seed))
+    print("batch lds_large_cells=%d" % lds_large_cells, flush=True)
+    for plain in (False, True):
+        got, info = engine.batch_run([p for p, _ in progs], [s for _, s in progs], groups=4,
+                                     max_passes=48, plain=plain, lds_cells=20,
+                                     lds_large_cells=lds_large_cells)
+        for (p, s), g in zip(progs, got):
+            assert g == expected(p, s, 4, 48), (p.ncells, p.len, plain)
+        if not plain:
+            assert (info["class_ops"][2] > 0) == (lds_large_cells == 0) and info["class_ops"][3] > 0
+
+
 @pytest.mark.parametrize("waves_per_cu", [1, 16])
 def test_batch_tortoise_placement(engine, waves_per_cu):
     # 128 groups = 32 items per program, 352 in all: at one wave per CU they outnumber the wave

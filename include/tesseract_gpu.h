@@ -257,6 +257,13 @@ typedef struct {
 } gpu_tape;
 int  gpu_enum_tape(const gpu_enum *en, unsigned max_slots, gpu_tape *out, char *err, size_t errlen);
 void gpu_tape_free(gpu_tape *t);
+/* The batch interpreter's records of a program (no device; no reference counterpart: the
+   reference compiles each program, this lists what gpu_batch_run runs instead): windows of `win`
+   slots (8 or 16), each 2 x win words, the load offsets (bit 31 = NAND) then the store offsets, in
+   bytes of cell x row with row = 4 x item_words; the sink is cell ncells. Writes up to cap words
+   to out and the total to *nwords; fails for a window size other than 8 or 16. */
+int  gpu_batch_records(const ts_program *p, unsigned item_words, unsigned win, uint32_t *out,
+                       size_t cap, size_t *nwords, char *err, size_t errlen);
 int  gpu_rtc_check(const char *source, const char *arch, double *seconds, size_t *code_bytes,
                    char *err, size_t errlen);
 

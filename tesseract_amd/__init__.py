@@ -212,6 +212,8 @@ def lib():
             "gpu_enum_compile": (i32, [vp, C.POINTER(GpuEnumStruct), C.POINTER(vp), C.c_char_p, sz]),
             "gpu_enum_exec": (i32, [vp, C.POINTER(EnumResultStruct), C.c_char_p, sz]),
             "gpu_enum_free": (None, [vp]),
+            "gpu_batch_records": (i32, [P, C.c_uint, C.c_uint, C.POINTER(C.c_uint32), sz,
+                                        C.POINTER(C.c_size_t), C.c_char_p, sz]),
             "gpu_batch_run": (i32, [vp, C.POINTER(GpuBatchItem), sz, C.POINTER(GpuBatchTune),
                                     C.POINTER(GpuBatchStats), C.POINTER(GpuBatchInfo), C.c_char_p, sz]),
             "gpu_server_open": (i32, [vp, C.POINTER(GpuServeTune), C.POINTER(vp), C.c_char_p, sz]),
@@ -633,6 +635,22 @@ def enum_tape(mode, a, b=None, spec=None, max_slots=0):
     finally:
         lib().gpu_tape_free(C.byref(t))
     return out
+
+
+def batch_records(prog, item_words=64, win=16):
+    """The batch interpreter's window records of a program (no device): an array of windows x 2 x
+    win (load offsets with bit 31 = NAND, then store offsets; bytes of cell x 4 x item_words)."""
+    import numpy as np
+    n = C.c_size_t()
+    e = _err()
+    if not lib().gpu_batch_records(prog.ptr, item_words, win, None, 0, C.byref(n), e, len(e)):
+        raise ValueError(e.value.decode())
+    out = np.zeros(n.value, np.uint32)
+    if not lib().gpu_batch_records(prog.ptr, item_words, win,
+                                   out.ctypes.data_as(C.POINTER(C.c_uint32)), n.value, C.byref(n),
+                                   e, len(e)):
+        raise ValueError(e.value.decode())
+    return out.reshape(-1, 2, win)
 
 
 def rtc_check(text, arch="gfx950"):

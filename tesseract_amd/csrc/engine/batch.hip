@@ -400,6 +400,22 @@ struct DevBuf {
         }                                                                                    \
     } while (0)
 
+extern "C" int gpu_batch_records(const ts_program *p, unsigned item_words, unsigned win,
+                                 uint32_t *out, size_t cap, size_t *nwords, char *err,
+                                 size_t errlen)
+{
+    if (!p || !nwords || (win != kWinSmall && win != kWinLarge) || !item_words ||
+        item_words > kLanes) {
+        say(err, errlen, "%s", "gpu_batch_records: a program, 8 or 16 slots and 1-64 item words");
+        return 0;
+    }
+    Encoded enc = encode(p, item_words * 4, win);
+    *nwords = enc.words.size();
+    if (out)
+        memcpy(out, enc.words.data(), std::min(cap, enc.words.size()) * sizeof(uint32_t));
+    return 1;
+}
+
 extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t n,
                              const gpu_batch_tune *tune_in, gpu_batch_stats *stats_out,
                              gpu_batch_info *info, char *err, size_t errlen)

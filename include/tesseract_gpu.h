@@ -146,6 +146,9 @@ void gpu_enum_free(gpu_enum_code *c);
 typedef struct {
     const ts_program *prog;
     uint64_t          seed;
+    uint64_t         *lam;  /* optional (NULL = not wanted): groups x 512 entries, machine
+                               512 g + k's cycle length, 0 if none within max_passes (the
+                               reference's t1_cycles lam_out per group, src/tester.c:96) */
 } gpu_batch_item;
 
 typedef struct {

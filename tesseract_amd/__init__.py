@@ -57,7 +57,8 @@ class GpuCodeInfo(C.Structure):
 
 
 class GpuBatchItem(C.Structure):
-    _fields_ = [("prog", C.POINTER(ProgramStruct)), ("seed", C.c_uint64)]
+    _fields_ = [("prog", C.POINTER(ProgramStruct)), ("seed", C.c_uint64),
+                ("lam", C.POINTER(C.c_uint64))]
 
 
 class GpuBatchStats(C.Structure):
@@ -723,13 +724,18 @@ class Engine:
         return float(lib().gpu_last_ms(self._e))
 
     def batch_run(self, progs, seeds, groups=2048, max_passes=64, plain=False, lds_cells=0,
-                  waves_per_cu=0, lds_large_cells=0):
-        """gpu_batch_run: per program a dict(hist, lam_sum, longest, passes); plus the run info."""
+                  waves_per_cu=0, lds_large_cells=0, lams=False):
+        """gpu_batch_run: per program a dict(hist, lam_sum, longest, passes), with lams also "lam"
+        (machine 512 g + k's cycle length, 0 = none found); plus the run info."""
+        import numpy as np
         n = len(progs)
         items = (GpuBatchItem * n)()
+        lam = [np.zeros(groups * 512, np.uint64) for _ in range(n)] if lams else None
         for i, (p, sd) in enumerate(zip(progs, seeds)):
             items[i].prog = p.ptr
             items[i].seed = sd
+            if lams:
+                items[i].lam = lam[i].ctypes.data_as(C.POINTER(C.c_uint64))
         tune = GpuBatchTune(groups, max_passes, int(plain), lds_cells, waves_per_cu, lds_large_cells)
         stats = (GpuBatchStats * n)()
         info = GpuBatchInfo()
@@ -738,6 +744,9 @@ class Engine:
             raise RuntimeError("gpu_batch_run: " + e.value.decode())
         out = [{"hist": list(st.hist), "lam_sum": st.lam_sum, "longest": st.longest,
                 "passes": st.passes} for st in stats]
+        if lams:
+            for o, a in zip(out, lam):
+                o["lam"] = a
         return out, {"ms": info.ms, "ops": info.ops, "launches": info.launches,
                      "class_ms": list(info.class_ms), "class_ops": list(info.class_ops)}
 

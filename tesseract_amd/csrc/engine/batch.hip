@@ -59,7 +59,10 @@ struct BatchProg {
     uint32_t sample_off; // into the sample list
     uint32_t first_item; // global index of the program's first work item
     uint64_t seed;       // t1_start stream seed
+    uint64_t lam_off;    // first of the program's per-machine cycle lengths, or kNoLam
 };
+
+constexpr uint64_t kNoLam = ~0ull;
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z)
 {
@@ -177,7 +180,8 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
                                                uint32_t *__restrict__ queue,
                                                uint32_t *__restrict__ scratch, uint32_t region_cells,
                                                uint64_t *__restrict__ stats, uint32_t max_passes,
-                                               uint32_t prog_words, uint32_t item_words)
+                                               uint32_t prog_words, uint32_t item_words,
+                                               uint64_t *__restrict__ lams)
 {
     extern __shared__ uint32_t lds[];
     const uint32_t lane = threadIdx.x;
@@ -268,8 +272,12 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
                         d |= hare.ld(c, lane) ^ tort.ld(c, lane);
                 }
                 const uint32_t found = active & ~d;
-                if (__any(found))
+                if (__any(found)) {
                     record(st, found, lam, lane);
+                    if (P.lam_off != kNoLam)  // machine 32 word + b of the program
+                        for (uint32_t m = found; m; m &= m - 1)
+                            lams[P.lam_off + (uint64_t)word * 32 + __builtin_ctz(m)] = lam;
+                }
                 active &= d;
                 const uint64_t searching = __ballot(active != 0);
                 for (int k = 0; k < 4; k++)
@@ -485,7 +493,7 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     std::vector<uint32_t> item_of[kClasses];
     uint32_t maxcells[kClasses] = {0, 0, 0, 0};
     uint64_t class_items[kClasses] = {0, 0, 0, 0};
-    uint64_t ops = 0;
+    uint64_t ops = 0, lam_total = 0;
     for (size_t i = 0; i < n; i++) {
         const int c = class_of(items[i].prog);
         class_items[c] += per_prog;
@@ -540,6 +548,11 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
         while ((samples.size() - bp.sample_off) % 8)  // the kernel compares 8 at a time
             samples.push_back(samples.back());
         bp.seed = items[i].seed;
+        bp.lam_off = kNoLam;
+        if (items[i].lam) {
+            bp.lam_off = lam_total;
+            lam_total += (uint64_t)prog_words * 32;
+        }
         bp.first_item = (uint32_t)item_of[cls].size();
         for (uint32_t k = 0; k < per_prog; k++)
             item_of[cls].push_back((uint32_t)i);
@@ -548,9 +561,9 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     DevBuf<BatchProg> d_progs;
     DevBuf<uint4> d_recs;
     DevBuf<uint32_t> d_samples, d_items[kClasses], d_queue, d_scratch;
-    DevBuf<uint64_t> d_stats;
+    DevBuf<uint64_t> d_stats, d_lams;
     bool ok = d_progs.alloc(n) && d_recs.alloc(recs.size() / 4) && d_samples.alloc(samples.size()) &&
-              d_queue.alloc(kClasses + n) && d_stats.alloc(n * kStats);
+              d_queue.alloc(kClasses + n) && d_stats.alloc(n * kStats) && d_lams.alloc(lam_total);
     for (int c = 0; c < kClasses; c++)
         ok = ok && d_items[c].alloc(item_of[c].size());
     if (!ok) {
@@ -576,6 +589,8 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
                                 hipMemcpyHostToDevice, s));
     BCHK(hipMemsetAsync(d_queue.p, 0, (kClasses + n) * 4, s));
     BCHK(hipMemsetAsync(d_stats.p, 0, n * kStats * 8, s));
+    if (lam_total)
+        BCHK(hipMemsetAsync(d_lams.p, 0, lam_total * 8, s));
 
     uint64_t launches = 0;
     auto launch_class = [&](int cls, hipStream_t st, uint32_t *queue, uint32_t first,
@@ -587,16 +602,16 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
             const size_t lds_bytes = 2 * (size_t)maxcells[cls] * row;
             hipLaunchKernelGGL((ts_batch<true, true>), dim3(grid), dim3(kLanes), lds_bytes, st,
                                d_progs.p, ip, first, count, d_recs.p, d_samples.p, queue, sc,
-                               maxcells[cls], d_stats.p, tune.max_passes, prog_words, item_words);
+                               maxcells[cls], d_stats.p, tune.max_passes, prog_words, item_words, d_lams.p);
         } else if (cls != kHbm) {
             const size_t lds_bytes = (size_t)maxcells[cls] * row;
             hipLaunchKernelGGL((ts_batch<true, false>), dim3(grid), dim3(kLanes), lds_bytes, st,
                                d_progs.p, ip, first, count, d_recs.p, d_samples.p, queue, sc,
-                               maxcells[cls], d_stats.p, tune.max_passes, prog_words, item_words);
+                               maxcells[cls], d_stats.p, tune.max_passes, prog_words, item_words, d_lams.p);
         } else {
             hipLaunchKernelGGL((ts_batch<false, false>), dim3(grid), dim3(kLanes), 0, st, d_progs.p,
                                ip, first, count, d_recs.p, d_samples.p, queue, sc, maxcells[cls],
-                               d_stats.p, tune.max_passes, prog_words, item_words);
+                               d_stats.p, tune.max_passes, prog_words, item_words, d_lams.p);
         }
         launches++;
         return hipGetLastError() == hipSuccess;
@@ -667,6 +682,10 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     e->last_ms = ms;
     std::vector<uint64_t> st(n * kStats);
     BCHK(hipMemcpy(st.data(), d_stats.p, st.size() * 8, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++)
+        if (items[i].lam)
+            BCHK(hipMemcpy(items[i].lam, d_lams.p + progs[i].lam_off,
+                           (size_t)prog_words * 32 * sizeof(uint64_t), hipMemcpyDeviceToHost));
     for (size_t i = 0; i < n; i++) {
         memcpy(stats_out[i].hist, &st[i * kStats], 64 * 8);
         stats_out[i].lam_sum = st[i * kStats + 64];

@@ -95,7 +95,7 @@ extern "C" int gpu_t1_run(gpu_engine *e, const ts_t1_config *cfg, ts_t1_size *ou
         }
         std::vector<gpu_batch_item> items(n);
         for (size_t j = 0; j < n; j++)
-            items[j] = gpu_batch_item{&r.progs[j], r.seeds[j]};
+            items[j] = gpu_batch_item{&r.progs[j], r.seeds[j], nullptr};
         std::vector<gpu_batch_stats> st(n);
         if (!gpu_batch_run(e, items.data(), n, &tune, st.data(), nullptr, err, errlen))
             return 0;

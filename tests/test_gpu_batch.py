@@ -73,6 +73,23 @@ def test_batch_narrow_items(engine, groups, lds_cells):
             assert g == expected(p, s, groups, 96), (p.ncells, p.len, plain)
 
 
+@pytest.mark.parametrize("groups", [1, 4])
+def test_batch_machine_cycle_lengths(engine, groups):
+    # the optional per-machine output: machine 512 g + k's cycle length is the oracle's Brent
+    # result for lane k of group g (the reference's t1_cycles lam_out), 0 where none was found;
+    # asking for it changes no statistic
+    progs = programs()
+    print("batch lams groups=%d" % groups, flush=True)
+    for plain in (False, True):
+        got, _ = engine.batch_run([p for p, _ in progs], [s for _, s in progs], groups=groups,
+                                  max_passes=64, plain=plain, lams=True)
+        for (p, s), g in zip(progs, got):
+            mem, rr = t.t1_start(p, s, groups)
+            want = np.concatenate([O.brent_lines(p, mem[k], rr[k], 64) for k in range(groups)])
+            assert np.array_equal(g.pop("lam"), want), (p.ncells, p.len, plain)
+            assert g == expected(p, s, groups, 64), (p.ncells, p.len, plain)
+
+
 @pytest.mark.parametrize("lds_large_cells", [0, 1])
 def test_batch_large_lds_class(engine, lds_large_cells):
     # programs too large for lds_cells: the 300-cell one by default in the large LDS class (two

@@ -149,6 +149,10 @@ typedef struct {
     uint64_t         *lam;  /* optional (NULL = not wanted): groups x 512 entries, machine
                                512 g + k's cycle length, 0 if none within max_passes (the
                                reference's t1_cycles lam_out per group, src/tester.c:96) */
+    const ts_line    *mem;  /* optional start state instead of t1_start(seed): groups x ncells
+                               lines (group-major, the reference's LaneMachine.mem) and rr, groups
+                               lines; cells ZERO and ONE keep their constants */
+    const ts_line    *rr;
 } gpu_batch_item;
 
 typedef struct {

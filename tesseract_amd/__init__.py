@@ -58,7 +58,7 @@ class GpuCodeInfo(C.Structure):
 
 class GpuBatchItem(C.Structure):
     _fields_ = [("prog", C.POINTER(ProgramStruct)), ("seed", C.c_uint64),
-                ("lam", C.POINTER(C.c_uint64))]
+                ("lam", C.POINTER(C.c_uint64)), ("mem", C.c_void_p), ("rr", C.c_void_p)]
 
 
 class GpuBatchStats(C.Structure):
@@ -724,16 +724,25 @@ class Engine:
         return float(lib().gpu_last_ms(self._e))
 
     def batch_run(self, progs, seeds, groups=2048, max_passes=64, plain=False, lds_cells=0,
-                  waves_per_cu=0, lds_large_cells=0, lams=False):
+                  waves_per_cu=0, lds_large_cells=0, lams=False, starts=None):
         """gpu_batch_run: per program a dict(hist, lam_sum, longest, passes), with lams also "lam"
-        (machine 512 g + k's cycle length, 0 = none found); plus the run info."""
+        (machine 512 g + k's cycle length, 0 = none found); plus the run info. starts: per program
+        None (t1_start of its seed) or (mem, rr) as uint64 arrays (groups, ncells, 8), (groups, 8)."""
         import numpy as np
         n = len(progs)
         items = (GpuBatchItem * n)()
         lam = [np.zeros(groups * 512, np.uint64) for _ in range(n)] if lams else None
+        keep = []
         for i, (p, sd) in enumerate(zip(progs, seeds)):
             items[i].prog = p.ptr
             items[i].seed = sd
+            if starts is not None and starts[i] is not None:
+                m = np.ascontiguousarray(starts[i][0], dtype=np.uint64)
+                r = np.ascontiguousarray(starts[i][1], dtype=np.uint64)
+                if m.shape != (groups, p.ncells, 8) or r.shape != (groups, 8):
+                    raise ValueError("a start state is (groups, ncells, 8) and (groups, 8) words")
+                keep += [m, r]
+                items[i].mem, items[i].rr = m.ctypes.data, r.ctypes.data
             if lams:
                 items[i].lam = lam[i].ctypes.data_as(C.POINTER(C.c_uint64))
         tune = GpuBatchTune(groups, max_passes, int(plain), lds_cells, waves_per_cu, lds_large_cells)

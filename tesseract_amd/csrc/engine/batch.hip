@@ -60,6 +60,7 @@ struct BatchProg {
     uint32_t first_item; // global index of the program's first work item
     uint64_t seed;       // t1_start stream seed
     uint64_t lam_off;    // first of the program's per-machine cycle lengths, or kNoLam
+    uint64_t state_off;  // first u64 of the program's start state, or kNoLam (t1_start stream)
 };
 
 constexpr uint64_t kNoLam = ~0ull;
@@ -181,7 +182,8 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
                                                uint32_t *__restrict__ scratch, uint32_t region_cells,
                                                uint64_t *__restrict__ stats, uint32_t max_passes,
                                                uint32_t prog_words, uint32_t item_words,
-                                               uint64_t *__restrict__ lams)
+                                               uint64_t *__restrict__ lams,
+                                               const uint64_t *__restrict__ states)
 {
     extern __shared__ uint32_t lds[];
     const uint32_t lane = threadIdx.x;
@@ -214,9 +216,12 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
             const uint64_t g = word >> 4;
             const uint32_t w64 = (word & 15) >> 1, hi = word & 1;
             const uint64_t s0 = P.seed ^ 0x5DEECE66Dull, per_group = (uint64_t)(ncells - 1) * 8;
+            // a caller's start state: group g's lines of cells 0..ncells-1, then its RR line
+            const bool given = P.state_off != kNoLam && word < prog_words;
+            const uint64_t *gs = states + (given ? P.state_off + g * (ncells + 1) * 8 + w64 : 0);
             for (uint32_t c = 2; c < ncells; c++) {
                 uint64_t k = g * per_group + (uint64_t)(c - 2) * 8 + w64;
-                uint64_t v = mix64(s0 + (k + 1) * 0x9E3779B97F4A7C15ull);
+                uint64_t v = given ? gs[(size_t)c * 8] : mix64(s0 + (k + 1) * 0x9E3779B97F4A7C15ull);
                 hare.st(c, lane, hi ? (uint32_t)(v >> 32) : (uint32_t)v);
             }
             hare.st(0, lane, 0u);
@@ -224,7 +229,7 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
             uint32_t rr;
             {
                 uint64_t k = g * per_group + (uint64_t)(ncells - 2) * 8 + w64;
-                uint64_t v = mix64(s0 + (k + 1) * 0x9E3779B97F4A7C15ull);
+                uint64_t v = given ? gs[(size_t)ncells * 8] : mix64(s0 + (k + 1) * 0x9E3779B97F4A7C15ull);
                 rr = hi ? (uint32_t)(v >> 32) : (uint32_t)v;
             }
 
@@ -490,6 +495,7 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     const uint32_t per_prog = (prog_words + item_words - 1) / item_words;  // items per program
     std::vector<BatchProg> progs(n);
     std::vector<uint32_t> recs, samples;
+    std::vector<uint64_t> states;  // callers' start states
     std::vector<uint32_t> item_of[kClasses];
     uint32_t maxcells[kClasses] = {0, 0, 0, 0};
     uint64_t class_items[kClasses] = {0, 0, 0, 0};
@@ -549,6 +555,16 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
             samples.push_back(samples.back());
         bp.seed = items[i].seed;
         bp.lam_off = kNoLam;
+        bp.state_off = kNoLam;
+        if (items[i].mem && items[i].rr) {  // groups x (ncells + 1) lines: the cells, then RR
+            bp.state_off = states.size();
+            for (uint32_t g = 0; g < tune.groups; g++) {
+                const ts_line *m = items[i].mem + (size_t)g * p->ncells;
+                for (uint32_t c = 0; c < p->ncells; c++)
+                    states.insert(states.end(), m[c].w, m[c].w + 8);
+                states.insert(states.end(), items[i].rr[g].w, items[i].rr[g].w + 8);
+            }
+        }
         if (items[i].lam) {
             bp.lam_off = lam_total;
             lam_total += (uint64_t)prog_words * 32;
@@ -561,9 +577,10 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     DevBuf<BatchProg> d_progs;
     DevBuf<uint4> d_recs;
     DevBuf<uint32_t> d_samples, d_items[kClasses], d_queue, d_scratch;
-    DevBuf<uint64_t> d_stats, d_lams;
+    DevBuf<uint64_t> d_stats, d_lams, d_states;
     bool ok = d_progs.alloc(n) && d_recs.alloc(recs.size() / 4) && d_samples.alloc(samples.size()) &&
-              d_queue.alloc(kClasses + n) && d_stats.alloc(n * kStats) && d_lams.alloc(lam_total);
+              d_queue.alloc(kClasses + n) && d_stats.alloc(n * kStats) && d_lams.alloc(lam_total) &&
+              d_states.alloc(states.size());
     for (int c = 0; c < kClasses; c++)
         ok = ok && d_items[c].alloc(item_of[c].size());
     if (!ok) {
@@ -583,6 +600,8 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     BCHK(hipMemcpyAsync(d_progs.p, progs.data(), n * sizeof(BatchProg), hipMemcpyHostToDevice, s));
     BCHK(hipMemcpyAsync(d_recs.p, recs.data(), recs.size() * 4, hipMemcpyHostToDevice, s));
     BCHK(hipMemcpyAsync(d_samples.p, samples.data(), samples.size() * 4, hipMemcpyHostToDevice, s));
+    if (!states.empty())
+        BCHK(hipMemcpyAsync(d_states.p, states.data(), states.size() * 8, hipMemcpyHostToDevice, s));
     for (int c = 0; c < kClasses; c++)
         if (!item_of[c].empty())
             BCHK(hipMemcpyAsync(d_items[c].p, item_of[c].data(), item_of[c].size() * 4,
@@ -602,16 +621,16 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
             const size_t lds_bytes = 2 * (size_t)maxcells[cls] * row;
             hipLaunchKernelGGL((ts_batch<true, true>), dim3(grid), dim3(kLanes), lds_bytes, st,
                                d_progs.p, ip, first, count, d_recs.p, d_samples.p, queue, sc,
-                               maxcells[cls], d_stats.p, tune.max_passes, prog_words, item_words, d_lams.p);
+                               maxcells[cls], d_stats.p, tune.max_passes, prog_words, item_words, d_lams.p, d_states.p);
         } else if (cls != kHbm) {
             const size_t lds_bytes = (size_t)maxcells[cls] * row;
             hipLaunchKernelGGL((ts_batch<true, false>), dim3(grid), dim3(kLanes), lds_bytes, st,
                                d_progs.p, ip, first, count, d_recs.p, d_samples.p, queue, sc,
-                               maxcells[cls], d_stats.p, tune.max_passes, prog_words, item_words, d_lams.p);
+                               maxcells[cls], d_stats.p, tune.max_passes, prog_words, item_words, d_lams.p, d_states.p);
         } else {
             hipLaunchKernelGGL((ts_batch<false, false>), dim3(grid), dim3(kLanes), 0, st, d_progs.p,
                                ip, first, count, d_recs.p, d_samples.p, queue, sc, maxcells[cls],
-                               d_stats.p, tune.max_passes, prog_words, item_words, d_lams.p);
+                               d_stats.p, tune.max_passes, prog_words, item_words, d_lams.p, d_states.p);
         }
         launches++;
         return hipGetLastError() == hipSuccess;

@@ -90,6 +90,30 @@ def test_batch_machine_cycle_lengths(engine, groups):
             assert g == expected(p, s, groups, 64), (p.ncells, p.len, plain)
 
 
+@pytest.mark.parametrize("groups", [1, 3])
+def test_batch_given_start_states(engine, groups):
+    # a caller's start state (the reference's lines) instead of t1_start: the oracle's Brent from
+    # the same state, per machine and per program
+    progs = programs()
+    starts = [O.random_lines(p, groups, 500 + k) for k, (p, _) in enumerate(progs)]
+    print("batch given states groups=%d" % groups, flush=True)
+    for plain in (False, True):
+        got, _ = engine.batch_run([p for p, _ in progs], [s for _, s in progs], groups=groups,
+                                  max_passes=64, plain=plain, lams=True, starts=starts)
+        for (p, _), (mem, rr), g in zip(progs, starts, got):
+            lams, hist, total, longest, passes = [], [0] * 64, 0, 0, 0
+            for k in range(groups):
+                lam, ran = O.brent_lines(p, mem[k], rr[k], 64, with_passes=True)
+                lams.append(lam)
+                passes += ran
+                for x in (int(v) for v in lam):
+                    hist[63 if x == 0 else bucket(x)] += 1
+                    total += x
+                    longest = max(longest, x)
+            assert np.array_equal(g.pop("lam"), np.concatenate(lams)), (p.ncells, plain)
+            assert g == {"hist": hist, "lam_sum": total, "longest": longest, "passes": passes}
+
+
 @pytest.mark.parametrize("lds_large_cells", [0, 1])
 def test_batch_large_lds_class(engine, lds_large_cells):
     # programs too large for lds_cells: the 300-cell one by default in the large LDS class (two

@@ -156,9 +156,9 @@ void enum_outputs(const ts_program *a, const ts_program *b, const int *in_map, c
 // ---- tape: an enumeration as data for the precompiled interpreter kernel ----------------------
 //
 // The hit word of an equiv or search pass (ticks == 1) as a list of AIG AND nodes in groups of up
-// to kGroup mutually independent nodes, every value in a slot of a per-lane file in LDS. Slots are
-// reused once their last reader's group has read them; all reads of a group happen before its
-// writes, so a group may overwrite the slots it frees. Node order interleaves the two programs of
+// to kGroup mutually independent nodes, every value in a slot of a per-lane file in LDS. A slot is
+// reused from the group after its last reader's group on, so a group's writes never touch a slot
+// any entry of the same group reads (its entries may run in any order). Node order interleaves the two programs of
 // an equiv by relative position, so the live values stay those of a window of each program.
 // Every group has kGroup entries: a short group is padded with entries that read slot 0 and write
 // the last slot (the sink, read by nobody), so the kernel runs each group without branches.

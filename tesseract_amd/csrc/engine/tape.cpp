@@ -116,12 +116,15 @@ bool build_tape(const ts_program *a, const ts_program *b, const int *in_map, con
             group.push_back(ready.top().second);
             ready.pop();
         }
-        // reads first: operands read for the last time free their slots for this group's writes
+        // operands read for the last time free their slots for the next group's writes: the
+        // kernel's two waves split a group's entries, so a write may land while the other wave
+        // still reads this group's operands
+        std::vector<uint32_t> freed;
         for (uint32_t v : group) {
             const Node &nd = g.nodes[v];
             for (uint32_t u : {nd.a >> 1, nd.b >> 1})
                 if (--uses[u] == 0 && u != 0)
-                    free_slots.push_back((uint32_t)slot[u]);
+                    freed.push_back((uint32_t)slot[u]);
         }
         const size_t base = t.ops.size();
         t.ops.resize(base + Tape::kWords, 0);
@@ -141,6 +144,7 @@ bool build_tape(const ts_program *a, const ts_program *b, const int *in_map, con
                 padding.push_back(base + 2 * Tape::kGroup + j);
             }
         }
+        free_slots.insert(free_slots.end(), freed.begin(), freed.end());
         t.count.push_back((uint32_t)group.size());
         scheduled += group.size();
         for (uint32_t v : group) {

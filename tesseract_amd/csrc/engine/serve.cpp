@@ -79,9 +79,9 @@ struct Module {
 };
 
 // AND nodes per second of the tape interpreter with a file of about 230 slots (two workgroups of
-// two waves per CU): measured 4.1e13 on MI355X for a pair of 3,366 nodes; the estimate errs on
+// four waves per CU): measured 5.0e13 on MI355X for a pair of 3,366 nodes; the estimate errs on
 // the slow side.
-constexpr double kTapeAndPerSecond = 3.5e13;
+constexpr double kTapeAndPerSecond = 4.0e13;
 // default estimated tape time below which a request worth a compile also starts on the tape
 // meanwhile: about a compile's time, so that the tape may finish first; a larger tape share is
 // mostly spent work (a compiled kernel does the same words about 80 times faster) that other

@@ -117,8 +117,8 @@ bool build_tape(const ts_program *a, const ts_program *b, const int *in_map, con
             ready.pop();
         }
         // operands read for the last time free their slots for the next group's writes: the
-        // kernel's two waves split a group's entries, so a write may land while the other wave
-        // still reads this group's operands
+        // kernel's waves split a group's entries, so a write may land while another wave still
+        // reads this group's operands
         std::vector<uint32_t> freed;
         for (uint32_t v : group) {
             const Node &nd = g.nodes[v];

@@ -2,13 +2,13 @@
 // tape (groups of independent AND nodes over a per-lane file of slots in LDS). It is the no-compile
 // tier of the engine: a request starts as soon as its tape is built on the host.
 //
-// Layout: a workgroup is two waves over the same 64 words: lane l of either wave owns the 32-bit
+// Layout: a workgroup is four waves over the same 64 words: lane l of any of them owns the 32-bit
 // word at byte slot * 256 + 4 l of the workgroup's LDS file, so every slot access of a wave is one
-// conflict-free 256-byte row, and the two waves split each group's 16 entries (8 each) with one
-// barrier per group: twice the waves per CU that one wave per file allows. A group's entries
-// are described by 80 words (the tape format in lower.hpp), each wave loading its half one group
-// ahead. Each lane enumerates one word (32 assignments) at a time exactly like the compiled
-// ts_enum kernel — same word order, same share rule, same valid mask, same hit list of
+// conflict-free 256-byte row, and the waves split each group's 16 entries (4 each) with one
+// barrier per group: four times the waves per CU that one wave per file allows. A group's
+// entries are described by 80 words (the tape format in lower.hpp), each wave loading its share
+// one group ahead. Each lane enumerates one word (32 assignments) at a time exactly like the
+// compiled ts_enum kernel — same word order, same share rule, same valid mask, same hit list of
 // (word, mask) pairs — so the host's collection of hits is shared by both tiers.
 #include <hip/hip_runtime.h>
 
@@ -30,60 +30,62 @@ __device__ inline uint32_t lane_pattern(uint32_t j)
                    : 0xFFFF0000u;
 }
 
-constexpr unsigned kHalf = kGroup / 2;  // entries per wave
+constexpr unsigned kWaves = 4;               // waves per workgroup, over one slot file
+constexpr unsigned kShare = kGroup / kWaves;  // entries per wave
+constexpr unsigned kQ = kShare / 4;           // 16-byte words per field and wave
 
-// A wave's half of a group's descriptors: fields a, b, d, ma, mb (16 words each), entries
-// 8h .. 8h + 7, as 10 16-byte vector loads. `zero` is 0 at run time; adding (lane & zero) makes
+// A wave's share of a group's descriptors: fields a, b, d, ma, mb (16 words each), entries
+// kShare h .. kShare h + kShare - 1, as 5 kQ 16-byte vector loads. `zero` is 0 at run time; adding (lane & zero) makes
 // the address per-lane for the compiler, so the words arrive through vector memory (counted by
 // vmcnt) rather than scalar loads, whose counter is shared with LDS reads and would make every
 // LDS wait also wait for the next group's descriptors.
-__device__ inline void load_half(uint4 (&d)[10], const uint4 *__restrict__ ops, uint32_t g,
-                                 uint32_t half, uint32_t lane_zero)
+__device__ inline void load_share(uint4 (&d)[5 * kQ], const uint4 *__restrict__ ops, uint32_t g,
+                                 uint32_t wave, uint32_t lane_zero)
 {
-    const uint4 *p = ops + (size_t)g * 20 + half * 2 + lane_zero;
+    const uint4 *p = ops + (size_t)g * 20 + wave * kQ + lane_zero;
 #pragma unroll
-    for (int f = 0; f < 5; f++) {
-        d[2 * f] = p[4 * f];
-        d[2 * f + 1] = p[4 * f + 1];
-    }
+    for (unsigned f = 0; f < 5; f++)
+#pragma unroll
+        for (unsigned q = 0; q < kQ; q++)
+            d[kQ * f + q] = p[4 * f + q];
 }
 
-// word j (0-7) of field f of the half
-__device__ inline uint32_t word(const uint4 (&d)[10], unsigned f, unsigned j)
+// word j of field f of the wave's share
+__device__ inline uint32_t word(const uint4 (&d)[5 * kQ], unsigned f, unsigned j)
 {
-    const uint4 &q = d[2 * f + (j >> 2)];
+    const uint4 &q = d[kQ * f + (j >> 2)];
     return (j & 3) == 0 ? q.x : (j & 3) == 1 ? q.y : (j & 3) == 2 ? q.z : q.w;
 }
 
-// a wave's 8 entries of one group: every read, then every write
-__device__ inline void run_half(const uint4 (&d)[10], char *my)
+// a wave's entries of one group: every read, then every write
+__device__ inline void run_share(const uint4 (&d)[5 * kQ], char *my)
 {
-    uint32_t v[kHalf];
+    uint32_t v[kShare];
 #pragma unroll
-    for (unsigned j = 0; j < kHalf; j++) {
+    for (unsigned j = 0; j < kShare; j++) {
         const uint32_t x = *(const uint32_t *)(my + word(d, 0, j)) ^ word(d, 3, j);
         const uint32_t y = *(const uint32_t *)(my + word(d, 1, j)) ^ word(d, 4, j);
         v[j] = x & y;
     }
 #pragma unroll
-    for (unsigned j = 0; j < kHalf; j++)
+    for (unsigned j = 0; j < kShare; j++)
         *(uint32_t *)(my + word(d, 2, j)) = v[j];
 }
 
-__global__ void __launch_bounds__(128)
+__global__ void __launch_bounds__(64 * kWaves)
     ts_enum_tape(const uint4 *__restrict__ ops, uint32_t ngroups, const uint32_t *__restrict__ leaves,
                  uint32_t nleaves, uint32_t root, uint32_t zero, uint64_t *__restrict__ out,
                  uint64_t *__restrict__ list, uint64_t cap, uint64_t i0, uint64_t i1, uint64_t space,
                  uint32_t parts, uint32_t part, uint32_t nshare)
 {
     extern __shared__ uint32_t file[];
-    const uint32_t lane = threadIdx.x & 63u, half = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     char *const my = (char *)file + lane * 4u;
     const uint32_t lane_zero = lane & zero;
-    if (half == 0)
+    if (wave == 0)
         *(uint32_t *)my = 0u;  // slot 0: the constant
-    // Both waves run the same rounds (the bounds depend on the workgroup only; a lane past i1
-    // computes a word with no valid assignment), so every barrier is reached by both.
+    // The waves run the same rounds (the bounds depend on the workgroup only; a lane past i1
+    // computes a word with no valid assignment), so every barrier is reached by all of them.
     for (uint64_t base = i0 + (uint64_t)blockIdx.x * 64u; base < i1;
          base += (uint64_t)gridDim.x * 64u) {
         const uint64_t i = base + lane;
@@ -96,7 +98,7 @@ __global__ void __launch_bounds__(128)
         const uint32_t valid = i >= i1 || xb >= space    ? 0u
                                : space - xb >= 32ull     ? 0xffffffffu
                                                          : ((1u << (uint32_t)(space - xb)) - 1u);
-        if (half == 0)
+        if (wave == 0)
             for (uint32_t k = 0; k < nleaves; k++) {
                 const uint32_t lf = leaves[k], j = lf >> 24;
                 *(uint32_t *)(my + (lf & 0xffffffu)) =
@@ -104,22 +106,22 @@ __global__ void __launch_bounds__(128)
             }
         __syncthreads();
         // two descriptor buffers: the next group's loads are in flight while this one runs
-        uint4 da[10], db[10];
+        uint4 da[5 * kQ], db[5 * kQ];
         if (ngroups)
-            load_half(da, ops, 0, half, lane_zero);
+            load_share(da, ops, 0, wave, lane_zero);
         for (uint32_t g = 0; g < ngroups; g += 2) {
             if (g + 1 < ngroups)
-                load_half(db, ops, g + 1, half, lane_zero);
-            run_half(da, my);
+                load_share(db, ops, g + 1, wave, lane_zero);
+            run_share(da, my);
             __syncthreads();
             if (g + 1 < ngroups) {
                 if (g + 2 < ngroups)
-                    load_half(da, ops, g + 2, half, lane_zero);
-                run_half(db, my);
+                    load_share(da, ops, g + 2, wave, lane_zero);
+                run_share(db, my);
                 __syncthreads();
             }
         }
-        if (half == 0) {
+        if (wave == 0) {
             const uint32_t hit =
                 (*(const uint32_t *)(my + (root & 0x7fffffffu)) ^ (0u - (root >> 31))) & valid;
             if (hit) {
@@ -155,7 +157,7 @@ bool tape_prepare(int device, const TapeDev &t, unsigned *blocks_per_cu)
                             max_lds) != hipSuccess)
         return false;
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, ts_enum_tape, 128, lds) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, ts_enum_tape, 64 * kWaves, lds) !=
             hipSuccess ||
         blocks < 1)
         return false;
@@ -167,7 +169,7 @@ bool tape_launch(const TapeDev &t, hipStream_t stream, unsigned grid, uint64_t *
                  uint64_t *list, uint64_t cap, uint64_t i0, uint64_t i1, uint64_t space,
                  uint32_t parts, uint32_t part, uint32_t nshare)
 {
-    ts_enum_tape<<<grid, 128, tape_lds_bytes(t), stream>>>((const uint4 *)t.ops, t.ngroups,
+    ts_enum_tape<<<grid, 64 * kWaves, tape_lds_bytes(t), stream>>>((const uint4 *)t.ops, t.ngroups,
                                                            t.leaves, t.nleaves, t.root, 0u, out,
                                                            list, cap, i0, i1, space, parts, part,
                                                            nshare);

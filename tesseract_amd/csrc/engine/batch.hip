@@ -35,6 +35,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <thread>
 #include <vector>
 
 #include "engine_internal.hpp"
@@ -389,6 +390,27 @@ Encoded pick_encoding(const ts_program *p, uint32_t row, double wait)
     return t_large < t_small ? large : small;
 }
 
+// fn(i) for i in [0, n) on up to 16 host threads
+template <class F>
+void parallel_for(size_t n, F fn)
+{
+    const size_t threads = std::min<size_t>({16, std::max(1u, std::thread::hardware_concurrency()),
+                                             (n + 63) / 64});
+    if (threads <= 1) {
+        for (size_t i = 0; i < n; i++)
+            fn(i);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (size_t t = 0; t < threads; t++)
+        pool.emplace_back([&, t] {
+            for (size_t i = t; i < n; i += threads)
+                fn(i);
+        });
+    for (std::thread &th : pool)
+        th.join();
+}
+
 void say(char *err, size_t errlen, const char *fmt, const char *what)
 {
     if (err && errlen)
@@ -536,10 +558,14 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     double wait[kClasses];
     for (int c = 0; c < kClasses; c++)
         wait[c] = c == kHbm || per_cu[c] <= 2 || class_items[c] <= 2ull * e->cus ? 150.0 : 20.0;
+    std::vector<Encoded> encs(n);  // the records of each program, encoded on host threads
+    parallel_for(n, [&](size_t i) {
+        encs[i] = pick_encoding(items[i].prog, row, wait[class_of(items[i].prog)]);
+    });
     for (size_t i = 0; i < n; i++) {
         const ts_program *p = items[i].prog;
         const int cls = class_of(p);
-        Encoded enc = pick_encoding(p, row, wait[cls]);
+        Encoded enc = std::move(encs[i]);
         BatchProg &bp = progs[i];
         bp.rec_off = (uint32_t)(recs.size() / 4);
         bp.nrec = enc.nwin;
@@ -576,7 +602,7 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     }
     DevBuf<BatchProg> d_progs;
     DevBuf<uint4> d_recs;
-    DevBuf<uint32_t> d_samples, d_items[kClasses], d_queue, d_scratch;
+    DevBuf<uint32_t> d_samples, d_items[kClasses], d_queue;
     DevBuf<uint64_t> d_stats, d_lams, d_states;
     bool ok = d_progs.alloc(n) && d_recs.alloc(recs.size() / 4) && d_samples.alloc(samples.size()) &&
               d_queue.alloc(kClasses + n) && d_stats.alloc(n * kStats) && d_lams.alloc(lam_total) &&
@@ -587,10 +613,20 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
         say(err, errlen, "%s", "out of device memory for the batch");
         return 0;
     }
-    if (!d_scratch.alloc(scratch_off[kClasses])) {
-        say(err, errlen, "%s", "out of device memory for the machines' HBM regions");
-        return 0;
+    const size_t scratch_bytes = std::max<size_t>(scratch_off[kClasses], 1) * sizeof(uint32_t);
+    if (e->batch_scratch_bytes < scratch_bytes) {
+        if (e->batch_scratch)
+            (void)hipFree(e->batch_scratch);
+        e->batch_scratch = nullptr;
+        e->batch_scratch_bytes = 0;
+        if (hipMalloc(&e->batch_scratch, scratch_bytes) != hipSuccess) {
+            e->batch_scratch = nullptr;
+            say(err, errlen, "%s", "out of device memory for the machines' HBM regions");
+            return 0;
+        }
+        e->batch_scratch_bytes = scratch_bytes;
     }
+    uint32_t *const scratch = (uint32_t *)e->batch_scratch;
     hipStream_t s = e->stream;
     // LDS beyond the default 64 KiB per workgroup for the LDS classes
     BCHK(hipFuncSetAttribute((const void *)ts_batch<true, false>,
@@ -616,7 +652,7 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
                             uint32_t count) -> int {
         const uint32_t *ip = d_items[cls].p;
         unsigned grid = std::min<unsigned>((unsigned)e->cus * per_cu[cls], count);
-        uint32_t *sc = d_scratch.p + scratch_off[cls];
+        uint32_t *sc = scratch + scratch_off[cls];
         if (cls != kHbm && tort_lds[cls]) {
             const size_t lds_bytes = 2 * (size_t)maxcells[cls] * row;
             hipLaunchKernelGGL((ts_batch<true, true>), dim3(grid), dim3(kLanes), lds_bytes, st,

@@ -200,6 +200,8 @@ extern "C" void gpu_engine_close(gpu_engine *e)
     if (!e)
         return;
     (void)hipSetDevice(e->device);
+    if (e->batch_scratch)
+        (void)hipFree(e->batch_scratch);
     if (e->ev0)
         (void)hipEventDestroy(e->ev0);
     if (e->ev1)

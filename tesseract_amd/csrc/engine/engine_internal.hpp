@@ -13,6 +13,10 @@ struct gpu_engine {
     size_t      lds_per_block = 160 * 1024;  // bytes of LDS one workgroup may allocate
     std::string arch, desc;
     double      last_ms = 0;
+    // the batch interpreter's per-wave HBM regions, kept between gpu_batch_run calls (freeing a
+    // large buffer waits for the device) and released by gpu_engine_close
+    void       *batch_scratch = nullptr;
+    size_t      batch_scratch_bytes = 0;
 };
 
 #include <cstdint>

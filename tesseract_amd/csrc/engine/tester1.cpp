@@ -3,6 +3,7 @@
 // summed per size in the reference's program order.
 #include <algorithm>
 #include <chrono>
+#include <thread>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -85,14 +86,27 @@ extern "C" int gpu_t1_run(gpu_engine *e, const ts_t1_config *cfg, ts_t1_size *ou
         r.progs.resize(n);
         r.seeds.resize(n);
         r.size_of.resize(n);
-        for (size_t j = 0; j < n; j++) {  // sizes take turns (src/tester.c:190-192)
+        // sizes take turns (src/tester.c:190-192); the programs are generated on host threads
+        const size_t threads = std::min<size_t>(
+            {16, std::max(1u, std::thread::hardware_concurrency()), (n + 63) / 64});
+        auto make = [&](size_t j) {
             const int si = (int)(j % (size_t)cfg->nsizes);
             const uint64_t k = k0 + j / (size_t)cfg->nsizes;
             const uint32_t cells = cfg->cells[si];
             r.seeds[j] = ts_t1_program_seed(cfg->seed, cells, k);
             ts_random_program(&r.progs[j], cells, out[si].len, r.seeds[j]);
             r.size_of[j] = si;
-        }
+        };
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < threads; t++)
+            pool.emplace_back([&, t] {
+                for (size_t j = t; j < n; j += threads)
+                    make(j);
+            });
+        for (size_t j = 0; j < n; j += std::max<size_t>(threads, 1))
+            make(j);
+        for (std::thread &th : pool)
+            th.join();
         std::vector<gpu_batch_item> items(n);
         for (size_t j = 0; j < n; j++)
             items[j] = gpu_batch_item{&r.progs[j], r.seeds[j], nullptr};

@@ -191,7 +191,10 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
     const uint32_t row = item_words * 4;
     // the hare (every instruction touches it) in LDS or HBM; the tortoise (Brent's compare and
     // copy only) in this wave's HBM region or after the hare in LDS
-    uint8_t *hbm = (uint8_t *)(scratch + (size_t)blockIdx.x * 2 * region_cells * item_words);
+    // a wave's HBM region: hare and tortoise for the HBM class, the tortoise alone for an LDS
+    // class that keeps it in HBM, nothing when both states are in LDS
+    const size_t stride = (size_t)(LDS ? (TORT_LDS ? 0 : 1) : 2) * region_cells * item_words;
+    uint8_t *hbm = (uint8_t *)(scratch + (size_t)blockIdx.x * stride);
     for (;;) {
         uint32_t item = 0;
         if (lane == 0)
@@ -205,7 +208,9 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
             const BatchProg P = progs[pi];
             const uint32_t ncells = P.ncells;
             Cells hare{LDS ? (uint8_t *)lds : hbm, row};
-            Cells tort{(TORT_LDS ? (uint8_t *)lds : hbm) + (size_t)region_cells * row, row};
+            Cells tort{TORT_LDS ? (uint8_t *)lds + (size_t)region_cells * row
+                                : LDS ? hbm : hbm + (size_t)region_cells * row,
+                       row};
             const uint4 *rec = recs + P.rec_off;
             uint64_t *st = stats + (size_t)pi * kStats;
 
@@ -547,7 +552,7 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
                 per_cu[c] = (unsigned)both;
             }
         }
-        const size_t region = (size_t)2 * maxcells[c] * row;
+        const size_t region = (size_t)(c == kHbm ? 2 : tort_lds[c] ? 0 : 1) * maxcells[c] * row;
         if (region)
             per_cu[c] = (unsigned)std::max<size_t>(
                 1, std::min<size_t>(per_cu[c], kRegionBudget / ((size_t)e->cus * region)));

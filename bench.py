@@ -145,7 +145,9 @@ def run_tester1(args):
     eng = t.Engine(0)
     cells = [int(c) for c in args.cells.split(",")]
     cfg = t.t1_config(cells, args.ratio, args.programs, args.max_passes, args.seed)
-    eng.t1_run(t.t1_config(cells, args.ratio, 1, 1 << 10, args.seed))  # warm-up
+    # warm-up, untimed: the same programs with cycle searches cut at 2^10 passes (the device
+    # buffers and kernels of the timed run, little of its work)
+    eng.t1_run(t.t1_config(cells, args.ratio, args.programs, 1 << 10, args.seed))
     got, gsec = eng.t1_run(cfg)
     ops = sum(d["passes"] * d["len"] * 512 for d in got)  # src/tester.c:302-305
     out = {"workload": "tester1", "cells": cells, "ratio": args.ratio,

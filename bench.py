@@ -6,9 +6,15 @@ the program evaluated 32 machines per 32-bit word, every output compared with a 
 counted (must be 0). The kernel is compiled once through hipRTC before the timed region.
 
 value = 1-bit NAND/STO ops per second over the timed steps (ops = instructions x assignments, the
-reference's convention, src/enum.c:164). The roofline is the 32-bit integer VALU:
-256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz x 32 bits = 1.258e15 1-bit ops/s (the ratio can pass 1:
-v_bitop3_b32 evaluates several NANDs at once).
+reference's convention, src/enum.c:164). The roofline is the 32-bit integer VALU: a wave64
+bitwise instruction issues in 2 cycles on a SIMD (the rate behind the published 157.3 TFLOP/s
+FP32 vector figure), so 256 CUs x 4 SIMDs x 32 lanes x 2.4 GHz = 7.864e13 32-bit lane-ops/s,
+x 32 bits = 2.516e15 1-bit slots/s. roofline.achieved is the kernel's executed VALU lane-ops
+per second: its VALU instructions per launch (SQ_INSTS_VALU, from the committed counter profile
+of this exact kernel source) x 64 lanes / the launch's live HIP-event time; frac = achieved /
+peak is the VALU issue utilisation. The algorithmic 1-bit ops per second are reported beside it
+(algorithmic_1bit_ops_per_s): they exceed the 1-bit slot peak because one v_bitop3_b32 LUT
+evaluates several NANDs and a STO costs no instruction.
 
 The CPU baseline is the reference's own AVX2 JIT engine (enum_run built from its sources into
 oracle/_ref by oracle/Makefile) on a bounded share of the same enumeration on this host's cores.
@@ -33,7 +39,19 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_1BIT = 256 * 4 * 16 * 2.4e9 * 32  # MI355X 32-bit integer VALU lane-ops/s x 32 bits
+PEAK_LANE_OPS = 256 * 4 * 32 * 2.4e9  # MI355X 32-bit VALU lane-ops/s (wave64 issues in 2 cycles)
+PEAK_1BIT = PEAK_LANE_OPS * 32        # the same in 1-bit slots
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(threads, parts):
@@ -50,7 +68,7 @@ def cpu_baseline(threads, parts):
     wall = time.perf_counter() - t0
     assert r["hits"] == 0
     return {"value": r["ops"] / r["seconds"], "unit": "1-bit ops/s", "cores": r["threads"],
-            "kind": "reference",
+            "cpu_model": cpu_model(), "kind": "reference",
             "sample": "mul16 verify p = a * b, share 0 of %d (%d assignments, %.3g ops), "
                       "reference jit enum_run, %.1f s (%.1f s wall)"
                       % (parts, r["tried"], r["ops"], r["seconds"], wall),
@@ -234,18 +252,26 @@ def run_workload(args):
     print(json.dumps(out), flush=True)
 
 
-def profiled_traffic():
-    """HBM bytes per ts_enum launch from the committed counter summary (FETCH_SIZE + WRITE_SIZE
-    runs, scaled by the copy-kernel factor), or None if it is absent."""
+def profiled(source_sha16):
+    """From the committed counter summary of the ts_enum kernel: HBM bytes per launch
+    (FETCH_SIZE + WRITE_SIZE runs scaled by the copy-kernel factor) and VALU instructions per
+    launch (SQ_INSTS_VALU). Each only if the summary was measured on the kernel source this run
+    compiles (its sha256 prefix), else None."""
     path = os.path.join(ROOT, "profiles", "ts_enum_mul16_verify_counters.txt")
+    got = {}
     try:
         with open(path) as f:
             for ln in f:
-                if ln.startswith("traffic_bytes_per_launch:"):
-                    return float(ln.split(":", 1)[1])
+                if ":" in ln and not ln.startswith(" "):
+                    k, v = ln.split(":", 1)
+                    got[k.strip()] = v.strip()
     except OSError:
-        pass
-    return None
+        return None, None
+    if got.get("kernel_source_sha16") != source_sha16:
+        return None, None
+    tr = got.get("traffic_bytes_per_launch")
+    va = got.get("valu_insts_per_launch")
+    return (float(tr) if tr else None), (float(va) if va else None)
 
 
 def main():
@@ -314,8 +340,12 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
     value = ops_per_step * args.steps / wall
     kavg = sum(kernel_ms) / len(kernel_ms)
-    achieved = ops_per_step / (kavg * 1e-3)
+    algorithmic = ops_per_step / (kavg * 1e-3)
     valu_rate = eng.valu_rate()  # measured v_bitop3_b32 lane-ops/s on this device
+    import hashlib
+    sha16 = hashlib.sha256(t.enum_source(t.EN_VERIFY, prog, spec="p = a * b").encode()).hexdigest()[:16]
+    traffic, valu_insts = profiled(sha16)
+    achieved = valu_insts * 64 / (kavg * 1e-3) if valu_insts else None
     line = {
         "metric": "1-bit NAND/STO ops/s on 1 GPU (% int-VALU peak); mul16 2^32 verify sec",
         "value": value,
@@ -334,16 +364,17 @@ def main():
                    "program_instructions": prog.len, "assignments": 1 << 32,
                    "verify_seconds": ms_per_step / 1e3, "hits": 0,
                    "compile_seconds": compile_s, "device": eng.describe()},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_1BIT,
-                     "unit": "1-bit op/s", "frac": achieved / PEAK_1BIT, "traffic": None,
-                     "kernel": "ts_enum", "kernel_ms": kavg,
-                     "measured_valu_lane_ops_per_s": valu_rate,
-                     "frac_of_measured": achieved / (valu_rate * 32)},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_LANE_OPS,
+                     "unit": "32-bit VALU lane-op/s",
+                     "frac": achieved / PEAK_LANE_OPS if achieved else None, "traffic": traffic,
+                     "kernel": "ts_enum", "kernel_ms": kavg, "kernel_source_sha16": sha16,
+                     "valu_insts_per_launch": valu_insts,
+                     "algorithmic_1bit_ops_per_s": algorithmic,
+                     "peak_1bit_slots_per_s": PEAK_1BIT,
+                     "algorithmic_1bit_ops_per_1bit_slot": algorithmic / PEAK_1BIT,
+                     "probe_valu_lane_ops_per_s": valu_rate},
         "cpu_baseline": None,
     }
-    traffic = profiled_traffic()
-    if traffic is not None:
-        line["roofline"]["traffic"] = traffic
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_threads, args.cpu_parts)
     print(json.dumps(line), flush=True)

@@ -243,6 +243,13 @@ int  gpu_server_poll(gpu_server *s, uint64_t ticket, ts_enum_result *r, int *sta
                      size_t errlen);
 int  gpu_server_wait(gpu_server *s, uint64_t ticket, ts_enum_result *r, char *err, size_t errlen);
 int  gpu_server_times(gpu_server *s, uint64_t ticket, gpu_request_times *t);
+/* Frees a finished (done or failed) request: its copies of the programs and its ticket, which no
+   call knows afterwards. A server that runs for long releases each request once its result has
+   been read; requests never released are freed by gpu_server_close. Fails for an unknown ticket
+   or a request still queued or running. No reference counterpart (the reference has no server). */
+int  gpu_server_release(gpu_server *s, uint64_t ticket, char *err, size_t errlen);
+/* Requests the server holds: submitted and not yet released. */
+uint64_t gpu_server_live(gpu_server *s);
 void gpu_server_close(gpu_server *s);
 
 /* Listings without a device (the counterpart of `alpha jit`, reference src/main.c:468-496):

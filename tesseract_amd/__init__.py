@@ -224,6 +224,8 @@ def lib():
             "gpu_server_wait": (i32, [vp, u64, C.POINTER(EnumResultStruct), C.c_char_p, sz]),
             "gpu_server_times": (i32, [vp, u64, C.POINTER(GpuRequestTimes)]),
             "gpu_server_close": (None, [vp]),
+            "gpu_server_release": (i32, [vp, u64, C.c_char_p, sz]),
+            "gpu_server_live": (u64, [vp]),
             "free": (None, [vp]),
         }
         for name, (res, args) in sig.items():
@@ -848,6 +850,16 @@ class Server:
         t = GpuRequestTimes()
         lib().gpu_server_times(self._s, ticket, C.byref(t))
         return {f: getattr(t, f) for f, _ in GpuRequestTimes._fields_}
+
+    def release(self, ticket):
+        """Frees a finished request; its ticket is unknown afterwards."""
+        e = _err()
+        if not lib().gpu_server_release(self._s, ticket, e, len(e)):
+            raise ValueError(e.value.decode())
+
+    def live(self):
+        """Requests the server holds (submitted, not released)."""
+        return int(lib().gpu_server_live(self._s))
 
     def close(self):
         if self._s:

@@ -16,7 +16,8 @@ OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libtesseract.so")
 
 HOST_C = ["program", "builder", "gen", "spec", "tester", "benes", "net", "report"]
-ENGINE_CPP = ["lower", "codegen", "tape", "tape_kernel", "engine", "calibrate", "batch", "serve", "tester1"]
+ENGINE_CPP = ["lower", "codegen", "tape", "tape_kernel", "engine", "calibrate", "batch", "serve", "tester1",
+              "clear"]
 ARCH = "gfx950"
 # Packed FP32 stays off in every HIP compile.
 NO_PACKED = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

@@ -422,6 +422,32 @@ void say(char *err, size_t errlen, const char *fmt, const char *what)
         snprintf(err, errlen, fmt, what);
 }
 
+// A program the interpreter can run: every instruction a NAND or a STO of a cell below ncells,
+// and every cell's byte offset (cell x row, the sink cell ncells included) below 2^31, since bit 31
+// of a record word marks a NAND. Returns 0 with a message otherwise.
+int check_program(const ts_program *p, uint32_t row, char *err, size_t errlen)
+{
+    char b[128];
+    if ((uint64_t)(p->ncells + 1ull) * row >= (1ull << 31)) {
+        snprintf(b, sizeof b, "%u cells of %u bytes each are too many for the batch interpreter",
+                 p->ncells, row);
+        say(err, errlen, "%s", b);
+        return 0;
+    }
+    for (size_t i = 0; i < p->len; i++) {
+        const ts_instr in = p->code[i];
+        if (in.addr >= p->ncells || in.op > TS_OP_STO) {
+            if (in.op > TS_OP_STO)
+                snprintf(b, sizeof b, "instruction %zu has opcode %u", i, (unsigned)in.op);
+            else
+                snprintf(b, sizeof b, "instruction %zu reads cell %u of %u", i, in.addr, p->ncells);
+            say(err, errlen, "%s", b);
+            return 0;
+        }
+    }
+    return 1;
+}
+
 template <class T>
 struct DevBuf {
     T *p = nullptr;
@@ -449,6 +475,8 @@ extern "C" int gpu_batch_records(const ts_program *p, unsigned item_words, unsig
         say(err, errlen, "%s", "gpu_batch_records: a program, 8 or 16 slots and 1-64 item words");
         return 0;
     }
+    if (!check_program(p, item_words * 4, err, errlen))
+        return 0;
     Encoded enc = encode(p, item_words * 4, win);
     *nwords = enc.words.size();
     if (out)
@@ -478,6 +506,12 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     }
     for (size_t i = 0; i < n; i++) {
         const ts_program *p = items[i].prog;
+        if (!p) {
+            say(err, errlen, "%s", "a batch item without a program");
+            return 0;
+        }
+        if (!check_program(p, kLanes * 4, err, errlen))
+            return 0;
         if (p->ncells < 3) {
             say(err, errlen, "%s", "a batched program needs at least one writable cell");
             return 0;

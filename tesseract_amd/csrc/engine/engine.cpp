@@ -572,35 +572,43 @@ extern "C" int gpu_run(gpu_code *c, gpu_state *s, uint64_t passes, char *err, si
         };
         // captured graphs hold passes before the last one; the last pass is launched directly
         const unsigned gp = c->tune.graph_passes;
-        uint64_t p = 0;
-        if (gp > 1 && passes > gp) {
-            if (!s->graph[s->latest]) {
-                // capture gp passes starting from this parity (kernel launches only)
-                hipGraph_t g;
-                HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
-                int lat = s->latest, ok = 1;
-                for (unsigned q = 0; q < gp && ok; q++, lat = 1 - lat)
-                    ok = one_pass(lat, false);
-                hipError_t ce = hipStreamEndCapture(e->stream, &g);
-                if (!ok || ce != hipSuccess) {
-                    say(err, errlen, "graph capture failed: %s", hipGetErrorString(ce));
-                    return 0;
-                }
-                hipError_t ie = hipGraphInstantiate(&s->graph[s->latest], g, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(g);
-                if (ie != hipSuccess) {
-                    say(err, errlen, "graph instantiate failed: %s", hipGetErrorString(ie));
-                    return 0;
-                }
+        // the graph of gp passes starting from parity `lat`, captured on first use (kernel
+        // launches only); an odd gp alternates between the two parities' graphs
+        auto graph_for = [&](int lat) -> hipGraphExec_t {
+            if (s->graph[lat])
+                return s->graph[lat];
+            hipGraph_t g;
+            if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+                say(err, errlen, "graph capture could not start");
+                return nullptr;
             }
+            int l = lat, ok = 1;
+            for (unsigned q = 0; q < gp && ok; q++, l = 1 - l)
+                ok = one_pass(l, false);
+            hipError_t ce = hipStreamEndCapture(e->stream, &g);
+            if (!ok || ce != hipSuccess) {
+                say(err, errlen, "graph capture failed: %s", hipGetErrorString(ce));
+                return nullptr;
+            }
+            hipError_t ie = hipGraphInstantiate(&s->graph[lat], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) {
+                s->graph[lat] = nullptr;
+                say(err, errlen, "graph instantiate failed: %s", hipGetErrorString(ie));
+                return nullptr;
+            }
+            return s->graph[lat];
+        };
+        uint64_t p = 0;
+        if (gp > 1)
             for (; p + gp < passes; p += gp) {
-                HIPCHK(hipGraphLaunch(s->graph[s->latest], e->stream));
+                hipGraphExec_t ge = graph_for(s->latest);
+                if (!ge)
+                    return 0;
+                HIPCHK(hipGraphLaunch(ge, e->stream));
                 if (gp & 1)
                     s->latest = 1 - s->latest;
-                if (!s->graph[s->latest] && p + 2 * gp < passes)
-                    break;  // an odd pass count changed the parity: capture that one next call
             }
-        }
         for (; p < passes; p++) {
             if (!one_pass(s->latest, p + 1 == passes))
                 return 0;
@@ -991,13 +999,6 @@ extern "C" int gpu_enum_compile(gpu_engine *e, const gpu_enum *en, gpu_enum_code
     c->n = (unsigned)en->a->ninputs;
     c->ticks = en->ticks;
     c->len_total = en->a->len + (en->mode == TS_EN_EQUIV ? en->b->len : 0);
-    if (hipMalloc(&c->d_out, 2 * sizeof(uint64_t)) != hipSuccess ||
-        hipMalloc(&c->d_list, 2 * kListCap * sizeof(uint64_t)) != hipSuccess) {
-        (void)hipFree(c->d_out);
-        fail();
-        say(err, errlen, "hipMalloc failed for the hit list");
-        return 0;
-    }
     *out = c.release();
     return 1;
 }
@@ -1017,26 +1018,31 @@ extern "C" void gpu_enum_free(gpu_enum_code *c)
     delete c;
 }
 
-extern "C" int gpu_enum_exec(gpu_enum_code *c, ts_enum_result *r, char *err, size_t errlen)
-{
-    memset(r, 0, sizeof *r);
-    gpu_engine *e = c->e;
-    HIPCHK(hipSetDevice(e->device));
-    const gpu_tune tune = c->tune;
-    ShareInfo si = ts_internal::shares(c->n, c->part, c->part_end, c->parts);
-    r->space = si.space;
-    r->tried = si.tried;
-    r->ops = si.passes_run * c->ticks * c->len_total * 512;
-    r->threads = 1;
-    const uint64_t cap = kListCap;
-    uint64_t *d_out = c->d_out, *d_list = c->d_list;
-    hipFunction_t fn = c->fn;
+namespace {
 
-    auto run = [&](uint64_t i0, uint64_t i1, uint64_t *hits, uint64_t *nlist,
-                   std::vector<uint64_t> &list, bool timed) -> int {
-        HIPCHK(hipMemsetAsync(d_out, 0, 2 * sizeof(uint64_t), e->stream));
-        uint64_t space = si.space, capv = cap;
+// The whole enumeration of c into r with the hits counted and listed in b. Host side it allocates
+// nothing: the listed hit words come back in fixed chunks through a stack buffer.
+struct EnumExec {
+    gpu_enum_code  *c;
+    ts_internal::HitBuffers b;
+    ts_enum_result *r;
+    char           *err;
+    size_t          errlen;
+    ShareInfo       si;
+
+    // one launch over words [i0, i1): the hit count and the number of hit words listed
+    int launch_range(uint64_t i0, uint64_t i1, uint64_t *hits, uint64_t *nlist, bool timed)
+    {
+        gpu_engine *e = c->e;
+        const gpu_tune &tune = c->tune;
+        // cleared by a kernel: no memset node when a caller captures the stream
+        if (!ts_internal::clear_device_words(b.d_out, 2, e->stream)) {
+            say(err, errlen, "the hit counters could not be cleared");
+            return 0;
+        }
+        uint64_t space = si.space, capv = b.cap;
         uint32_t parts = (uint32_t)c->parts, part = (uint32_t)c->part, nshare = si.nshare;
+        uint64_t *d_out = b.d_out, *d_list = b.d_list;
         if (timed)
             HIPCHK(hipEventRecord(e->ev0, e->stream));
         if (c->tape) {
@@ -1053,7 +1059,7 @@ extern "C" int gpu_enum_exec(gpu_enum_code *c, ts_enum_result *r, char *err, siz
             uint64_t blocks = (i1 - i0 + tune.block - 1) / tune.block;
             unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, tune.grid));
             void *args[] = {&d_out, &d_list, &capv, &i0, &i1, &space, &parts, &part, &nshare};
-            if (!launch(e, fn, grid, tune.block, args, err, errlen))
+            if (!launch(e, c->fn, grid, tune.block, args, err, errlen))
                 return 0;
         }
         if (timed)
@@ -1068,46 +1074,88 @@ extern "C" int gpu_enum_exec(gpu_enum_code *c, ts_enum_result *r, char *err, siz
         }
         *hits = host[0];
         *nlist = host[1];
-        uint64_t keep = std::min<uint64_t>(host[1], cap);
-        list.resize(2 * keep);
-        if (keep)
-            HIPCHK(hipMemcpy(list.data(), d_list, 2 * keep * sizeof(uint64_t), hipMemcpyDeviceToHost));
         return 1;
-    };
-    auto collect = [&](const std::vector<uint64_t> &list) {
-        for (size_t i = 0; i + 1 < list.size(); i += 2)
-            for (uint32_t m = (uint32_t)list[i + 1]; m; m &= m - 1)
-                ts_enum_add_hit(r, list[i] * 32 + (uint64_t)__builtin_ctz(m));
-    };
+    }
+
+    // the listed hit words into r (the 16 smallest hits are kept)
+    int collect(uint64_t nlist)
+    {
+        constexpr uint64_t kChunk = 256;
+        uint64_t buf[2 * kChunk];
+        const uint64_t keep = std::min(nlist, b.cap);
+        for (uint64_t k = 0; k < keep; k += kChunk) {
+            const uint64_t n = std::min(kChunk, keep - k);
+            HIPCHK(hipMemcpy(buf, b.d_list + 2 * k, 2 * n * sizeof(uint64_t),
+                             hipMemcpyDeviceToHost));
+            for (uint64_t i = 0; i < n; i++)
+                for (uint32_t m = (uint32_t)buf[2 * i + 1]; m; m &= m - 1)
+                    ts_enum_add_hit(r, buf[2 * i] * 32 + (uint64_t)__builtin_ctz(m));
+        }
+        return 1;
+    }
+
     // Words are visited in increasing x within the share, so a range's hits are all smaller than
     // the next range's: when a range has more hit words than the list holds, its left half is
     // listed first and the right half only while fewer than 16 hits are known.
-    std::vector<uint64_t> list;
-    std::function<int(uint64_t, uint64_t)> collect_range = [&](uint64_t i0, uint64_t i1) -> int {
+    int collect_range(uint64_t i0, uint64_t i1)
+    {
         if (i0 >= i1 || r->nhit >= TS_EN_SHOW)
             return 1;
         uint64_t h = 0, nl = 0;
-        if (!run(i0, i1, &h, &nl, list, false))
+        if (!launch_range(i0, i1, &h, &nl, false))
             return 0;
-        if (nl <= cap) {
-            collect(list);
-            return 1;
-        }
-        uint64_t mid = i0 + (i1 - i0) / 2;
+        if (nl <= b.cap)
+            return collect(nl);
+        const uint64_t mid = i0 + (i1 - i0) / 2;
         return collect_range(i0, mid) && collect_range(mid, i1);
-    };
+    }
 
-    uint64_t hits = 0, nlist = 0;
-    auto t0 = std::chrono::steady_clock::now();
-    if (!run(0, si.nact_words, &hits, &nlist, list, true))
-        return 0;
-    r->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    r->hits = hits;
-    if (nlist <= cap)
-        collect(list);
-    else if (!collect_range(0, si.nact_words))
-        return 0;
-    return 1;
+    int run()
+    {
+        memset(r, 0, sizeof *r);
+        gpu_engine *e = c->e;
+        HIPCHK(hipSetDevice(e->device));
+        si = ts_internal::shares(c->n, c->part, c->part_end, c->parts);
+        r->space = si.space;
+        r->tried = si.tried;
+        r->ops = si.passes_run * c->ticks * c->len_total * 512;
+        r->threads = 1;
+        uint64_t hits = 0, nlist = 0;
+        auto t0 = std::chrono::steady_clock::now();
+        if (!launch_range(0, si.nact_words, &hits, &nlist, true))
+            return 0;
+        r->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        r->hits = hits;
+        if (nlist <= b.cap)
+            return collect(nlist);
+        return collect_range(0, si.nact_words);
+    }
+};
+
+}  // namespace
+
+extern "C" int gpu_enum_exec(gpu_enum_code *c, ts_enum_result *r, char *err, size_t errlen)
+{
+    memset(r, 0, sizeof *r);
+    HIPCHK(hipSetDevice(c->e->device));
+    if (!c->d_out) {  // the hit list, allocated on the first execution and kept with the code
+        if (hipMalloc(&c->d_out, 2 * sizeof(uint64_t)) != hipSuccess ||
+            hipMalloc(&c->d_list, 2 * kListCap * sizeof(uint64_t)) != hipSuccess) {
+            if (c->d_out)
+                (void)hipFree(c->d_out);
+            c->d_out = nullptr;
+            say(err, errlen, "hipMalloc failed for the hit list");
+            return 0;
+        }
+    }
+    return ts_internal::enum_exec(c, {c->d_out, c->d_list, kListCap}, r, err, errlen);
+}
+
+int ts_internal::enum_exec(gpu_enum_code *c, const HitBuffers &b, ts_enum_result *r, char *err,
+                           size_t errlen)
+{
+    EnumExec x{c, b, r, err, errlen, {}};
+    return x.run();
 }
 
 extern "C" int gpu_enum_run(gpu_engine *e, const gpu_enum *en, ts_enum_result *r, char *err,

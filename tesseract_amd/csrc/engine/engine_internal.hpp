@@ -38,6 +38,19 @@ ShareInfo shares(unsigned n, int part, int part_end, int parts);
 // hit words a launch can list before the list overflows
 constexpr uint64_t kListCap = 1u << 16;
 
+// n words from p set to 0 by a kernel on `stream` (clear.hip; no memset node under graph capture)
+bool clear_device_words(uint64_t *p, uint32_t n, hipStream_t stream);
+
+// Where an enumeration's kernels count and list their hits: d_out[0] = hits, d_out[1] = hit words
+// listed, d_list = `cap` hit words of two words each (word index, 32-bit hit mask).
+struct HitBuffers {
+    uint64_t *d_out, *d_list;
+    uint64_t  cap;
+};
+// the whole enumeration of c into r, hits counted and listed in b (gpu_enum_exec with its own
+// buffers); allocates nothing
+int enum_exec(gpu_enum_code *c, const HitBuffers &b, ts_enum_result *r, char *err, size_t errlen);
+
 // HIP source -> code object for arch (no device needed)
 int rtc_compile(const std::string &src, const std::string &arch, std::vector<char> &code,
                 double *seconds, char *err, size_t errlen);

@@ -121,6 +121,8 @@ struct Request {
     int state = ST_QUEUED;
     gpu_request_times times{};
     uint64_t len_total = 0;
+    bool in_compile = false;       // a compile thread reads the request's programs
+    bool release_wanted = false;   // released while in_compile: dropped when the compile ends
 };
 
 struct Slot {
@@ -175,6 +177,18 @@ void prepared(gpu_server *s, Request *r, bool ok, const char *err)
         s->ready.push_back(r);
         s->cv_sched.notify_all();
     }
+}
+
+// Frees a finished request (under the lock): its copied programs and its entry.
+void drop_request(gpu_server *s, Request *r)
+{
+    auto it = s->all.find(r->id);
+    if (it == s->all.end())
+        return;
+    ts_program_free(&r->a);
+    if (r->has_b)
+        ts_program_free(&r->b);
+    s->all.erase(it);
 }
 
 // Planner threads pick each request's tier as it arrives: the tape when its estimated device
@@ -238,6 +252,7 @@ void compile_loop(gpu_server *s)
                 return;
             r = s->to_compile.front();
             s->to_compile.pop_front();
+            r->in_compile = true;
         }
         char err[1024] = "";
         ts::KernelSource ks;
@@ -256,6 +271,11 @@ void compile_loop(gpu_server *s)
                 ok = rtc_compile(source, s->e->arch, code, nullptr, err, sizeof err) != 0;
         }
         std::lock_guard<std::mutex> lk(s->mu);
+        r->in_compile = false;
+        if (r->release_wanted) {  // finished on the tape and released meanwhile
+            drop_request(s, r);
+            continue;
+        }
         if (r->upgrade) {  // running on the tape: the scheduler switches it over if still running
             if (r->state == ST_RUNNING || r->state == ST_WAITING) {
                 r->source = std::move(source);
@@ -776,6 +796,41 @@ extern "C" int gpu_server_times(gpu_server *s, uint64_t ticket, gpu_request_time
         return 0;
     *t = it->second->times;
     return 1;
+}
+
+extern "C" int gpu_server_release(gpu_server *s, uint64_t ticket, char *err, size_t errlen)
+{
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(ticket);
+    if (it == s->all.end()) {
+        put_err(err, errlen, "no request %llu", (unsigned long long)ticket);
+        return 0;
+    }
+    Request *r = it->second.get();
+    if (r->state != ST_DONE && r->state != ST_FAILED) {
+        put_err(err, errlen, "request %llu has not finished", (unsigned long long)ticket);
+        return 0;
+    }
+    // a request that finished on the tape may still wait for, or be in, its kernel's compile
+    for (size_t k = 0; k < s->to_compile.size(); k++)
+        if (s->to_compile[k] == r) {
+            s->to_compile.erase(s->to_compile.begin() + (long)k);
+            break;
+        }
+    if (r->in_compile)
+        r->release_wanted = true;
+    else
+        drop_request(s, r);
+    return 1;
+}
+
+extern "C" uint64_t gpu_server_live(gpu_server *s)
+{
+    std::lock_guard<std::mutex> lk(s->mu);
+    uint64_t n = 0;
+    for (auto &kv : s->all)
+        n += !kv.second->release_wanted;
+    return n;
 }
 
 extern "C" void gpu_server_close(gpu_server *s)

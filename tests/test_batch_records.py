@@ -80,3 +80,17 @@ def test_records_slots_per_instruction():
 def test_records_refuse_other_windows():
     with pytest.raises(ValueError):
         t.batch_records(t.random_program(5, 20, 1), 64, 12)
+
+
+def test_records_refuse_a_cell_outside_the_program():
+    # a caller's program whose instruction names a cell >= ncells gets an error, not an encoding
+    # whose byte offsets run past the item's cells
+    prog = t.random_program(16, 40, 3)
+    t.batch_records(prog)
+    prog._p.code[7].addr = prog.ncells + 5
+    with pytest.raises(ValueError, match="reads cell"):
+        t.batch_records(prog)
+    prog._p.code[7].addr = 2
+    prog._p.code[9].op = 7
+    with pytest.raises(ValueError, match="opcode"):
+        t.batch_records(prog)

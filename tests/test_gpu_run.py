@@ -93,17 +93,19 @@ def test_life_cube_small(engine):
     check(engine, prog, runs=(1, 2))
 
 
-def test_graph_replays_match_eager(engine):
-    # segmented program, passes replayed from captured graphs of 4 passes: several replays with
-    # eager launches in between, each checked against the oracle
+@pytest.mark.parametrize("gp", [4, 3, 5])
+def test_graph_replays_match_eager(engine, gp):
+    # segmented program, passes replayed from captured graphs of gp passes: several replays with
+    # eager launches in between, each checked against the oracle. An odd gp flips the parity at
+    # every replay, so both parities' graphs are captured and replayed within one call.
     prog = random_cube_program(20, 300, 12, seed=77)
-    code = engine.compile(prog, {"graph_passes": 4})
+    code = engine.compile(prog, {"graph_passes": gp})
     assert code.stats()["segments"] > 1
     mem, rr = O.random_lines(prog, prog.groups, 5)
     st = code.state()
     st.put(mem, rr)
     done = 0
-    for n in (9, 8, 3, 4):
+    for n in (9, 8, 3, 4, 10, 20):
         st.run(n)
         done += n
         gm, gr = st.get()
@@ -193,3 +195,24 @@ def test_empty_program(engine):
     gm, gr = st.get()
     assert np.array_equal(gr, rr) and np.array_equal(gm[:, 2:], mem[:, 2:])
     assert st.pc == 0
+
+
+@pytest.mark.parametrize("dims", [9, 12])
+def test_power_on_state(engine, dims):
+    # a new state and a reset one hold the reference's power-on state (src/ref.c:24-32,
+    # src/lanes.c:110-192): every cell 0, ONE all ones, RR 0, id cells their machine-number
+    # patterns; then one pass from it equals the oracle's pass from its own power-on lines
+    prog = random_cube_program(16, 200, dims, seed=90 + dims)
+    code = engine.compile(prog)
+    groups = 2 * prog.groups
+    want_mem, want_rr = O.reset_lines(prog, groups)
+    st = code.state(groups)
+    got_mem, got_rr = st.get()
+    assert np.array_equal(got_mem, want_mem) and np.array_equal(got_rr, want_rr)
+    st.run(2)
+    run_mem, run_rr = O.run_lines(prog, want_mem, want_rr, 2)
+    got_mem, got_rr = st.get()
+    assert np.array_equal(got_mem, run_mem) and np.array_equal(got_rr, run_rr)
+    st.reset()
+    got_mem, got_rr = st.get()
+    assert np.array_equal(got_mem, want_mem) and np.array_equal(got_rr, want_rr)

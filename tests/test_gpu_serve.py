@@ -125,3 +125,33 @@ def test_server_tape_then_compiled(engine):
         assert key(res) == key(engine.enum_run(t.EN_EQUIV, a, b))
         for x in res["hit"]:
             assert c.evaluate(x) != r.evaluate(x)
+
+
+def test_server_release_frees_finished_requests(engine):
+    # a long-running server releases each request once its result has been read: the server then
+    # holds nothing for it (its ticket is unknown); unfinished requests cannot be released
+    srv = t.Server(engine, streams=2, chunk_words=1 << 18)
+    tickets, expect = [], []
+    for k in range(8):
+        c, r, faulty = small_request(21, k, [10, 16, 22, 26][k % 4], [60, 300][k % 2])
+        a, b = t.net_parse(c.bench()), t.net_parse(r.bench())
+        tickets.append(srv.submit(t.EN_EQUIV, a, b))
+        expect.append(engine.enum_run(t.EN_EQUIV, a, b))
+    rng = np.random.default_rng(5)
+    big_c = N.circuit(30, 4, 3000, rng)
+    big = srv.submit(t.EN_EQUIV, t.net_parse(big_c.bench()), t.net_parse(N.rewrite(big_c, rng).bench()))
+    with pytest.raises(ValueError, match="has not finished"):
+        srv.release(big)
+    assert srv.live() == 9
+    for tk, want in zip(tickets, expect):
+        got = srv.wait(tk)
+        assert (got["hits"], got["hit"], got["tried"]) == (want["hits"], want["hit"], want["tried"])
+        srv.release(tk)
+    assert srv.wait(big)["hits"] == 0
+    srv.release(big)
+    assert srv.live() == 0
+    with pytest.raises(ValueError, match="no request"):
+        srv.release(tickets[0])
+    with pytest.raises(ValueError, match="no request"):
+        srv.wait(tickets[0])
+    srv.close()

@@ -132,6 +132,16 @@ int  gpu_enum_compile(gpu_engine *e, const gpu_enum *en, gpu_enum_code **out, ch
                       size_t errlen);
 int  gpu_enum_exec(gpu_enum_code *c, ts_enum_result *r, char *err, size_t errlen);
 void gpu_enum_free(gpu_enum_code *c);
+/* gpu_enum_exec that allocates nothing: no counterpart in the reference. For callers that cannot
+   allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc). The hit counters and the hit list live in a fixed file-scope device array of
+   2 + 2 x 4,096 words per device (64 KiB), held until the process ends; list_words (at most 4,096,
+   0 = 4,096) bounds the hit words one launch lists before its words are re-run by halves, as
+   gpu_enum_exec does with its own list of 65,536. Larger values fail with a message. Results
+   are exactly gpu_enum_exec's. Every call on one device shares the array, so calls that overlap
+   on one device race; the array keeps the last call's contents. */
+int  gpu_enum_exec_noalloc(gpu_enum_code *c, unsigned list_words, ts_enum_result *r, char *err,
+                           size_t errlen);
 
 /* Batched cycle search: many link-free programs, each on `groups` x 512 machines started from the
    reference's t1_start stream of its seed (group 0 exactly t1_start, src/tester.c:83-91; later

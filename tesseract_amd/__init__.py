@@ -212,6 +212,8 @@ def lib():
                                     C.POINTER(sz), C.c_char_p, sz]),
             "gpu_enum_compile": (i32, [vp, C.POINTER(GpuEnumStruct), C.POINTER(vp), C.c_char_p, sz]),
             "gpu_enum_exec": (i32, [vp, C.POINTER(EnumResultStruct), C.c_char_p, sz]),
+            "gpu_enum_exec_noalloc": (i32, [vp, C.c_uint, C.POINTER(EnumResultStruct), C.c_char_p,
+                                            sz]),
             "gpu_enum_free": (None, [vp]),
             "gpu_batch_records": (i32, [P, C.c_uint, C.c_uint, C.POINTER(C.c_uint32), sz,
                                         C.POINTER(C.c_size_t), C.c_char_p, sz]),
@@ -806,6 +808,15 @@ class EnumCode:
         e = _err()
         if not lib().gpu_enum_exec(self._c, C.byref(r), e, len(e)):
             raise RuntimeError(e.value.decode())
+        return _result(r)
+
+    def run_noalloc(self, list_words=0):
+        """gpu_enum_exec_noalloc: the same enumeration with the hit list in the library's static
+        device scratch (at most 4,096 hit words per launch; 0 = all of it)."""
+        r = EnumResultStruct()
+        e = _err()
+        if not lib().gpu_enum_exec_noalloc(self._c, list_words, C.byref(r), e, len(e)):
+            raise ValueError(e.value.decode())
         return _result(r)
 
 

@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libtesseract.so")
 
 HOST_C = ["program", "builder", "gen", "spec", "tester", "benes", "net", "report"]
 ENGINE_CPP = ["lower", "codegen", "tape", "tape_kernel", "engine", "calibrate", "batch", "serve", "tester1",
-              "clear"]
+              "clear", "noalloc"]
 ARCH = "gfx950"
 # Packed FP32 stays off in every HIP compile.
 NO_PACKED = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

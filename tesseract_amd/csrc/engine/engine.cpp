@@ -191,6 +191,8 @@ extern "C" int gpu_engine_open(int device, gpu_engine **out, char *err, size_t e
     HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&e->ev0));
     HIPCHK(hipEventCreate(&e->ev1));
+    if (!ts_internal::static_scratch_init(device, err, errlen))
+        return 0;
     *out = e.release();
     return 1;
 }
@@ -1149,6 +1151,11 @@ extern "C" int gpu_enum_exec(gpu_enum_code *c, ts_enum_result *r, char *err, siz
         }
     }
     return ts_internal::enum_exec(c, {c->d_out, c->d_list, kListCap}, r, err, errlen);
+}
+
+int ts_internal::enum_device(const gpu_enum_code *c)
+{
+    return c->e->device;
 }
 
 int ts_internal::enum_exec(gpu_enum_code *c, const HitBuffers &b, ts_enum_result *r, char *err,

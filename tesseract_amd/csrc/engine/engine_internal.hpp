@@ -50,6 +50,12 @@ struct HitBuffers {
 // the whole enumeration of c into r, hits counted and listed in b (gpu_enum_exec with its own
 // buffers); allocates nothing
 int enum_exec(gpu_enum_code *c, const HitBuffers &b, ts_enum_result *r, char *err, size_t errlen);
+// the device an enumeration runs on
+int enum_device(const gpu_enum_code *c);
+
+// gpu_enum_exec_noalloc's file-scope scratch (noalloc.hip): its address on `device`, which must be
+// current, taken once per device when an engine opens
+int static_scratch_init(int device, char *err, size_t errlen);
 
 // HIP source -> code object for arch (no device needed)
 int rtc_compile(const std::string &src, const std::string &arch, std::vector<char> &code,

@@ -319,7 +319,8 @@ def main():
 
     import tesseract_amd as t
     eng = t.Engine(0)
-    prog = t.gen_mul(16)
+    from tesseract_amd.programs import golden
+    prog = golden("mul16")  # the reference's gen_mul(16), a program file
     t0 = time.perf_counter()
     code = t.EnumCode(eng, t.EN_VERIFY, prog, spec="p = a * b")
     compile_s = time.perf_counter() - t0

@@ -1,6 +1,6 @@
 /* Host side of the engine: everything that produces or describes the programs the GPU engine runs.
-   Program helpers, the gate builder, the workload generators, verify/search specifications, random
-   programs for the cycle tester, and Beneš routing. Plain C, no device code.
+   Program helpers, the program writer and the .bench netlist loader, verify/search specifications,
+   random programs for the cycle tester, and Beneš routing. No device code.
 
    Each function names the reference routine whose behaviour it reproduces. */
 #ifndef TESSERACT_HOST_H
@@ -27,64 +27,48 @@ int      ts_program_id_name(const char *name);  /* ID0..ID15 -> bit, else -1 */
 unsigned ts_addr_bits(uint32_t ncells);
 /* A deep copy (so a caller's program can be freed while the copy lives). */
 int      ts_program_copy(ts_program *dst, const ts_program *src);
-/* Writes the program as assembly (reference src/build.c:432-444). */
-void     ts_program_print(FILE *f, const ts_program *p);
+/* A program from flat arrays (the form program files store), in memory ts_program_free releases:
+   code[i] = addr | op << 31; names = ncells NUL-terminated names one after another; kind, dim, src
+   per cell, or all three NULL for a program without id and link cells. Checks that every address
+   is a cell and every special cell is well formed; returns 0 with a message otherwise. */
+int      ts_program_from_arrays(ts_program *out, const uint32_t *code, size_t len, uint32_t ncells,
+                                const char *names, size_t names_bytes, const uint32_t *inputs,
+                                size_t ninputs, const uint32_t *outputs, size_t noutputs,
+                                const uint8_t *kind, const uint8_t *dim, const uint32_t *src,
+                                char *err, size_t errlen);
 
-/* ---- gate builder (reference src/build.h) ---------------------------------------------------- */
+/* ---- the program writer: cells, gates, registers (the reference's src/build.h interface) ------ */
 
 enum { TS_G_BUF, TS_G_NOT, TS_G_AND, TS_G_NAND, TS_G_OR, TS_G_NOR, TS_G_XOR, TS_G_XNOR, TS_G_COUNT };
 
 typedef struct ts_builder ts_builder;
 
+/* NULL if out of memory. A misuse (an unknown gate, a write to a read-only cell, a cell that does
+   not exist) or running out of memory is recorded and reported by ts_b_finish. */
 ts_builder *ts_b_new(void);
-uint32_t    ts_b_cell(ts_builder *b, const char *name);
+uint32_t    ts_b_cell(ts_builder *b, const char *name);  /* created on first use; ID0..ID15 are id cells */
 int         ts_b_has(const ts_builder *b, const char *name);
 void        ts_b_input(ts_builder *b, uint32_t cell);
 void        ts_b_output(ts_builder *b, uint32_t cell);
+/* out = gate(in[0..n)); the gate reads all its inputs before it writes out (out may be an input) */
 void        ts_b_gate(ts_builder *b, int type, uint32_t out, const uint32_t *in, int n);
 void        ts_b_gate2(ts_builder *b, int type, uint32_t out, uint32_t x, uint32_t y);
+/* q <- d at the end of every pass, all registers at once */
 void        ts_b_reg(ts_builder *b, uint32_t q, uint32_t d);
-void        ts_b_mux(ts_builder *b, uint32_t out, uint32_t s, uint32_t ns, uint32_t a1, uint32_t a0);
+/* s = x ^ y, k = x & y (11 instructions; s may be x or y) */
 void        ts_b_half_adder(ts_builder *b, uint32_t s, uint32_t k, uint32_t x, uint32_t y);
+/* s = x ^ y ^ c, k = the carry (19 instructions; s may be x or y, k may be c) */
 void        ts_b_full_adder(ts_builder *b, uint32_t s, uint32_t k, uint32_t x, uint32_t y, uint32_t c);
-void        ts_b_select(ts_builder *b, uint32_t out, const uint32_t *sel, const uint32_t *val, int n);
+/* the new read-only cell `name` reads `src` of the machine across dimension dim */
 uint32_t    ts_b_link(ts_builder *b, const char *name, uint32_t src, int dim);
 size_t      ts_b_len(const ts_builder *b);
-/* Raw instructions, for generators that hand-schedule (e.g. the cube library programs). */
+/* Raw instructions on any existing cell (a STO to a read-only cell is kept; machines ignore it). */
 void        ts_b_nand(ts_builder *b, uint32_t cell);
 void        ts_b_sto(ts_builder *b, uint32_t cell);
-void        ts_b_finish(ts_builder *b, ts_program *out);  /* frees the builder */
+/* Emits the registers, fills out and frees the builder; 0 with the first recorded misuse in err. */
+int         ts_b_finish(ts_builder *b, ts_program *out, char *err, size_t errlen);
 const char *ts_gate_name(int type);
-int         ts_gate_type(const char *name);
-
-/* ---- generators (reference src/gen.h) -------------------------------------------------------- */
-
-void     ts_gen_mul(ts_program *p, int bits);
-int      ts_gen_hash(ts_program *p, int bits, int rounds);
-uint64_t ts_toyhash(uint64_t x, int bits, int rounds);
-void     ts_gen_life(ts_program *p, int w, int h, uint32_t *board);
-unsigned ts_grid_machine(unsigned row, unsigned col, int cbits);
-void     ts_gen_life_cube(ts_program *p, int w, int h, int rbits, int cbits, uint32_t *board);
-void     ts_gen_rank(ts_program *p, int dims);
-void     ts_gen_wht(ts_program *p, int bits);
-void     ts_gen_xorconv(ts_program *p);
-/* An n-bit ripple-carry adder with carry in: inputs a0.., b0.., c; outputs s0.. and c (the carry
-   out replaces the carry in), 19 instructions per bit (the shape of reference examples/adder8.asm). */
-void     ts_gen_adder(ts_program *p, int bits);
-/* Cube library programs (the semantics of reference lib/cube.inc PERM8 and SORT8):
-   ts_gen_perm8: inputs v0..v7 and c0..c16; 17 Beneš rounds over dims 0..8..0; round s swaps v with
-   the machine across benes_dim(s) where c<s> is 1. Output v0..v7.
-   ts_gen_sort8: input v0..v7; a bitonic sort over the 512 machines, machine k ends with the k-th
-   smallest value. Output v0..v7. */
-void     ts_gen_perm8(ts_program *p);
-void     ts_gen_sort8(ts_program *p);
-/* c = a * b in Z_12289[x] / (x^512 + 1) by the number-theoretic transform on the 512-machine
-   cube (reference gen_polymul, src/gen.c:634-669; 220,758 instructions): machine k holds
-   coefficient k of a (a0..a13) and b (b0..b13) and the twiddles of ts_ntt_twiddles (inputs
-   zfD_0..13, ziD_0..13 for rounds D = 0..8); after one pass c0..c13 hold coefficient k of c. */
-void     ts_gen_polymul(ts_program *p);
-void     ts_ntt_twiddles(uint16_t zf[TS_CUBE_DIMS][TS_CUBE_SIZE],
-                         uint16_t zi[TS_CUBE_DIMS][TS_CUBE_SIZE]);
+int         ts_gate_type(const char *name);  /* -1 if unknown; BUF and BUFF both mean TS_G_BUF */
 
 /* ---- specifications (reference src/spec.h) --------------------------------------------------- */
 

@@ -1,9 +1,10 @@
 """tesseract_amd: an MI355X engine for the bit-sliced NAND/STO machine.
 
-Python mirror of the C interfaces in include/: programs and their generators (the reference's
-gen_mul, gen_hash, gen_life_cube, ...), verify/search specs, and the GPU engine (gpu_compile,
-gpu_run, gpu_enum_run). Everything runs in libtesseract.so; this module only binds it with ctypes.
-There is no CPU fallback: without the built library, or without a GPU, the engine calls raise.
+Python mirror of the C interfaces in include/: programs (written with the program writer, loaded
+from .bench netlists or from program files, see tesseract_amd.programs), verify/search specs, and
+the GPU engine (gpu_compile, gpu_run, gpu_enum_run, batches, the server). Everything runs in
+libtesseract.so; this module only binds it with ctypes. There is no CPU fallback: without the
+built library, or without a GPU, the engine calls raise.
 """
 import ctypes as C
 import os
@@ -131,20 +132,6 @@ def lib():
             "ts_program_free": (None, [P]),
             "ts_program_dims": (C.c_uint, [P]),
             "ts_program_copy": (i32, [P, P]),
-            "ts_gen_mul": (None, [P, i32]),
-            "ts_gen_adder": (None, [P, i32]),
-            "ts_gen_hash": (i32, [P, i32, i32]),
-            "ts_toyhash": (u64, [u64, i32, i32]),
-            "ts_gen_life": (None, [P, i32, i32, C.POINTER(u32)]),
-            "ts_gen_life_cube": (None, [P, i32, i32, i32, i32, C.POINTER(u32)]),
-            "ts_grid_machine": (C.c_uint, [C.c_uint, C.c_uint, i32]),
-            "ts_gen_rank": (None, [P, i32]),
-            "ts_gen_wht": (None, [P, i32]),
-            "ts_gen_xorconv": (None, [P]),
-            "ts_gen_perm8": (None, [P]),
-            "ts_gen_sort8": (None, [P]),
-            "ts_gen_polymul": (None, [P]),
-            "ts_ntt_twiddles": (None, [vp, vp]),
             "ts_random_program": (None, [P, u32, sz, u64]),
             "ts_t1_program_seed": (u64, [u64, u32, u64]),
             "ts_t1_start": (None, [P, u64, C.c_uint, vp, vp]),
@@ -178,7 +165,11 @@ def lib():
             "ts_b_gate": (None, [vp, i32, u32, C.POINTER(u32), i32]),
             "ts_b_reg": (None, [vp, u32, u32]),
             "ts_b_len": (sz, [vp]),
-            "ts_b_finish": (None, [vp, P]),
+            "ts_b_finish": (i32, [vp, P, C.c_char_p, sz]),
+            "ts_b_half_adder": (None, [vp, u32, u32, u32, u32]),
+            "ts_b_full_adder": (None, [vp, u32, u32, u32, u32, u32]),
+            "ts_program_from_arrays": (i32, [P, vp, sz, u32, vp, sz, vp, sz, vp, sz, vp, vp, vp,
+                                              C.c_char_p, sz]),
             "ts_enum_add_hit": (None, [C.POINTER(EnumResultStruct), u64]),
             "gpu_engine_open": (i32, [i32, C.POINTER(vp), C.c_char_p, sz]),
             "gpu_engine_close": (None, [vp]),
@@ -245,7 +236,11 @@ def _err():
 
 
 class Program:
-    """A NAND/STO program owned by the native library (a ts_program)."""
+    """A NAND/STO program owned by the native library (a ts_program). Programs loaded from files
+    carry their extra arrays (extra) and origin (made_by)."""
+
+    extra = {}
+    made_by = None
 
     def __init__(self):
         self._p = ProgramStruct()
@@ -321,73 +316,6 @@ def _make(fn, *args):
     return p, r
 
 
-def gen_mul(bits):
-    return _make(lib().ts_gen_mul, bits)[0]
-
-
-def gen_adder(bits):
-    return _make(lib().ts_gen_adder, bits)[0]
-
-
-def gen_hash(bits, rounds):
-    p, ok = _make(lib().ts_gen_hash, bits, rounds)
-    if not ok:
-        raise ValueError("no bit permutation for %d bits" % bits)
-    return p
-
-
-def toyhash(x, bits, rounds):
-    return int(lib().ts_toyhash(x, bits, rounds))
-
-
-def gen_life(w, h):
-    board = (C.c_uint32 * (w * h))()
-    p = _make(lib().ts_gen_life, w, h, board)[0]
-    return p, np.array(board, dtype=np.uint32)
-
-
-def gen_life_cube(w, h, rbits, cbits):
-    board = (C.c_uint32 * (w * h))()
-    p = _make(lib().ts_gen_life_cube, w, h, rbits, cbits, board)[0]
-    return p, np.array(board, dtype=np.uint32)
-
-
-def grid_machine(row, col, cbits):
-    return int(lib().ts_grid_machine(row, col, cbits))
-
-
-def gen_rank(dims):
-    return _make(lib().ts_gen_rank, dims)[0]
-
-
-def gen_wht(bits):
-    return _make(lib().ts_gen_wht, bits)[0]
-
-
-def gen_xorconv():
-    return _make(lib().ts_gen_xorconv)[0]
-
-
-def gen_perm8():
-    return _make(lib().ts_gen_perm8)[0]
-
-
-def gen_sort8():
-    return _make(lib().ts_gen_sort8)[0]
-
-
-def gen_polymul():
-    return _make(lib().ts_gen_polymul)[0]
-
-
-def ntt_twiddles():
-    """(zf, zi): (9, 512) u16 arrays, the twiddle inputs of gen_polymul per round and machine."""
-    zf = np.zeros((CUBE_DIMS, CUBE_SIZE), np.uint16)
-    zi = np.zeros((CUBE_DIMS, CUBE_SIZE), np.uint16)
-    lib().ts_ntt_twiddles(zf.ctypes.data, zi.ctypes.data)
-    return zf, zi
-
-
 def random_program(cells, length, seed):
     return _make(lib().ts_random_program, cells, length, seed)[0]
 
@@ -455,10 +383,20 @@ class Builder:
     def __len__(self):
         return int(lib().ts_b_len(self._b))
 
+    def half_adder(self, s, k, x, y):
+        lib().ts_b_half_adder(self._b, s, k, x, y)
+
+    def full_adder(self, s, k, x, y, c):
+        lib().ts_b_full_adder(self._b, s, k, x, y, c)
+
     def finish(self):
         p = Program()
-        lib().ts_b_finish(self._b, p.ptr)
+        e = _err()
+        ok = lib().ts_b_finish(self._b, p.ptr, e, len(e))
         self._b = None
+        if not ok:
+            p._owned = False
+            raise ValueError(e.value.decode())
         return p
 
 

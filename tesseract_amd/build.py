@@ -1,7 +1,8 @@
 """Builds the native library in-tree: tesseract_amd/libtesseract.so.
 
-Host C (program model, builder, generators, specs, netlists) is compiled with gcc; the engine (C++
-over the HIP runtime and hipRTC, plus the hand-written gfx950 kernels) with hipcc for gfx950.
+Host code (program model, program writer and .bench loader, specs, random programs, reports) is
+compiled with gcc / g++; the engine (C++ over the HIP runtime and hipRTC, plus the hand-written
+gfx950 kernels) with hipcc for gfx950.
 Run as `python -m tesseract_amd.build`.
 """
 import os
@@ -15,7 +16,8 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libtesseract.so")
 
-HOST_C = ["program", "builder", "gen", "spec", "tester", "benes", "net", "report"]
+HOST_C = ["program", "tester", "benes", "report"]
+HOST_CPP = ["spec", "circuit"]
 ENGINE_CPP = ["lower", "codegen", "tape", "tape_kernel", "engine", "calibrate", "batch", "serve", "tester1",
               "clear", "noalloc"]
 ARCH = "gfx950"
@@ -37,6 +39,12 @@ def build(verbose=False, jobs=8):
         obj = os.path.join(OBJ, name + ".o")
         cmds.append(["gcc", "-std=gnu11", "-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-maybe-uninitialized",
                      "-I" + INCLUDE, "-c", src, "-o", obj])
+        objs.append(obj)
+    for name in HOST_CPP:
+        src = os.path.join(CSRC, "host", name + ".cpp")
+        obj = os.path.join(OBJ, name + ".o")
+        cmds.append(["g++", "-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra", "-I" + INCLUDE, "-c", src,
+                     "-o", obj])
         objs.append(obj)
     for name in ENGINE_CPP:
         src = os.path.join(CSRC, "engine", name + ".cpp")

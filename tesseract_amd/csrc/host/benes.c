@@ -7,7 +7,6 @@
    by bit d; the two values of every input pair and of every output pair must take different
    halves. Walking the cycles those constraints form assigns every value a half; each half is then
    routed the same way one dimension up. */
-#include <stdlib.h>
 #include <string.h>
 
 #include "tesseract_host.h"
@@ -29,7 +28,7 @@ static void route(route_item *v, int n, int d, uint8_t ctrl[TS_BENES_ROUNDS][TS_
     /* index of the value at each entry / exit position (positions are < 512) */
     static __thread int at_from[TS_CUBE_DIMS][TS_CUBE_SIZE], at_to[TS_CUBE_DIMS][TS_CUBE_SIZE];
     int *by_from = at_from[d], *by_to = at_to[d];
-    signed char *half = malloc((size_t)n);
+    signed char half[TS_CUBE_SIZE];
     for (int i = 0; i < n; i++) {
         by_from[v[i].from] = i;
         by_to[v[i].to] = i;
@@ -44,7 +43,7 @@ static void route(route_item *v, int n, int d, uint8_t ctrl[TS_BENES_ROUNDS][TS_
             cur = by_from[v[partner_out].from ^ bit];      /* shares that one's entry pair */
         }
     }
-    route_item *sub = malloc((size_t)n * sizeof *sub);
+    route_item sub[TS_CUBE_SIZE];
     int fill[2] = {0, n / 2};
     for (int i = 0; i < n; i++) {
         unsigned h = (unsigned)half[i];
@@ -57,8 +56,12 @@ static void route(route_item *v, int n, int d, uint8_t ctrl[TS_BENES_ROUNDS][TS_
     }
     route(sub, n / 2, d + 1, ctrl);
     route(sub + n / 2, n / 2, d + 1, ctrl);
-    free(sub);
-    free(half);
+}
+
+/* the dimension round s crosses: 0, 1, .., 8, .., 1, 0 */
+int ts_benes_dim(int round)
+{
+    return round < TS_CUBE_DIMS ? round : TS_BENES_ROUNDS - 1 - round;
 }
 
 void ts_benes_route(const uint16_t *dest, uint8_t ctrl[TS_BENES_ROUNDS][TS_CUBE_SIZE])

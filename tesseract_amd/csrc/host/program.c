@@ -1,5 +1,6 @@
 /* Program helpers: the read-only rule, link resolution, cube size (reference src/asm.c:767-820). */
 #include <ctype.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -134,29 +135,76 @@ int ts_program_copy(ts_program *dst, const ts_program *src)
     return 1;
 }
 
-static void print_list(FILE *f, const char *directive, const ts_program *p, const uint32_t *cells,
-                       size_t n)
+int ts_program_from_arrays(ts_program *out, const uint32_t *code, size_t len, uint32_t ncells,
+                           const char *names, size_t names_bytes, const uint32_t *inputs,
+                           size_t ninputs, const uint32_t *outputs, size_t noutputs,
+                           const uint8_t *kind, const uint8_t *dim, const uint32_t *src, char *err,
+                           size_t errlen)
 {
-    for (size_t i = 0; i < n; i++) {
-        fputs(i % 16 == 0 ? (i ? "\n" : "") : " ", f);
-        if (i % 16 == 0)
-            fputs(directive, f);
-        fputs(p->cell_names[cells[i]], f);
+    memset(out, 0, sizeof *out);
+    if (ncells < TS_CELL_FIRST_FREE) {
+        snprintf(err, errlen, "a program has at least the cells ZERO and ONE");
+        return 0;
     }
-    if (n)
-        fputc('\n', f);
-}
-
-void ts_program_print(FILE *f, const ts_program *p)
-{
-    print_list(f, ".in  ", p, p->inputs, p->ninputs);
-    print_list(f, ".out ", p, p->outputs, p->noutputs);
-    for (uint32_t c = 0; p->special && c < p->ncells; c++)
-        if (p->special[c].kind == TS_CK_LINK)
-            fprintf(f, ".link %s = %s @ %u\n", p->cell_names[c], p->cell_names[p->special[c].src],
-                    p->special[c].dim);
-    fputc('\n', f);
-    for (size_t i = 0; i < p->len; i++)
-        fprintf(f, "    %-4s %s\n", p->code[i].op == TS_OP_NAND ? "NAND" : "STO",
-                p->cell_names[p->code[i].addr]);
+    for (size_t i = 0; i < len; i++)
+        if ((code[i] & 0x7FFFFFFFu) >= ncells) {
+            snprintf(err, errlen, "instruction %zu names cell %u of %u", i, code[i] & 0x7FFFFFFFu,
+                     ncells);
+            return 0;
+        }
+    for (size_t i = 0; i < ninputs + noutputs; i++) {
+        uint32_t c = i < ninputs ? inputs[i] : outputs[i - ninputs];
+        if (c >= ncells) {
+            snprintf(err, errlen, "input or output %zu names cell %u of %u", i, c, ncells);
+            return 0;
+        }
+    }
+    out->code = malloc((len ? len : 1) * sizeof *out->code);
+    out->cell_names = calloc(ncells, sizeof *out->cell_names);
+    out->inputs = dup_bytes(inputs, ninputs * sizeof *inputs);
+    out->outputs = dup_bytes(outputs, noutputs * sizeof *outputs);
+    if (kind)
+        out->special = calloc(ncells, sizeof *out->special);
+    if (!out->code || !out->cell_names || !out->inputs || !out->outputs || (kind && !out->special)) {
+        ts_program_free(out);
+        snprintf(err, errlen, "out of memory");
+        return 0;
+    }
+    out->ncells = ncells;
+    out->len = len;
+    out->ninputs = ninputs;
+    out->noutputs = noutputs;
+    for (size_t i = 0; i < len; i++) {
+        out->code[i].op = (uint8_t)(code[i] >> 31);
+        out->code[i].addr = code[i] & 0x7FFFFFFFu;
+    }
+    size_t at = 0;
+    for (uint32_t c = 0; c < ncells; c++) {
+        size_t n = at < names_bytes ? strnlen(names + at, names_bytes - at) : 0;
+        if (at + n >= names_bytes) {
+            ts_program_free(out);
+            snprintf(err, errlen, "the names end before cell %u", c);
+            return 0;
+        }
+        out->cell_names[c] = dup_bytes(names + at, n + 1);
+        if (!out->cell_names[c]) {
+            ts_program_free(out);
+            snprintf(err, errlen, "out of memory");
+            return 0;
+        }
+        at += n + 1;
+    }
+    for (uint32_t c = 0; kind && c < ncells; c++) {
+        out->special[c].kind = kind[c];
+        out->special[c].dim = dim[c];
+        out->special[c].src = src[c];
+        int bad = kind[c] > TS_CK_LINK || (kind[c] != TS_CK_PLAIN && dim[c] >= TS_CUBE_MAX_DIMS) ||
+                  (kind[c] == TS_CK_LINK && src[c] >= c);  /* a link reads an older cell */
+        if (bad) {
+            ts_program_free(out);
+            snprintf(err, errlen, "cell %u is a malformed id or link cell", c);
+            return 0;
+        }
+    }
+    return 1;
 }

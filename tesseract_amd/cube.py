@@ -1,12 +1,14 @@
 """Programs on the 512-machine cube, the front end of the reference's cube demos
-(src/cubedemo.c:253-377): a Benes permutation (PERM8) and a bitonic sort (SORT8) of one byte per
-machine, checked against benes_apply and qsort there, and the polynomial product by the
-number-theoretic transform, checked against polymul_scalar. Many independent cubes run in one
-state: cube i is group i.
+(src/cubedemo.c:253-377): a Benes permutation (PERM8 of lib/cube.inc) and a bitonic sort (SORT8)
+of one byte per machine, checked against benes_apply and qsort there, and the polynomial product
+by the number-theoretic transform (gen_polymul), checked against polymul_scalar. The programs are
+the reference's, from program files (default: tests/golden/programs). Many independent cubes run
+in one state: cube i is group i.
 """
 import numpy as np
 
-from . import BENES_ROUNDS, CUBE_DIMS, benes_route, gen_perm8, gen_polymul, gen_sort8, ntt_twiddles
+from . import BENES_ROUNDS, CUBE_DIMS, benes_route
+from .programs import golden
 
 
 def _bits_to_lines(bits):
@@ -51,8 +53,8 @@ class CubeByteProgram:
 class Perm8(CubeByteProgram):
     """Machine k's byte moves to machine dest[k] through 17 Benes rounds."""
 
-    def __init__(self, engine):
-        super().__init__(engine, gen_perm8())
+    def __init__(self, engine, prog=None):
+        super().__init__(engine, prog if prog is not None else golden("perm8"))
         self.c = [self.prog.cell("c%d" % s) for s in range(BENES_ROUNDS)]
 
     def run(self, values, dests, repeats=0):
@@ -66,8 +68,8 @@ class Perm8(CubeByteProgram):
 class Sort8(CubeByteProgram):
     """Afterwards machine k holds the k-th smallest of the cube's 512 bytes."""
 
-    def __init__(self, engine):
-        super().__init__(engine, gen_sort8())
+    def __init__(self, engine, prog=None):
+        super().__init__(engine, prog if prog is not None else golden("sort8"))
 
     def run(self, values, repeats=0):
         return self._run(np.atleast_2d(values), repeats=repeats)
@@ -81,8 +83,8 @@ class PolyMul:
 
     BITS = 14
 
-    def __init__(self, engine, tune=None):
-        self.prog = gen_polymul()
+    def __init__(self, engine, tune=None, prog=None):
+        self.prog = prog if prog is not None else golden("polymul")
         self._cells()
         self.code = engine.compile(self.prog, tune)
 
@@ -91,7 +93,7 @@ class PolyMul:
         self.a = [p.cell("a%d" % i) for i in range(self.BITS)]
         self.b = [p.cell("b%d" % i) for i in range(self.BITS)]
         self.c = [p.cell("c%d" % i) for i in range(self.BITS)]
-        zf, zi = ntt_twiddles()
+        zf, zi = p.extra["zf"], p.extra["zi"]  # the reference's ntt_twiddles, per round
         self.twiddles = {}  # cell -> its line, the same in every cube
         for d in range(CUBE_DIMS):
             for i in range(self.BITS):

@@ -2,13 +2,14 @@
 (src/work.c:490-701) on the GPU engine.
 
 One torus of (2^rbits * h) x (2^cbits * w) cells, a w x h tile in each of the 2^(rbits+cbits)
-machines, tile (R, C) in machine grid_machine(R, C) = gray(R) << cbits | gray(C), one generation per
-pass of gen_life_cube's program. The start board is the reference's: splitmix64 from seed 42, 8
-words per board cell per group of 512 machines, group by group (src/work.c:588-592).
+machines, tile (R, C) in machine gray(R) << cbits | gray(C) (the reference's grid_machine,
+src/gen.c:223-226), one generation per pass of the reference's gen_life_cube program (a program
+file carrying its board map and geometry). The start board is the reference's: splitmix64 from
+seed 42, 8 words per board cell per group of 512 machines, group by group (src/work.c:588-592).
 """
 import numpy as np
 
-from . import gen_life_cube
+from .programs import golden
 
 GAMMA = 0x9E3779B97F4A7C15
 M1 = 0xBF58476D1CE4E5B9
@@ -42,9 +43,14 @@ TUNED_SLICE_NODES = {(32, 32, 8, 8): 450}
 
 
 class LifeCube:
-    def __init__(self, engine, w=32, h=32, rbits=8, cbits=8, seed=42, tune=None):
+    """prog: a gen_life_cube program file (default: the 8192 x 8192 board, 32 x 32 tiles on
+    2^16 machines)."""
+
+    def __init__(self, engine, prog=None, seed=42, tune=None):
+        self.prog = prog if prog is not None else golden("life_cube_32_32_8_8")
+        w, h, rbits, cbits = (int(v) for v in self.prog.extra["geometry"])
         self.w, self.h, self.rbits, self.cbits = w, h, rbits, cbits
-        self.prog, self.board = gen_life_cube(w, h, rbits, cbits)
+        self.board = np.asarray(self.prog.extra["board"], np.uint32)
         if tune is None and (w, h, rbits, cbits) in TUNED_SLICE_NODES:
             tune = {"slice_nodes": TUNED_SLICE_NODES[(w, h, rbits, cbits)]}
         self.code = engine.compile(self.prog, tune)

@@ -175,7 +175,8 @@ def run_tape(engine, inputs=26, gates=2000, seed=3, runs=5):
 
 
 def run_hash(engine, target=0x165C2A07):
-    prog = t.gen_hash(32, 8)
+    from .programs import golden
+    prog = golden("hash32_8")  # the reference's gen_hash(32, 8)
     code = t.EnumCode(engine, t.EN_SEARCH, prog, spec="h = %d" % target)
     code.run()
     r = code.run()

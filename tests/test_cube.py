@@ -7,12 +7,13 @@ import pytest
 import tesseract_amd as t
 from oracle import pyoracle as O
 from tesseract_amd.cube import Perm8, Sort8
+from tests.helpers import golden
 
 
 def test_perm8_on_the_oracle_machine():
     # the generated PERM8 program itself, run by the oracle, moves byte k to dest[k]
     rng = np.random.default_rng(7)
-    prog = t.gen_perm8()
+    prog = golden("perm8")
     vals = rng.integers(0, 256, 512).astype(np.uint8)
     dest = rng.permutation(512)
     ctrl = t.benes_route(dest.astype(np.uint16))

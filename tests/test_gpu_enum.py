@@ -6,12 +6,13 @@ import pytest
 
 import tesseract_amd as t
 from oracle import pyoracle as O
+from tests.helpers import golden
 
 pytestmark = pytest.mark.gpu
 
 
 def test_adder8_known_answers(engine):
-    add = t.gen_adder(8)
+    add = golden("adder8")
     print("adder8 verify", flush=True)
     r = engine.enum_run(t.EN_VERIFY, add, spec="c:s = a + b + c")
     assert r["tried"] == 131072 and r["hits"] == 0 and r["space"] == 131072
@@ -33,7 +34,7 @@ def test_adder8_known_answers(engine):
 
 @pytest.mark.parametrize("bits", range(1, 9))
 def test_small_multipliers(engine, bits):
-    r = engine.enum_run(t.EN_VERIFY, t.gen_mul(bits), spec="p = a * b")
+    r = engine.enum_run(t.EN_VERIFY, golden("mul%d" % bits), spec="p = a * b")
     assert r["hits"] == 0 and r["tried"] == 1 << (2 * bits)
 
 
@@ -49,7 +50,7 @@ def _rev(x, bits):
 
 @pytest.mark.parametrize("bits", [3, 16])
 def test_hash_search(engine, bits):
-    prog = t.gen_hash(bits, 5)
+    prog = golden("hash%d_5" % bits)
     target = O.toyhash(12345 % (1 << bits), bits, 5)
     want = _brute_preimages(bits, 5, target)
     r = engine.enum_run(t.EN_SEARCH, prog, spec="h = %d" % target)
@@ -73,36 +74,78 @@ def test_equiv(engine):
         engine.enum_run(t.EN_EQUIV, x1, bad)
 
 
-def test_against_oracle_random(engine):
+def _random_io_program(rng, n_in, n_out, cells, length):
+    """A random NAND/STO program with n_in declared inputs and n_out outputs: inputs are read and
+    may be overwritten, outputs are plain cells written somewhere in the pass."""
+    b = t.Builder()
+    ins = [b.cell("x%d" % i) for i in range(n_in)]
+    work = [b.cell("w%d" % i) for i in range(cells)]
+    outs = [b.cell("o%d" % i) for i in range(n_out)]
+    readable = [0, 1] + ins + work + outs
+    writable = ins + work + outs
+    for c in ins:
+        b.input(c)
+    for _ in range(length):
+        if rng.random() < 0.55:
+            b.nand(readable[int(rng.integers(0, len(readable)))])
+        else:
+            b.sto(writable[int(rng.integers(0, len(writable)))])
+    for c in outs:
+        b.output(c)
+    return b.finish()
+
+
+def test_random_programs_against_oracle(engine):
+    # random programs with inputs and outputs, enumerated exhaustively on the device and by the
+    # oracle's reference machine: verify against a spec, equiv of two programs, search for a
+    # value; one and three ticks (state carried between ticks), whole and in shares
     rng = np.random.default_rng(5)
-    for k in range(6):
+    for k in range(8):
         n = int(rng.integers(3, 14))
-        prog = t.random_program(12, 200, 1000 + k)
-        # make inputs/outputs: a netlist is simpler; use the mul generator shape instead
-        mul = t.gen_mul(max(1, n // 2))
+        nout = int(rng.integers(1, 6))
+        a = _random_io_program(rng, n, nout, int(rng.integers(2, 30)), int(rng.integers(20, 400)))
+        bb = _random_io_program(rng, n, nout, int(rng.integers(2, 30)), int(rng.integers(20, 400)))
+        text = "o = x * 3 + 1"
+        spec = t.Spec(text, a)
+        want_tab = np.array([spec.value(x) for x in range(1 << n)], np.uint64)
+        cells = [spec.out_cell(i) for i in range(spec.width)]
         for ticks in (1, 3):
-            spec = t.Spec("p = a * b + 1", mul)
-            want_tab = np.array([spec.value(x) for x in range(1 << len(mul.inputs))], np.uint64)
-            cells = [spec.out_cell(b) for b in range(spec.width)]
-            o = O.enum(t.EN_VERIFY, mul, out_cells=cells, expect=want_tab, ticks=ticks)
-            g = engine.enum_run(t.EN_VERIFY, mul, spec="p = a * b + 1", ticks=ticks)
-            assert (g["hits"], g["hit"], g["tried"]) == (o["hits"], o["hit"], o["tried"])
+            o = O.enum(t.EN_VERIFY, a, out_cells=cells, expect=want_tab, ticks=ticks)
+            g = engine.enum_run(t.EN_VERIFY, a, spec=text, ticks=ticks)
+            assert (g["hits"], g["hit"], g["tried"]) == (o["hits"], o["hit"], o["tried"]), (k, ticks)
+            o = O.enum(t.EN_EQUIV, a, bb, b_in=bb.inputs, b_out=bb.outputs, ticks=ticks)
+            g = engine.enum_run(t.EN_EQUIV, a, bb, ticks=ticks)
+            assert (g["hits"], g["hit"], g["tried"]) == (o["hits"], o["hit"], o["tried"]), (k, ticks)
+        target = int(rng.integers(0, 1 << nout))
+        o = O.enum(t.EN_SEARCH, a, out_cells=cells, target=target)
+        g = engine.enum_run(t.EN_SEARCH, a, spec="o = %d" % target)
+        assert (g["hits"], g["hit"]) == (o["hits"], o["hit"]), k
+        parts = [engine.enum_run(t.EN_VERIFY, a, spec=text, part=q, part_end=q, parts=3)
+                 for q in range(3)]
+        o = O.enum(t.EN_VERIFY, a, out_cells=cells, expect=want_tab)
+        assert sum(p["hits"] for p in parts) == o["hits"]
+        assert sorted(sum((p["hit"] for p in parts), []))[:16] == o["hit"]
 
 
 def test_mul16_exhaustive(engine):
     print("mul16 verify over 2^32", flush=True)
-    r = engine.enum_run(t.EN_VERIFY, t.gen_mul(16), spec="p = a * b")
+    r = engine.enum_run(t.EN_VERIFY, golden("mul16"), spec="p = a * b")
     assert r["tried"] == 1 << 32 and r["hits"] == 0
     assert r["ops"] == 6040 * (1 << 32)
 
 
-def test_toyhash32_published_target(engine):
+def test_toyhash32_published_target(engine, ref):
+    # the published answer: exactly one preimage of 0x165c2a07 over 2^32 (README.md:67); the
+    # reference's own enum_run on the same program gives the same count and hit list
     print("toyhash32 search for 0x165c2a07 over 2^32", flush=True)
-    r = engine.enum_run(t.EN_SEARCH, t.gen_hash(32, 8), spec="h = 0x165c2a07")
+    prog = golden("hash32_8")
+    r = engine.enum_run(t.EN_SEARCH, prog, spec="h = 0x165c2a07")
     assert r["tried"] == 1 << 32
-    for x in r["hit"]:
-        assert O.toyhash(_rev(x, 32), 32, 8) == 0x165c2a07
-    assert r["hits"] == len(r["hit"]) and r["hits"] >= 1
+    assert r["hits"] == 1 and len(r["hit"]) == 1
+    assert O.toyhash(_rev(r["hit"][0], 32), 32, 8) == 0x165c2a07
+    print("the reference's enum_run over 2^32", flush=True)
+    w = ref.enum_run(t.EN_SEARCH, prog.ptr, spec_text="h = 0x165c2a07", spec_inputs=False, threads=16)
+    assert (r["tried"], r["hits"], r["hit"]) == (w["tried"], w["hits"], w["hit"])
 
 
 def test_mul16_counts_every_assignment(engine):
@@ -110,19 +153,26 @@ def test_mul16_counts_every_assignment(engine):
     # 2^32 assignments, so the count proves every assignment was evaluated and compared; a0 is
     # bit 31 of x and b0 bit 15, so the smallest are 2^31 + 2^15 + 0..15
     print("mul16 with a planted spec difference over 2^32", flush=True)
-    r = engine.enum_run(t.EN_VERIFY, t.gen_mul(16), spec="p = a * b + (a & b & 1)")
+    r = engine.enum_run(t.EN_VERIFY, golden("mul16"), spec="p = a * b + (a & b & 1)")
     assert r["tried"] == 1 << 32 and r["hits"] == 1 << 30
     assert r["hit"] == [(1 << 31) + (1 << 15) + k for k in range(16)]
 
 
-def test_toyhash32_search_random_target(engine):
+def test_toyhash32_search_random_target(engine, ref):
+    # a seeded target with a known preimage: every hit maps back to the target, and the count
+    # and the 16 smallest equal the reference's enum_run over all 2^32 assignments
     rng = np.random.default_rng(11)
     x = int(rng.integers(0, 1 << 32))
     target = O.toyhash(x, 32, 8)
-    r = engine.enum_run(t.EN_SEARCH, t.gen_hash(32, 8), spec="h = %d" % target)
+    prog = golden("hash32_8")
+    r = engine.enum_run(t.EN_SEARCH, prog, spec="h = %d" % target)
     assert _rev(x, 32) in r["hit"] or r["hits"] > 16
     for h in r["hit"]:
         assert O.toyhash(_rev(h, 32), 32, 8) == target
+    w = ref.enum_run(t.EN_SEARCH, prog.ptr, spec_text="h = %d" % target, spec_inputs=False,
+                     threads=16)
+    assert (r["tried"], r["hits"], r["hit"]) == (w["tried"], w["hits"], w["hit"])
+    assert r["hits"] >= 1
 
 
 def test_empty_program_enumeration(engine):

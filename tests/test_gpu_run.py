@@ -6,6 +6,7 @@ import pytest
 
 import tesseract_amd as t
 from oracle import pyoracle as O
+from tests.helpers import golden, life_cube
 
 pytestmark = pytest.mark.gpu
 
@@ -52,8 +53,8 @@ def random_cube_program(cells, length, dims, seed, links=24):
 
 @pytest.mark.parametrize("name", ["adder8", "mul4", "hash16", "rand20", "rand400"])
 def test_linkfree(engine, name):
-    prog = {"adder8": lambda: t.gen_adder(8), "mul4": lambda: t.gen_mul(4),
-            "hash16": lambda: t.gen_hash(16, 5), "rand20": lambda: t.random_program(20, 500, 42),
+    prog = {"adder8": lambda: golden("adder8"), "mul4": lambda: golden("mul4"),
+            "hash16": lambda: golden("hash16_5"), "rand20": lambda: t.random_program(20, 500, 42),
             "rand400": lambda: t.random_program(400, 3000, 7)}[name]()
     print("check", name, flush=True)
     check(engine, prog, groups=4, ticks=8, runs=(3, 5))
@@ -75,20 +76,20 @@ def test_random_cube(engine, dims):
 @pytest.mark.parametrize("dims", [9, 12])
 def test_rank(engine, dims):
     print("check rank", dims, flush=True)
-    check(engine, t.gen_rank(dims), runs=(1, 1))
+    check(engine, golden("rank%d" % dims), runs=(1, 1))
 
 
 def test_perm8_sort8(engine):
     print("check perm8/sort8", flush=True)
-    check(engine, t.gen_perm8(), runs=(2,))
-    check(engine, t.gen_sort8(), groups=2, runs=(1,))
+    check(engine, golden("perm8"), runs=(2,))
+    check(engine, golden("sort8"), groups=2, runs=(1,))
 
 
 def test_life_cube_small(engine):
-    prog, board = t.gen_life_cube(4, 4, 3, 3)  # 64 machines? no: dims = max(9, 6) -> one group
+    prog, board = life_cube(4, 4, 3, 3)  # 64 machines? no: dims = max(9, 6) -> one group
     print("check life cube 4x4 on 3+3 dims", flush=True)
     check(engine, prog, runs=(2,))
-    prog, board = t.gen_life_cube(4, 4, 6, 6)  # dims 12: cross-wave links, segments
+    prog, board = life_cube(4, 4, 6, 6)  # dims 12: cross-wave links, segments
     print("check life cube 4x4 on 6+6 dims", flush=True)
     check(engine, prog, runs=(1, 2))
 

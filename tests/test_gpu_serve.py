@@ -10,6 +10,7 @@ import pytest
 import tesseract_amd as t
 from oracle import pyoracle as O
 from tesseract_amd import netlists as N
+from tests.helpers import golden
 
 pytestmark = pytest.mark.gpu
 
@@ -37,7 +38,7 @@ def test_server_mix_equals_alone(engine):
         gates = [60, 300, 1500][k % 3]
         c, r, faulty = small_request(7, k, n_in, gates)
         reqs.append((t.net_parse(c.bench()), t.net_parse(r.bench()), c, r, faulty))
-    adder = t.gen_adder(8)
+    adder = golden("adder8")
     srv = t.Server(engine, streams=4, chunk_words=1 << 20)
     big_tk = srv.submit(t.EN_EQUIV, *big)
     while srv.poll(big_tk)[0] < 3:  # running
@@ -79,7 +80,7 @@ def test_server_mix_equals_alone(engine):
 
 def test_server_reports_errors(engine):
     srv = t.Server(engine)
-    rank = t.gen_rank(9)
+    rank = golden("rank9")
     tk = srv.submit(t.EN_VERIFY, rank, spec="p = x")
     with pytest.raises(ValueError, match="id and link cells"):
         srv.wait(tk)

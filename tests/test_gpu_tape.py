@@ -5,6 +5,7 @@ import pytest
 import tesseract_amd as t
 from tesseract_amd import netlists as N
 from oracle import pyoracle as O
+from tests.helpers import golden
 
 pytestmark = pytest.mark.gpu
 
@@ -51,7 +52,7 @@ def test_tape_large_space_and_parts(engine):
 
 def test_tape_search(engine):
     for bits, target in ((16, 0x1234), (24, 0xABCDEF)):
-        prog = t.gen_hash(bits, 5)
+        prog = golden("hash%d_5" % bits)
         spec = "h = %d" % target
         print("search", bits, flush=True)
         assert _same(engine.enum_run(t.EN_SEARCH, prog, spec=spec, tune=TAPE),

@@ -10,6 +10,7 @@ import pytest
 import tesseract_amd as t
 from oracle import pyoracle as O
 from oracle import pyref
+from tests.helpers import golden
 
 pytestmark = pytest.mark.gpu
 
@@ -120,7 +121,7 @@ def test_t1_device_and_host_runs_merge(engine, tmp_path):
 
 
 def test_enum_shares_on_device_merge_to_the_whole(engine, tmp_path):
-    prog = t.gen_mul(10)
+    prog = golden("mul10")
     whole = engine.enum_run(t.EN_VERIFY, prog, spec="p = a * b + 3")
     paths = []
     for k in range(3):

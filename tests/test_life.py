@@ -6,6 +6,7 @@ import pytest
 import tesseract_amd as t
 from oracle import pyoracle as O
 from tesseract_amd.life import LifeCube, splitmix64_stream
+from tests.helpers import life_cube
 
 
 class _CpuLife(LifeCube):
@@ -13,7 +14,7 @@ class _CpuLife(LifeCube):
 
     def __init__(self, w, h, rbits, cbits, seed=42):
         self.w, self.h, self.rbits, self.cbits = w, h, rbits, cbits
-        self.prog, self.board = t.gen_life_cube(w, h, rbits, cbits)
+        self.prog, self.board = life_cube(w, h, rbits, cbits)
         self.groups = self.prog.groups
         mem, self.rr = O.reset_lines(self.prog, self.groups)
         draws = splitmix64_stream(seed, self.groups * w * h * 8).reshape(self.groups, w * h, 8)
@@ -38,7 +39,7 @@ def test_front_end_on_the_oracle_machine():
 @pytest.mark.gpu
 def test_life_dims12(engine):
     print("life 4x4 tiles on 6+6 dims", flush=True)
-    life = LifeCube(engine, 4, 4, 6, 6)
+    life = LifeCube(engine, life_cube(4, 4, 6, 6)[0])
     start = life.start_bits()
     life.run(7)
     assert np.array_equal(life.board_bits(), O.life_words(start, 7))
@@ -46,10 +47,13 @@ def test_life_dims12(engine):
 
 @pytest.mark.gpu
 def test_life_8192_dims16(engine):
-    print("life 8192 x 8192 on 65,536 machines, 100 generations", flush=True)
+    # the configuration: 1,000 generations of the 8192 x 8192 torus, in two calls of uneven
+    # length (graph replays of both parities), against the oracle's bit-parallel C Life
+    print("life 8192 x 8192 on 65,536 machines, 1,000 generations", flush=True)
     life = LifeCube(engine)  # 32 x 32 tiles, rbits = cbits = 8
     start = life.start_bits()
     assert start.shape == (8192, 8192)
-    life.run(40)
-    life.run(60)
-    assert np.array_equal(life.board_bits(), O.life_words(start, 100))
+    life.run(333)
+    life.run(667)
+    print("oracle: 1,000 generations of C Life", flush=True)
+    assert np.array_equal(life.board_bits(), O.life_words(start, 1000))

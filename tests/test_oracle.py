@@ -6,6 +6,7 @@ import pytest
 
 import tesseract_amd as t
 from oracle import pyoracle as O
+from tests.helpers import golden
 
 
 def _cube_prog(cells, length, dims, seed):
@@ -57,7 +58,7 @@ def ref_ptr(prog):
 
 
 def test_run_lines_adder_closed_form():
-    add = t.gen_adder(8)
+    add = golden("adder8")
     mem, rr = O.reset_lines(add, 1)
     rng = np.random.default_rng(4)
     a = rng.integers(0, 256, 512)
@@ -76,7 +77,7 @@ def test_run_lines_adder_closed_form():
 
 
 def test_enum_known_answers():
-    add = t.gen_adder(8)
+    add = golden("adder8")
     spec = t.Spec("c:s = a + b + c", add)
     cells = [spec.out_cell(i) for i in range(spec.width)]
     x = np.arange(1 << 17, dtype=np.uint64)
@@ -91,7 +92,7 @@ def test_enum_known_answers():
 
 
 def test_enum_against_reference(ref):
-    mul = t.gen_mul(6)
+    mul = golden("mul6")
     theirs = ref.make("gen_mul", 6)
     for text in ("p = a * b", "p = a * b + 1", "p = a + b"):
         spec = t.Spec(text, mul)
@@ -153,7 +154,39 @@ def test_toyhash_published_target_form():
         xi, x1, x2 = x >> i & 1, x >> (i + 1) % 3 & 1, x >> (i + 2) % 3 & 1
         y |= (xi ^ ((1 - x1) & x2)) << i
     z = sum(((y >> (5 * i + 3) % 3) & 1) << i for i in range(3))
-    rc = t.toyhash(0, 3, 1) ^ O.toyhash(0, 3, 1)  # same constant both ways
-    assert rc == 0
     # toyhash(0) after one round = the round constant (all y bits are 0 for x = 0)
     assert O.toyhash(x, 3, 1) == z ^ O.toyhash(0, 3, 1)
+
+
+@pytest.mark.parametrize("bits", range(1, 8))
+def test_mul_programs_multiply(bits):
+    # the reference's gen_mul programs (program files) on the oracle machine: p = a * b for every
+    # assignment (a0 is input 0, the highest bit of x; b follows)
+    mul = golden("mul%d" % bits)
+    spec = t.Spec("p = a * b", mul)
+    n = 2 * bits
+    x = np.arange(1 << n, dtype=np.uint64)
+    a = sum(((x >> np.uint64(n - 1 - i)) & np.uint64(1)) << np.uint64(i) for i in range(bits))
+    b = sum(((x >> np.uint64(bits - 1 - i)) & np.uint64(1)) << np.uint64(i) for i in range(bits))
+    assert all(spec.value(int(v)) == int(a[v] * b[v]) for v in range(0, 1 << n, max(1, (1 << n) // 300)))
+    r = O.enum(t.EN_VERIFY, mul, out_cells=[spec.out_cell(i) for i in range(spec.width)], expect=a * b)
+    assert r["hits"] == 0 and r["tried"] == 1 << n
+
+
+@pytest.mark.parametrize("name,bits,rounds", [("hash12_4", 12, 4), ("hash16_5", 16, 5),
+                                              ("hash32_8", 32, 8)])
+def test_hash_programs_compute_toyhash(name, bits, rounds):
+    # the reference's gen_hash programs on the oracle machine against the closed form, at random
+    # inputs: input k (h's x bit k) is x's bit k, so machine value v hashes to toyhash(v)
+    prog = golden(name)
+    rng = np.random.default_rng(bits)
+    v = rng.integers(0, 1 << bits, 512, dtype=np.uint64)
+    mem, rr = O.reset_lines(prog, 1)
+    for k, cell in enumerate(prog.inputs):
+        mem[0, cell] = np.packbits(((v >> np.uint64(k)) & np.uint64(1)).astype(np.uint8),
+                                   bitorder="little").view(np.uint64)
+    mem, rr = O.run_lines(prog, mem, rr, 1)
+    h = np.zeros(512, np.uint64)
+    for k, cell in enumerate(prog.outputs):
+        h |= np.unpackbits(mem[0, cell].view(np.uint8), bitorder="little").astype(np.uint64) << np.uint64(k)
+    assert all(int(h[i]) == O.toyhash(int(v[i]), bits, rounds) for i in range(512))

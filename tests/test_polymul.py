@@ -1,7 +1,8 @@
 """The NTT polynomial product on the 512-machine cube (reference gen_polymul, src/gen.c:456-669):
-our generator gives the reference's program instruction for instruction and its twiddle table,
-and the oracle's product is the reference's polymul_scalar and obeys the ring's identities.
-The program itself is run through the oracle's reference machine on one cube. CPU only."""
+the program file holds the reference's program instruction for instruction and its twiddle table,
+the twiddles satisfy their closed form, and the oracle's product is the reference's polymul_scalar
+and obeys the ring's identities. The program itself is run through the oracle's reference machine
+on one cube, and on the GPU on many."""
 import ctypes as C
 
 import numpy as np
@@ -9,12 +10,13 @@ import pytest
 
 import tesseract_amd as t
 from oracle import pyoracle as O
+from tests.helpers import golden
 
 Q = 12289
 
 
 def test_polymul_program_is_the_references(ref):
-    ours = t.gen_polymul()
+    ours = golden("polymul")
     theirs = ref.make("gen_polymul")
     assert ours.len == theirs.s.len == 220758  # the count the reference quotes for this program
     oa, aa = ours.code()
@@ -24,8 +26,13 @@ def test_polymul_program_is_the_references(ref):
     assert ours.dims == 9
 
 
+def _twiddles():
+    p = golden("polymul")
+    return p.extra["zf"], p.extra["zi"]
+
+
 def test_twiddles_are_the_references(ref):
-    zf, zi = t.ntt_twiddles()
+    zf, zi = _twiddles()
     rf = np.zeros((9, 512), np.uint16)
     ri = np.zeros((9, 512), np.uint16)
     ref.lib().ntt_twiddles(rf.ctypes.data, ri.ctypes.data)
@@ -36,7 +43,7 @@ def test_twiddles_closed_form():
     # in Montgomery form: z * 2^14 mod q. The machines with bit d clear multiply by 1 on the way
     # back; psi^512 = -1 makes the round-8 forward twiddle of every machine psi^256 (a 4th root
     # of 1 squared to -1).
-    zf, zi = t.ntt_twiddles()
+    zf, zi = _twiddles()
     mont = (1 << 14) % Q
     minv = pow(mont, Q - 2, Q)
     for d in range(9):
@@ -83,7 +90,7 @@ def test_program_computes_the_product_on_the_reference_machine():
     a = rng.integers(0, Q, (1, 512)).astype(np.uint16)
     b = rng.integers(0, Q, (1, 512)).astype(np.uint16)
     pm = PolyMul.__new__(PolyMul)
-    pm.prog = t.gen_polymul()
+    pm.prog = golden("polymul")
     pm._cells()
     mem, rr = pm.lines(a, b)
     out, _ = O.run_lines(pm.prog, mem, rr, 1)
@@ -97,7 +104,7 @@ def test_reference_jit_computes_the_product(ref):
     a = rng.integers(0, Q, (1, 512)).astype(np.uint16)
     b = rng.integers(0, Q, (1, 512)).astype(np.uint16)
     pm = PolyMul.__new__(PolyMul)
-    pm.prog = t.gen_polymul()
+    pm.prog = golden("polymul")
     pm._cells()
     mem, _ = pm.lines(a, b)
     theirs = ref.make("gen_polymul")

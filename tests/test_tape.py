@@ -8,6 +8,7 @@ import pytest
 import tesseract_amd as t
 from tesseract_amd import netlists as N
 from oracle import pyoracle as O
+from tests.helpers import adder, golden
 
 PATTERNS = (0xAAAAAAAA, 0xCCCCCCCC, 0xF0F0F0F0, 0xFF00FF00, 0xFFFF0000)
 
@@ -81,7 +82,7 @@ def _rev(x, bits):
 def test_tape_search_matches_toyhash():
     # assignment x sets input k from bit n-1-k, so the hash input is x bit-reversed
     bits = 12
-    prog = t.gen_hash(bits, 4)
+    prog = golden("hash%d_4" % bits)
     h = [O.toyhash(_rev(x, bits), bits, 4) for x in range(1 << bits)]
     for target in (h[0], h[1234], 0xFFF):
         want = [x for x in range(1 << bits) if h[x] == target]
@@ -105,7 +106,7 @@ def test_tape_constant_hit_words():
 
 
 def test_tape_refuses_what_it_cannot_run():
-    prog = t.gen_adder(4)
+    prog = adder(4)
     with pytest.raises(ValueError, match="tape form"):
         t.enum_tape(t.EN_VERIFY, prog, spec="c:s = a + b")
     rng = np.random.default_rng(5)

@@ -5,6 +5,7 @@ tier, and the size limit."""
 import pytest
 
 import tesseract_amd as t
+from tests.helpers import golden
 
 pytestmark = pytest.mark.gpu
 
@@ -17,8 +18,8 @@ def same(a, b):
 def test_noalloc_equals_exec(engine):
     # first: mul10 against a spec it meets only on odd a and b (2^18 of 2^20 hits, 8,192 hit
     # words: more than the static list holds); second: mul6 with a few hits, a smaller input
-    big = t.EnumCode(engine, t.EN_VERIFY, t.gen_mul(10), spec="p = a * b + (a & b & 1)")
-    small = t.EnumCode(engine, t.EN_VERIFY, t.gen_mul(6), spec="p = a * b + 1")
+    big = t.EnumCode(engine, t.EN_VERIFY, golden("mul10"), spec="p = a * b + (a & b & 1)")
+    small = t.EnumCode(engine, t.EN_VERIFY, golden("mul6"), spec="p = a * b + 1")
     print("noalloc: mul10 planted difference, then mul6", flush=True)
     want_big, want_small = big.run(), small.run()
     assert want_big["hits"] == 1 << 18 and want_small["hits"] == 1 << 12
@@ -33,7 +34,7 @@ def test_noalloc_equals_exec(engine):
 
 
 def test_noalloc_tape_tier(engine):
-    prog = t.gen_hash(16, 5)
+    prog = golden("hash16_5")
     for spec in ("h = 12345", "h = 7"):
         code = t.EnumCode(engine, t.EN_SEARCH, prog, spec=spec, tune={"tape": 1})
         print("noalloc: tape search", spec, flush=True)
@@ -42,7 +43,7 @@ def test_noalloc_tape_tier(engine):
 
 
 def test_noalloc_rejects_a_larger_list(engine):
-    code = t.EnumCode(engine, t.EN_VERIFY, t.gen_mul(4), spec="p = a * b")
+    code = t.EnumCode(engine, t.EN_VERIFY, golden("mul4"), spec="p = a * b")
     with pytest.raises(ValueError, match="exceeds the static scratch"):
         code.run_noalloc(4097)
     assert code.run_noalloc(4096)["hits"] == 0

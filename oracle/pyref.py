@@ -141,6 +141,9 @@ def enum_run(mode, a_ptr, b_ptr=None, spec_text=None, spec_inputs=True, ticks=1,
              parts=1, threads=0):
     L = lib()
     err = C.create_string_buffer(512)
+    # any pointer to a layout-identical program (a reference Prog's or the engine's ts_program)
+    a_ptr = C.cast(a_ptr, C.c_void_p)
+    b_ptr = None if b_ptr is None else C.cast(b_ptr, C.c_void_p)
     spec = None
     if spec_text is not None:
         spec = L.spec_parse(spec_text.encode(), a_ptr, int(spec_inputs), err, len(err))

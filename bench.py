@@ -23,8 +23,11 @@ oracle/_ref by oracle/Makefile) on a bounded share of the same enumeration on th
 and adds beside its device figures a CPU baseline and the check of its results:
   life   Life B3/S23 on the 8192 x 8192 torus; the final board compared bit for bit with the
          oracle's bit-parallel C Life from the same start; that C Life timed as the baseline
-  batch  the random-program mix; the reference's t1_cycles on group 0 of the first programs
-  serve  the seeded equiv request stream; the reference's enum_run on the first requests
+  batch  the random-program mix; every program's group-0 cycle lengths against the reference's
+         own t1_cycles (timed: the CPU baseline), sampled groups above 0 (>= 50 programs of 4,096
+         cells among them) against the oracle's Brent
+  serve  the seeded request stream; the reference's enum_run on the first requests (at most
+         2^28 assignments each), every result compared with the device's
   tester1  tester 1 (gpu_t1_run) at the reference's defaults; the reference's own t1_run on the
          same programs on host threads, its summary compared with the device's
   polymul  the NTT polynomial product on many cubes; every cube's product checked against the
@@ -75,66 +78,141 @@ def cpu_baseline(threads, parts):
             "seconds_full_2p32_extrapolated": r["seconds"] * (1 << 32) / r["tried"]}
 
 
-def cpu_batch_baseline(n, seed, passes, seconds, threads):
-    """The reference's own t1_cycles (jit-compiled, oracle/_ref) on group 0 (512 machines) of the
-    programs of the batch mix, in order, on `threads` host threads for about `seconds`. Like the
-    reference's tester, each call compiles the program for its machines."""
-    import ctypes as C
-    import math
+def batch_sample_groups(progs, groups, seed=7, big=60, other=40):
+    """The group whose cycle lengths the batched run returns, per program: a random group above
+    0 for the first `big` programs of 4,096 cells of at most 20,000 instructions and `other`
+    further programs of other sizes (checked against the oracle's Brent from the continued
+    start stream), group 0 for the rest (checked against the reference's own t1_cycles)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    choice = [0] * len(progs)
+    if groups < 2:
+        return choice
+    big_k = [k for k, p in enumerate(progs) if p.ncells == 4098 and p.len <= 20000][:big]
+    rest = [k for k, p in enumerate(progs) if p.ncells != 4098 and p.len <= 20000]
+    other_k = list(rng.choice(rest, size=min(other, len(rest)), replace=False)) if rest else []
+    for k in big_k + other_k:
+        choice[int(k)] = int(rng.integers(1, groups))
+    return choice
+
+
+def batch_checks(keep, passes, threads):
+    """The batched run's cycle lengths against the reference and the oracle:
+      group 0 of a program: the reference's own t1_cycles (jit-compiled, oracle/_ref) on its own
+        random_program of the same seed, on `threads` host threads; those calls, timed, are the
+        CPU baseline (like the reference's tester, each compiles its program);
+      a sampled group g > 0: the oracle's Brent on that group's start state (the t1_start stream
+        continued), on `threads` host threads.
+    Returns (check fields, cpu_baseline)."""
+    import concurrent.futures as F
+    import numpy as np
     from oracle import pyref
-    import tesseract_amd as t
-    from tesseract_amd.workloads import SIZES
-    if not pyref.available():
-        return None
-    s = C.c_uint64(seed)
-    progs, seeds = [], []
-    for k in range(n):  # the mix of tesseract_amd.workloads.batch_mix, built by the reference
-        d0 = t.lib().ts_splitmix64(C.byref(s))
-        d1 = t.lib().ts_splitmix64(C.byref(s))
-        cells = SIZES[d0 % 5]
-        length = int(math.floor(10 ** (2 + 3 * ((d1 >> 11) / float(1 << 53)))))
-        ps = t.t1_program_seed(seed, cells, k)
-        progs.append(pyref.make("random_program", cells, length, ps))
-        seeds.append(ps)
-    done, ops, secs = pyref.t1_sample(progs, seeds, passes, threads, seconds)
-    return {"value": ops / secs, "unit": "1-bit ops/s", "cores": threads, "kind": "reference",
-            "sample": "reference t1_cycles on group 0 (512 machines) of the first %d programs of "
-                      "the mix, %.1f s" % (done, secs)}
+    from oracle import pyoracle as O
+    from tesseract_amd.workloads import t1_start_group
+    progs, seeds, stats, lg = keep["progs"], keep["seeds"], keep["stats"], keep["lam_groups"]
+    out = {}
+    ref_k = [k for k in range(len(progs)) if lg[k] == 0]
+    ora_k = [k for k in range(len(progs)) if lg[k] != 0]
+    if pyref.available():
+        def theirs(k):
+            p = progs[k]
+            rp = pyref.make("random_program", p.ncells - 2, p.len, seeds[k])
+            t0 = time.perf_counter()
+            lam, ran = pyref.t1_cycles(rp, seeds[k], passes)
+            return lam, ran, time.perf_counter() - t0
+        t0 = time.perf_counter()
+        bad, ops, busy = [], 0, 0.0
+        with F.ThreadPoolExecutor(threads) as pool:
+            for k, (lam, ran, secs) in zip(ref_k, pool.map(theirs, ref_k)):
+                if not np.array_equal(stats[k]["lam"], lam):
+                    bad.append(k)
+                ops += ran * progs[k].len * 512  # src/tester.c:305
+                busy += secs
+        wall = time.perf_counter() - t0
+        out["bit_exact_vs_reference"] = not bad
+        out["reference_checked"] = {"programs": len(ref_k), "machines": 512 * len(ref_k),
+                                    "differing_programs": bad[:10]}
+        base = {"value": ops / wall, "unit": "1-bit ops/s", "cores": threads, "kind": "reference",
+                "cpu_model": cpu_model(),
+                "sample": "the reference's t1_cycles (compile + cycle search) on group 0 (512 "
+                          "machines) of %d programs of the mix, %.1f s on %d threads"
+                          % (len(ref_k), wall, threads)}
+    else:
+        base = None
+
+    def ours(k):
+        mem, rr = t1_start_group(progs[k], seeds[k], lg[k])
+        return O.brent_lines(progs[k], mem, rr, passes)
+    bad = []
+    with F.ThreadPoolExecutor(threads) as pool:
+        for k, lam in zip(ora_k, pool.map(ours, ora_k)):
+            if not np.array_equal(stats[k]["lam"], lam):
+                bad.append(k)
+    out["bit_exact_vs_oracle"] = not bad
+    out["oracle_checked"] = {"programs": len(ora_k),
+                             "of_4096_cells": sum(progs[k].ncells == 4098 for k in ora_k),
+                             "groups": [int(lg[k]) for k in ora_k[:8]],
+                             "differing_programs": bad[:10]}
+    return out, base
 
 
-def cpu_serve_baseline(reqs, seconds, threads):
+def serve_checks(eng, reqs, results, seconds, threads):
     """The reference's own enum_run (its netlist parser and jit, oracle/_ref) on the requests in
     arrival order until `seconds` have passed. A request of more than 2^28 assignments runs share 0
-    of 2^(inputs - 28) (its --part), so that no single request overruns the bound. Timed: enum_run
-    (its jit compile and enumeration), as the device side is timed from a parsed request on;
-    the reference's netlist parsing is left out of the rate."""
+    of 2^(inputs - 28) (its --part), so that no single request overruns the bound. Each reference
+    result is compared with the device's: the served result itself when the share is the whole
+    request, else gpu_enum_run of the same share. Timed, as the CPU baseline: enum_run (its jit
+    compile and enumeration), as the device side is timed from a parsed request on; the
+    reference's netlist parsing is left out of the rate. Returns (check fields, cpu_baseline)."""
     import ctypes as C
     from oracle import pyref
     import tesseract_amd as t
     if not pyref.available():
-        return None
+        return {}, None
     t0 = time.perf_counter()
-    ops = done = tried = 0
+    ops = done = tried = whole = 0
     secs = 0.0
-    for a, b, c, r, n_in, faulty in reqs:
+    bad = []
+    for k, q in enumerate(reqs):
         if time.perf_counter() - t0 > seconds:
             break
-        ra, rb = pyref.Prog(), pyref.Prog()
-        e = C.create_string_buffer(512)
-        for prog, circ in ((ra, c), (rb, r)):
-            if not pyref.lib().net_parse(circ.bench().encode(), b"net", prog.ptr, e, len(e)):
-                raise ValueError(e.value.decode())
-        parts = 1 << max(0, n_in - 28)
-        t1 = time.perf_counter()
-        res = pyref.enum_run(t.EN_EQUIV, ra.ptr, rb.ptr, parts=parts, threads=threads)
+        parts = 1 << max(0, q["n_in"] - 28)
+        if q["mode"] == t.EN_EQUIV:  # the reference's own netlist loader on the same text
+            ra, rb = pyref.Prog(), pyref.Prog()
+            e = C.create_string_buffer(512)
+            for prog, circ in ((ra, q["c"]), (rb, q["r"])):
+                if not pyref.lib().net_parse(circ.bench().encode(), b"net", prog.ptr, e, len(e)):
+                    raise ValueError(e.value.decode())
+            t1 = time.perf_counter()
+            res = pyref.enum_run(t.EN_EQUIV, ra.ptr, rb.ptr, parts=parts, threads=threads)
+        else:  # the reference's own multiplier, the same program as the request's
+            ra = pyref.make("gen_mul", q["n_in"] // 2)
+            t1 = time.perf_counter()
+            res = pyref.enum_run(t.EN_VERIFY, ra.ptr, spec_text=q["spec"], parts=parts,
+                                 threads=threads)
         secs += time.perf_counter() - t1
+        if parts == 1:
+            dev = results[k]
+            whole += 1
+        else:
+            dev = eng.enum_run(q["mode"], q["a"], q["b"], spec=q["spec"], part=0, part_end=0,
+                               parts=parts)
+        if (dev["tried"], dev["hits"], dev["hit"]) != (res["tried"], res["hits"], res["hit"]):
+            bad.append(k)
         ops += res["ops"]
         tried += res["tried"]
         done += 1
-    return {"value": ops / secs, "unit": "1-bit ops/s", "cores": threads, "kind": "reference",
+    checks = {"bit_exact_vs_reference": not bad,
+              "reference_checked": {"requests": done, "whole_requests": whole,
+                                    "share_0_of_larger": done - whole, "assignments": tried,
+                                    "hits": int(sum(results[k]["hits"] for k in range(done))),
+                                    "differing_requests": bad[:10]}}
+    base = {"value": ops / secs, "unit": "1-bit ops/s", "cores": threads, "kind": "reference",
+            "cpu_model": cpu_model(),
             "sample": "reference enum_run (jit compile + enumeration) on the first %d requests in "
                       "order (at most 2^28 assignments of each: %d in all), %.1f s of enum_run in "
                       "%.1f s" % (done, tried, secs, time.perf_counter() - t0)}
+    return checks, base
 
 
 def life_check(life, gens):
@@ -228,11 +306,16 @@ def run_workload(args):
                 print(json.dumps(out), flush=True)
                 sys.exit("life: the final board differs from the oracle's")
     elif args.workload == "batch":
-        out = W.run_batch(eng, args.programs, args.groups, args.passes, args.seed)
-        if not args.no_cpu:  # the configuration's 10,000 programs, as many as fit the budget
-            out["cpu_baseline"] = cpu_batch_baseline(max(args.programs, 10000), args.seed,
-                                                     args.passes, args.cpu_seconds,
-                                                     args.cpu_threads)
+        out = W.run_batch(eng, args.programs, args.groups, args.passes, args.seed,
+                          lam_groups=lambda progs: batch_sample_groups(progs, args.groups),
+                          keep=keep)
+        if not args.no_cpu:
+            checks, base = batch_checks(keep, args.passes, args.cpu_threads)
+            out.update(checks)
+            out["cpu_baseline"] = base
+            if not (checks.get("bit_exact_vs_reference", True) and checks["bit_exact_vs_oracle"]):
+                print(json.dumps(out), flush=True)
+                sys.exit("batch: cycle lengths differ from the reference's or the oracle's")
     elif args.workload == "polymul":
         out = W.run_polymul(eng, args.cubes, args.seed, keep=keep)
         if not args.no_cpu:
@@ -246,8 +329,13 @@ def run_workload(args):
     else:
         out = W.run_serve(eng, args.requests, args.rate, args.seed, keep=keep)
         if not args.no_cpu:
-            out["cpu_baseline"] = cpu_serve_baseline(keep["requests"], args.cpu_seconds,
-                                                     args.cpu_threads)
+            checks, base = serve_checks(eng, keep["requests"], keep["results"], args.cpu_seconds,
+                                        args.cpu_threads)
+            out.update(checks)
+            out["cpu_baseline"] = base
+            if not checks.get("bit_exact_vs_reference", True):
+                print(json.dumps(out), flush=True)
+                sys.exit("serve: results differ from the reference's")
     out["device"] = eng.describe()
     print(json.dumps(out), flush=True)
 

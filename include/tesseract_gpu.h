@@ -163,6 +163,8 @@ typedef struct {
                                lines (group-major, the reference's LaneMachine.mem) and rr, groups
                                lines; cells ZERO and ONE keep their constants */
     const ts_line    *rr;
+    unsigned          lam_group, lam_groups;  /* lam covers groups [lam_group, lam_group +
+                               lam_groups) only (lam_groups x 512 entries); lam_groups 0 = all */
 } gpu_batch_item;
 
 typedef struct {

@@ -59,7 +59,8 @@ class GpuCodeInfo(C.Structure):
 
 class GpuBatchItem(C.Structure):
     _fields_ = [("prog", C.POINTER(ProgramStruct)), ("seed", C.c_uint64),
-                ("lam", C.POINTER(C.c_uint64)), ("mem", C.c_void_p), ("rr", C.c_void_p)]
+                ("lam", C.POINTER(C.c_uint64)), ("mem", C.c_void_p), ("rr", C.c_void_p),
+                ("lam_group", C.c_uint), ("lam_groups", C.c_uint)]
 
 
 class GpuBatchStats(C.Structure):
@@ -666,14 +667,16 @@ class Engine:
         return float(lib().gpu_last_ms(self._e))
 
     def batch_run(self, progs, seeds, groups=2048, max_passes=64, plain=False, lds_cells=0,
-                  waves_per_cu=0, lds_large_cells=0, lams=False, starts=None):
+                  waves_per_cu=0, lds_large_cells=0, lams=False, starts=None, lam_groups=None):
         """gpu_batch_run: per program a dict(hist, lam_sum, longest, passes), with lams also "lam"
         (machine 512 g + k's cycle length, 0 = none found); plus the run info. starts: per program
-        None (t1_start of its seed) or (mem, rr) as uint64 arrays (groups, ncells, 8), (groups, 8)."""
+        None (t1_start of its seed) or (mem, rr) as uint64 arrays (groups, ncells, 8), (groups, 8).
+        lam_groups: per program a group g, whose 512 cycle lengths alone come back (lams=True)."""
         import numpy as np
         n = len(progs)
         items = (GpuBatchItem * n)()
-        lam = [np.zeros(groups * 512, np.uint64) for _ in range(n)] if lams else None
+        per = 512 if lam_groups is not None else groups * 512
+        lam = [np.zeros(per, np.uint64) for _ in range(n)] if lams else None
         keep = []
         for i, (p, sd) in enumerate(zip(progs, seeds)):
             items[i].prog = p.ptr
@@ -687,6 +690,8 @@ class Engine:
                 items[i].mem, items[i].rr = m.ctypes.data, r.ctypes.data
             if lams:
                 items[i].lam = lam[i].ctypes.data_as(C.POINTER(C.c_uint64))
+                if lam_groups is not None:
+                    items[i].lam_group, items[i].lam_groups = int(lam_groups[i]), 1
         tune = GpuBatchTune(groups, max_passes, int(plain), lds_cells, waves_per_cu, lds_large_cells)
         stats = (GpuBatchStats * n)()
         info = GpuBatchInfo()

@@ -62,6 +62,7 @@ struct BatchProg {
     uint64_t seed;       // t1_start stream seed
     uint64_t lam_off;    // first of the program's per-machine cycle lengths, or kNoLam
     uint64_t state_off;  // first u64 of the program's start state, or kNoLam (t1_start stream)
+    uint32_t lam_w0, lam_w1;  // words [lam_w0, lam_w1) have their cycle lengths written
 };
 
 constexpr uint64_t kNoLam = ~0ull;
@@ -285,9 +286,9 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
                 const uint32_t found = active & ~d;
                 if (__any(found)) {
                     record(st, found, lam, lane);
-                    if (P.lam_off != kNoLam)  // machine 32 word + b of the program
-                        for (uint32_t m = found; m; m &= m - 1)
-                            lams[P.lam_off + (uint64_t)word * 32 + __builtin_ctz(m)] = lam;
+                    if (P.lam_off != kNoLam && word >= P.lam_w0 && word < P.lam_w1)
+                        for (uint32_t m = found; m; m &= m - 1)  // machine 32 word + b
+                            lams[P.lam_off + (uint64_t)(word - P.lam_w0) * 32 + __builtin_ctz(m)] = lam;
                 }
                 active &= d;
                 const uint64_t searching = __ballot(active != 0);
@@ -512,6 +513,11 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
         }
         if (!check_program(p, kLanes * 4, err, errlen))
             return 0;
+        if (items[i].lam_groups &&
+            (uint64_t)items[i].lam_group + items[i].lam_groups > tune.groups) {
+            say(err, errlen, "%s", "the cycle lengths asked for are of groups the run does not have");
+            return 0;
+        }
         if (p->ncells < 3) {
             say(err, errlen, "%s", "a batched program needs at least one writable cell");
             return 0;
@@ -631,8 +637,12 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
             }
         }
         if (items[i].lam) {
+            const uint32_t g1 = items[i].lam_groups ? items[i].lam_group + items[i].lam_groups
+                                                    : tune.groups;
+            bp.lam_w0 = items[i].lam_groups ? items[i].lam_group * 16 : 0;
+            bp.lam_w1 = g1 * 16;
             bp.lam_off = lam_total;
-            lam_total += (uint64_t)prog_words * 32;
+            lam_total += (uint64_t)(bp.lam_w1 - bp.lam_w0) * 32;
         }
         bp.first_item = (uint32_t)item_of[cls].size();
         for (uint32_t k = 0; k < per_prog; k++)
@@ -779,7 +789,8 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     for (size_t i = 0; i < n; i++)
         if (items[i].lam)
             BCHK(hipMemcpy(items[i].lam, d_lams.p + progs[i].lam_off,
-                           (size_t)prog_words * 32 * sizeof(uint64_t), hipMemcpyDeviceToHost));
+                           (size_t)(progs[i].lam_w1 - progs[i].lam_w0) * 32 * sizeof(uint64_t),
+                           hipMemcpyDeviceToHost));
     for (size_t i = 0; i < n; i++) {
         memcpy(stats_out[i].hist, &st[i * kStats], 64 * 8);
         stats_out[i].lam_sum = st[i * kStats + 64];

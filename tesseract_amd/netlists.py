@@ -1,6 +1,7 @@
-"""Seeded random NAND netlists, NAND-level rewrites of them and injected gate faults: the circuits
-of the serving configuration, written as ISCAS .bench text for the netlist loader (the reference's
-net.c reads the same format, src/net.c:110-357).
+"""Seeded random NAND netlists, NAND-level rewrites of them and injected gate faults: the equiv
+circuits of the serving configuration, written as ISCAS .bench text for the netlist loader (the
+reference's net.c reads the same format, src/net.c:110-357); and its verify requests, multipliers
+against their spec.
 
 circuit(n_in, n_out, gates, rng): gate k = NAND of two earlier signals with the locality logic has:
 the first from the most recent 64 signals, the second a primary input (1 in 10) or one of the most
@@ -95,3 +96,18 @@ def request(seed, k):
     if faulty:
         r = fault(r, rng)
     return c, r, n_in, faulty
+
+
+# Verify requests of the stream: arithmetic circuits (the reference's gen_mul multipliers, 4 to 10
+# bits a side: input spaces 2^8..2^20) checked against their spec; one in ten carries a planted
+# spec difference (a product off by one where a and b are both odd).
+VERIFY_EVERY = 5
+
+
+def verify_request(seed, k):
+    """(program name, spec, n_inputs, planted) of verify request k."""
+    rng = np.random.default_rng([seed, k, 1])
+    bits = int(rng.integers(4, 11))
+    planted = rng.random() < 0.1
+    spec = "p = a * b + (a & b & 1)" if planted else "p = a * b"
+    return "mul%d" % bits, spec, 2 * bits, planted

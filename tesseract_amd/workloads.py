@@ -48,15 +48,41 @@ def batch_mix(n, seed):
     return progs, seeds
 
 
+def t1_start_group(prog, seed, g):
+    """Group g's start state in the t1_start stream of `seed` (group 0 is the reference's
+    t1_start, src/tester.c:83-91; later groups continue the stream, cells then RR, group after
+    group): (mem (ncells, 8), rr (8,)) u64 lines, computed directly at the stream's offset."""
+    per = (prog.ncells - 2) * 8 + 8
+    k = np.arange(g * per + 1, (g + 1) * per + 1, dtype=np.uint64)
+    base = np.uint64((seed ^ 0x5DEECE66D) & (2 ** 64 - 1))
+    with np.errstate(over="ignore"):
+        z = base + np.uint64(0x9E3779B97F4A7C15) * k
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    mem = np.zeros((prog.ncells, 8), np.uint64)
+    mem[1] = ~np.uint64(0)
+    mem[2:] = z[:-8].reshape(prog.ncells - 2, 8)
+    return mem, z[-8:].copy()
+
+
 def run_batch(engine, n, groups, passes, seed, plain_too=True, lds_cells=0, waves_per_cu=0,
-              lds_large_cells=0):
+              lds_large_cells=0, lam_groups=None, keep=None):
+    """The mix batched (and, with plain_too, one launch per program). lam_groups: per program
+    the group whose 512 cycle lengths the batched run returns (for checks); keep, a dict, then
+    receives the programs, seeds and per-program results."""
     progs, seeds = batch_mix(n, seed)
+    if callable(lam_groups):
+        lam_groups = lam_groups(progs)
     out = {"workload": "batch", "programs": n, "machines_per_program": groups * 512,
            "passes": passes, "instructions": sum(p.len for p in progs)}
     t0 = time.perf_counter()
     stats, info = engine.batch_run(progs, seeds, groups=groups, max_passes=passes,
                                    lds_cells=lds_cells, waves_per_cu=waves_per_cu,
-                                   lds_large_cells=lds_large_cells)
+                                   lds_large_cells=lds_large_cells,
+                                   lams=lam_groups is not None, lam_groups=lam_groups)
+    if keep is not None:
+        keep.update(progs=progs, seeds=seeds, stats=stats, lam_groups=lam_groups)
     out["batched_wall_s"] = time.perf_counter() - t0
     out["batched_ms"] = info["ms"]
     out["ops"] = info["ops"]
@@ -72,7 +98,8 @@ def run_batch(engine, n, groups, passes, seed, plain_too=True, lds_cells=0, wave
         out["plain_ops_per_s"] = info_p["ops"] / (info_p["ms"] * 1e-3)
         out["launches_plain"] = info_p["launches"]
         out["batched_over_plain"] = info_p["ms"] / info["ms"]
-        out["same_results_alone_and_batched"] = stats == stats_p
+        strip = lambda L: [{k: v for k, v in d.items() if k != "lam"} for d in L]
+        out["same_results_alone_and_batched"] = strip(stats) == strip(stats_p)
     hist = np.zeros(64, np.uint64)
     for st in stats:
         hist += np.array(st["hist"], np.uint64)
@@ -185,22 +212,38 @@ def run_hash(engine, target=0x165C2A07):
             "ms": engine.last_ms, "ops_per_s": r["ops"] / (engine.last_ms * 1e-3)}
 
 
-def run_serve(engine, n, rate, seed, streams=4, chunk_words=1 << 24, check=True, keep=None,
-              tape_race_ms=0.0):
-    """A seeded Poisson stream of equiv requests (netlists.request) submitted at their arrival
-    times; latency = done - arrival. Offered load is the stream's; served load is what finished."""
-    import threading
+def serve_requests(seed, n):
+    """The stream's requests: dicts of mode, a, b (equiv) or spec (verify), n_in, faulty, and for
+    equiv the circuits c, r themselves. Every VERIFY_EVERY-th request is a verify."""
     from . import netlists as N
-    rng = np.random.default_rng(seed)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    from .programs import golden
     reqs = []
     t_gen = time.perf_counter()
     for k in range(n):
-        c, r, n_in, faulty = N.request(seed, k)
-        reqs.append((t.net_parse(c.bench()), t.net_parse(r.bench()), c, r, n_in, faulty))
+        if k % N.VERIFY_EVERY == N.VERIFY_EVERY - 1:
+            name, spec, n_in, planted = N.verify_request(seed, k)
+            reqs.append({"mode": t.EN_VERIFY, "a": golden(name), "b": None, "spec": spec,
+                         "program": name, "n_in": n_in, "faulty": planted})
+        else:
+            c, r, n_in, faulty = N.request(seed, k)
+            reqs.append({"mode": t.EN_EQUIV, "a": t.net_parse(c.bench()),
+                         "b": t.net_parse(r.bench()), "spec": None, "c": c, "r": r,
+                         "n_in": n_in, "faulty": faulty})
         if (k + 1) % 50 == 0:
             print("serve: %d requests generated (%.1f s)" % (k + 1, time.perf_counter() - t_gen),
                   file=sys.stderr, flush=True)
+    return reqs
+
+
+def run_serve(engine, n, rate, seed, streams=4, chunk_words=1 << 24, check=True, keep=None,
+              tape_race_ms=0.0):
+    """A seeded Poisson stream of equiv and verify requests (serve_requests) submitted at their
+    arrival times; latency = done - arrival. Offered load is the stream's; served load is what
+    finished."""
+    import threading
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    reqs = serve_requests(seed, n)
     srv = t.Server(engine, streams=streams, chunk_words=chunk_words, tape_race_ms=tape_race_ms)
     tickets = [None] * n
     t0 = time.perf_counter() + 0.05
@@ -210,7 +253,8 @@ def run_serve(engine, n, rate, seed, streams=4, chunk_words=1 << 24, check=True,
             delay = t0 + arrivals[k] - time.perf_counter()
             if delay > 0:
                 time.sleep(delay)
-            tickets[k] = srv.submit(t.EN_EQUIV, reqs[k][0], reqs[k][1])
+            q = reqs[k]
+            tickets[k] = srv.submit(q["mode"], q["a"], q["b"], spec=q["spec"])
 
     th = threading.Thread(target=feeder)
     th.start()
@@ -221,14 +265,15 @@ def run_serve(engine, n, rate, seed, streams=4, chunk_words=1 << 24, check=True,
     first = min(tm["submitted_s"] for tm in times)
     srv.close()
     lat = np.array([tm["done_s"] - tm["submitted_s"] for tm in times])
-    n_in = np.array([q[4] for q in reqs])
+    n_in = np.array([q["n_in"] for q in reqs])
     out = {"workload": "serve", "requests": n, "offered_rate_per_s": rate,
            "served_rate_per_s": n / (end - first), "wall_s": end - first,
            "ops": int(sum(r["ops"] for r in results)),
            "ops_per_s": sum(r["ops"] for r in results) / (end - first),
            "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
            "compile_p50_ms": float(np.median([tm["compiled_s"] - tm["submitted_s"] for tm in times]) * 1e3),
-           "faulty": int(sum(q[5] for q in reqs)), "mismatching": int(sum(r["hits"] > 0 for r in results)),
+           "verify_requests": int(sum(q["mode"] == t.EN_VERIFY for q in reqs)),
+           "faulty": int(sum(q["faulty"] for q in reqs)), "mismatching": int(sum(r["hits"] > 0 for r in results)),
            "tape_tier_requests": int(sum(tm["tier"] == 1 for tm in times)),
            "tape_then_compiled_requests": int(sum(tm["tier"] == 2 for tm in times))}
     classes = {}
@@ -241,13 +286,19 @@ def run_serve(engine, n, rate, seed, streams=4, chunk_words=1 << 24, check=True,
     out["latency_by_input_space"] = classes
     if keep is not None:
         keep["requests"] = reqs
+        keep["results"] = results
     if check:
+        # fault-free pairs and specs have no mismatch; every counterexample separates the two
+        # netlists; a planted spec difference hits exactly where a and b are both odd
         bad = 0
-        for (a, b, c, r, ni, faulty), res in zip(reqs, results):
-            if not faulty and res["hits"]:
+        for q, res in zip(reqs, results):
+            if not q["faulty"] and res["hits"]:
                 bad += 1
-            for x in res["hit"]:
-                bad += c.evaluate(x) == r.evaluate(x)
+            if q["mode"] == t.EN_EQUIV:
+                for x in res["hit"]:
+                    bad += q["c"].evaluate(x) == q["r"].evaluate(x)
+            elif q["faulty"]:
+                bad += res["hits"] != 1 << (q["n_in"] - 2)
         out["verdicts_checked"] = bad == 0
     return out
 

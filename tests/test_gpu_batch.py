@@ -220,3 +220,32 @@ def test_batch_matches_reference_t1_cycles(engine, ref):
         found = [int(x) for x in lam if x]
         assert g == {"hist": hist, "lam_sum": sum(found), "longest": max(found, default=0),
                      "passes": ran}, (p.ncells, p.len)
+
+
+def test_batch_mix_10000_matches_reference(engine, ref):
+    # the batching configuration's 10,000 programs (cells 16..4096, 10^2..10^5 instructions), one
+    # 512-machine group each, 64 passes: every machine's cycle length and every program's passes
+    # equal the reference's own t1_cycles (its jit-compiled code, src/tester.c:96-164) on the same
+    # program and seed, the programs run by 16 host threads
+    import concurrent.futures as F
+    from tesseract_amd.workloads import batch_mix
+    progs, seeds = batch_mix(10000, 1)
+    print("batch: 10,000 programs, %d instructions" % sum(p.len for p in progs), flush=True)
+    got, info = engine.batch_run(progs, seeds, groups=1, max_passes=64, lams=True)
+    print("device %.1f ms; the reference's t1_cycles on 16 threads" % info["ms"], flush=True)
+
+    def theirs(k):
+        p = progs[k]
+        rp = ref.make("random_program", p.ncells - 2, p.len, seeds[k])
+        assert rp.s.len == p.len and rp.s.ncells == p.ncells
+        return ref.t1_cycles(rp, seeds[k], 64)
+
+    bad = []
+    with F.ThreadPoolExecutor(16) as pool:
+        for k, (lam, ran) in enumerate(pool.map(theirs, range(len(progs)))):
+            if not (np.array_equal(got[k]["lam"], lam) and got[k]["passes"] == ran):
+                bad.append(k)
+            if k % 2000 == 1999:
+                print("%d programs compared" % (k + 1), flush=True)
+    assert not bad, "programs differing: %s" % bad[:10]
+    assert info["ops"] == sum(p.len * g["passes"] * 512 for p, g in zip(progs, got))

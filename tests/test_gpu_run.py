@@ -217,3 +217,13 @@ def test_power_on_state(engine, dims):
     st.reset()
     got_mem, got_rr = st.get()
     assert np.array_equal(got_mem, want_mem) and np.array_equal(got_rr, want_rr)
+
+
+@pytest.mark.parametrize("name", ["cube_any", "cube_rank", "cube_max", "rule110", "adder8",
+                                  "counter4", "mux", "xor", "full_adder", "c17", "perm8", "sort8"])
+def test_reference_examples(engine, name):
+    # the reference's own example and cube-library programs (examples/*.asm, lib/cube.inc through
+    # its assembler): the check protocol against the oracle, whole passes over several calls
+    prog = golden(name)
+    print("check", name, prog.len, "instructions, dims", prog.dims, flush=True)
+    check(engine, prog, groups=2 * prog.groups, runs=(1, 2))

@@ -611,3 +611,14 @@ def test_enum_validation_messages():
         t.enum_source(t.EN_VERIFY, golden("rank9"), spec="p = x")
     with pytest.raises(ValueError, match="bad --part"):
         t.enum_source(t.EN_VERIFY, golden("mul2"), spec="p = a * b", part=2, part_end=1, parts=3)
+
+
+def test_t1_start_group_continues_the_stream():
+    # the start state of one group, computed at its offset in the splitmix64 stream, equals the
+    # library's t1_start of all groups up to it (group 0: the reference's t1_start)
+    from tesseract_amd.workloads import t1_start_group
+    p = t.random_program(9, 40, 5)
+    mem, rr = t.t1_start(p, 1234, groups=5)
+    for g in range(5):
+        m, r = t1_start_group(p, 1234, g)
+        assert np.array_equal(m, mem[g]) and np.array_equal(r, rr[g]), g

@@ -190,3 +190,15 @@ def test_hash_programs_compute_toyhash(name, bits, rounds):
     for k, cell in enumerate(prog.outputs):
         h |= np.unpackbits(mem[0, cell].view(np.uint8), bitorder="little").astype(np.uint64) << np.uint64(k)
     assert all(int(h[i]) == O.toyhash(int(v[i]), bits, rounds) for i in range(512))
+
+
+@pytest.mark.parametrize("name", ["cube_any", "cube_rank", "cube_max", "rule110", "perm8"])
+def test_run_lines_on_reference_examples(ref, name):
+    # the oracle machine on the reference's own cube programs against its ref_cube_tick
+    prog = golden(name)
+    mem, rr = O.random_lines(prog, prog.groups, 3)
+    want = ref.cube_run(ref_ptr(prog), prog.ncells, mem, rr, 2)
+    got = O.run_lines(prog, mem, rr, 2)
+    writable = [c for c in range(2, prog.ncells) if prog.kind(c) == t.CK_PLAIN]
+    assert np.array_equal(got[1], want[1])
+    assert np.array_equal(got[0][:, writable], want[0][:, writable])

@@ -435,6 +435,13 @@ def main():
     sha16 = hashlib.sha256(t.enum_source(t.EN_VERIFY, prog, spec="p = a * b").encode()).hexdigest()[:16]
     traffic, valu_insts = profiled(sha16)
     achieved = valu_insts * 64 / (kavg * 1e-3) if valu_insts else None
+    # the same kernel shape with a planted spec difference (a * b + (a & b & 1)): 2^30 of the 2^32
+    # assignments mismatch, every one counted and listed through the wave-level hit reports
+    planted = t.EnumCode(eng, t.EN_VERIFY, prog, spec="p = a * b + (a & b & 1)")
+    planted.run()
+    rp = planted.run()
+    assert rp["hits"] == 1 << 30 and rp["hit"][0] == (1 << 31) + (1 << 15)
+    planted_ms = eng.last_ms
     line = {
         "metric": "1-bit NAND/STO ops/s on 1 GPU (% int-VALU peak); mul16 2^32 verify sec",
         "value": value,
@@ -452,7 +459,10 @@ def main():
                                "spec p = a * b, all 2^32 assignments",
                    "program_instructions": prog.len, "assignments": 1 << 32,
                    "verify_seconds": ms_per_step / 1e3, "hits": 0,
-                   "compile_seconds": compile_s, "device": eng.describe()},
+                   "planted_spec_difference": {"spec": "p = a * b + (a & b & 1)",
+                                               "hits": rp["hits"], "kernel_ms": planted_ms},
+                   "compile_seconds": compile_s, "device": eng.describe(),
+                   "scheduling": code.stats()},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_LANE_OPS,
                      "unit": "32-bit VALU lane-op/s",
                      "frac": achieved / PEAK_LANE_OPS if achieved else None, "traffic": traffic,

@@ -42,6 +42,13 @@ typedef struct {
                               2 for a resident program, which caps it at 256 VGPRs, the
                               compiler's choice for segments and enumerations; 1 = the
                               compiler's choice everywhere) */
+    unsigned max_waves;    /* enumerations: at most this many waves per SIMD run at once (each
+                              workgroup reserves LDS it does not use; 0 = no cap) */
+    unsigned sched;        /* the compiler's instruction scheduling: 1 = its default,
+                              2 = iterative-ilp, 3 = max-ilp, 4 = iterative-minreg; 0 = auto: an
+                              enumeration of at least 2^40 1-bit ops is compiled with 1, 2 and 3,
+                              each timed on a sample of its words, and the fastest kept;
+                              otherwise the default */
 } gpu_tune;
 
 typedef struct {
@@ -132,6 +139,17 @@ int  gpu_enum_compile(gpu_engine *e, const gpu_enum *en, gpu_enum_code **out, ch
                       size_t errlen);
 int  gpu_enum_exec(gpu_enum_code *c, ts_enum_result *r, char *err, size_t errlen);
 void gpu_enum_free(gpu_enum_code *c);
+/* How a compiled enumeration was built: its compile time, the tape tier or a kernel, the
+   scheduling strategy kept (gpu_tune.sched numbering) and, when it was chosen by timing, each
+   candidate's time on the sample (ms, 0 = not tried). */
+typedef struct {
+    double   compile_seconds;
+    unsigned tape;
+    unsigned sched;
+    double   sched_ms[5];
+    uint64_t sample_words;
+} gpu_enum_info;
+int  gpu_enum_stats(const gpu_enum_code *c, gpu_enum_info *info);
 /* gpu_enum_exec that allocates nothing: no counterpart in the reference. For callers that cannot
    allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
    hipMalloc). The hit counters and the hit list live in a fixed file-scope device array of

@@ -48,7 +48,13 @@ class EnumResultStruct(C.Structure):
 
 class GpuTune(C.Structure):
     _fields_ = [("block", C.c_uint), ("slice_nodes", C.c_uint), ("lut3", C.c_uint), ("grid", C.c_uint),
-                ("graph_passes", C.c_uint), ("tape", C.c_uint), ("waves", C.c_uint)]
+                ("graph_passes", C.c_uint), ("tape", C.c_uint), ("waves", C.c_uint),
+                ("max_waves", C.c_uint), ("sched", C.c_uint)]
+
+
+class GpuEnumInfo(C.Structure):
+    _fields_ = [("compile_seconds", C.c_double), ("tape", C.c_uint), ("sched", C.c_uint),
+                ("sched_ms", C.c_double * 5), ("sample_words", C.c_uint64)]
 
 
 class GpuCodeInfo(C.Structure):
@@ -206,6 +212,7 @@ def lib():
             "gpu_enum_exec": (i32, [vp, C.POINTER(EnumResultStruct), C.c_char_p, sz]),
             "gpu_enum_exec_noalloc": (i32, [vp, C.c_uint, C.POINTER(EnumResultStruct), C.c_char_p,
                                             sz]),
+            "gpu_enum_stats": (i32, [vp, C.POINTER(GpuEnumInfo)]),
             "gpu_enum_free": (None, [vp]),
             "gpu_batch_records": (i32, [P, C.c_uint, C.c_uint, C.POINTER(C.c_uint32), sz,
                                         C.POINTER(C.c_size_t), C.c_char_p, sz]),
@@ -752,6 +759,14 @@ class EnumCode:
         if not lib().gpu_enum_exec(self._c, C.byref(r), e, len(e)):
             raise RuntimeError(e.value.decode())
         return _result(r)
+
+    def stats(self):
+        """gpu_enum_stats: compile_seconds, tape, sched (the scheduling strategy kept), sched_ms
+        (each timed candidate's sample time), sample_words."""
+        i = GpuEnumInfo()
+        lib().gpu_enum_stats(self._c, C.byref(i))
+        return {"compile_seconds": i.compile_seconds, "tape": i.tape, "sched": i.sched,
+                "sched_ms": list(i.sched_ms), "sample_words": i.sample_words}
 
     def run_noalloc(self, list_words=0):
         """gpu_enum_exec_noalloc: the same enumeration with the hit list in the library's static

@@ -355,6 +355,39 @@ std::string xl_expr(const std::string &v, uint32_t x)
 }
 
 // Common device helpers, emitted once per module.
+// Hits, reported a wave at a time. Each lane counts its hits in a register (*acc) and the wave
+// adds its total to out[0] with one atomic at the end (ts_count_hits); a word with hits takes a
+// list entry (word index, 32-bit hit mask), one atomic on out[1] reserving the wave's entries.
+// A wave that finds the list full (out[1] at or past the capacity: the caller lists the range
+// again in parts) makes sure out[1] is past it and stops reserving (*full, the same in every
+// lane). Every lane of the wave takes part; a wave without hits only ballots.
+const char *kReportHits = R"(TS_DEV void ts_report_hits(u64 *out, u64 *list, u64 cap, u64 gw, u32 hit, u32 *acc, u32 *full) {
+  const u64 who = __ballot(hit != 0u);
+  if (!who) return;
+  *acc += __builtin_popcount(hit);
+  if (*full) return;
+  const u64 n = (u64)__builtin_popcountll(who);
+  const int leader = __builtin_ctzll(who);
+  u64 base = 0;
+  if ((int)(threadIdx.x & 63u) == leader) {
+    const u64 seen = __atomic_load_n(&out[1], __ATOMIC_RELAXED);
+    if (seen < cap) base = atomicAdd(&out[1], n);
+    else { base = cap; if (seen == cap) atomicAdd(&out[1], 1ull); }
+  }
+  base = __shfl(base, leader);
+  if (base + n > cap) *full = 1u;
+  if (hit) {
+    const u64 k = base + __builtin_amdgcn_mbcnt_hi((u32)(who >> 32), __builtin_amdgcn_mbcnt_lo((u32)who, 0u));
+    if (k < cap) { list[2 * k] = gw; list[2 * k + 1] = hit; }
+  }
+}
+TS_DEV void ts_count_hits(u64 *out, u32 acc) {
+  u64 sum = acc;
+  for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+  if ((threadIdx.x & 63u) == 0u && sum) atomicAdd(&out[0], sum);
+}
+)";
+
 const char *kPrelude = R"(typedef unsigned int u32;
 typedef unsigned long long u64;
 #define TS_DEV __device__ static inline __attribute__((always_inline))
@@ -655,6 +688,10 @@ struct SpecCode {
     std::string prelude;                 // per word: the var bases
     std::vector<std::string> lane_expr;  // per lane
     bool u32 = false;
+    // The result's low bits that no lane bit reaches: with only +, -, *, &, |, ^, ~, negation and
+    // left shifts by constants, bit i of the result depends on bits <= i of the operands, so bits
+    // below the lowest vector bit fed by a lane (x bits 0..4) are the same in all 32 lanes.
+    int lane_free = 0;
 };
 
 bool lower_spec(const ts_spec *sp, SpecCode &sc, std::string &err)
@@ -671,6 +708,18 @@ bool lower_spec(const ts_spec *sp, SpecCode &sc, std::string &err)
             u32 = false;
     }
     sc.u32 = u32;
+    bool low_monotone = true;
+    for (int i = 0; i < nr; i++)
+        if (rpn[i].op == TS_T_SHR || (rpn[i].op == TS_T_SHL && (i == 0 || rpn[i - 1].op != TS_T_NUM)))
+            low_monotone = false;
+    if (low_monotone) {
+        int m = 64;
+        for (int k = 0; k < nv; k++)
+            for (int b = 0; b < ts_spec_var_bits(sp, k); b++)
+                if (ts_spec_var_pos(sp, k, b) < 5)
+                    m = std::min(m, b);
+        sc.lane_free = std::min(m, width);
+    }
     const char *T = u32 ? "u32" : "u64";
     std::ostringstream pre;
     for (int k = 0; k < nv; k++) {
@@ -853,32 +902,48 @@ bool codegen_enum(const ts_program *a, const ts_program *b, const int *in_map, c
     }
 
     std::ostringstream o;
-    o << kPrelude;
+    o << kPrelude << kReportHits;
     o << "TS_DEV u32 ts_shl32(u32 a, u32 s) { return s < 32u ? a << s : 0u; }\n"
          "TS_DEV u64 ts_shl64(u64 a, u64 s) { return s < 64ull ? a << s : 0ull; }\n"
          "TS_DEV u64 ts_shr64(u64 a, u64 s) { return s < 64ull ? a >> s : 0ull; }\n";
+    // Word indices: 32 bits while the space has at most 2^36 assignments (i1 <= 2^31, so a
+    // round's base plus the grid's stride stays below 2^32), else 64. The loop is wave-uniform: every lane of a wave takes the same
+    // rounds (a lane past i1 has no valid assignment), so the hit report's cross-lane steps see
+    // every lane.
+    const bool narrow = n <= 36;
+    const char *W = narrow ? "u32" : "u64";
     o << "extern \"C\" __global__ void __launch_bounds__(" << opt.block << ")"
       << (opt.waves > 1 ? " __attribute__((amdgpu_waves_per_eu(" + std::to_string(opt.waves) + ")))"
                         : std::string())
-      << " ts_enum(u64* __restrict__ out, u64* __restrict__ list, u64 cap, u64 i0, u64 i1, "
+      << " ts_enum(u64* __restrict__ out, u64* __restrict__ list, u64 cap, u64 i0_, u64 i1_, "
          "u64 space, u32 parts, u32 part, u32 nshare) {\n"
-         "  for (u64 i = i0 + (u64)blockIdx.x * "
-      << opt.block << "u + threadIdx.x; i < i1; i += (u64)gridDim.x * " << opt.block
+      << "  const " << W << " i0 = (" << W << ")i0_, i1 = (" << W << ")i1_;\n"
+      << "  u32 hits = 0u, full = 0u;  // this lane's hits; the hit list is full\n"
+      << "  for (" << W << " base = i0 + (" << W << ")blockIdx.x * " << opt.block
+      << "u + (threadIdx.x & ~63u); base < i1; base += (" << W << ")gridDim.x * " << opt.block
       << "u) {\n"
-         "    u64 gw = i;\n"
+      << "    const " << W << " i = base + (threadIdx.x & 63u);\n"
+      << "    " << W << " gw = i;\n"
          "    if (parts > 1u) {  // active chunk (64 passes of 512 = 1024 words) -> chunk index\n"
-         "      const u64 ci = i >> 10, q = ci / nshare, r = part + ci % nshare;\n"
-         "      gw = ((q * parts + r) << 10) | (i & 1023u);\n"
+      << "      const " << W << " ci = i >> 10, q = ci / nshare, r = part + ci % nshare;\n"
+      << "      gw = ((q * parts + r) << 10) | (i & 1023u);\n"
          "    }\n"
-         "    const u64 xb = gw << 5;\n"
-         "    if (xb >= space) continue;\n"
-         "    const u32 valid = space - xb >= 32ull ? 0xffffffffu : ((1u << (u32)(space - xb)) - 1u);\n";
-    // input bit j of x: j < 5 lane pattern, else a bit of the word index
+         "    const u64 xb = (u64)gw << 5;\n";
+    if (n >= 5)  // the space is a multiple of 32: a word is wholly valid or wholly not
+        o << "    const u32 valid = i < i1 && xb < space ? 0xffffffffu : 0u;\n";
+    else
+        o << "    const u32 valid = i >= i1 || xb >= space ? 0u\n"
+             "                      : space - xb >= 32ull ? 0xffffffffu : ((1u << (u32)(space - xb)) - 1u);\n";
+    // input bit j of x: j < 5 lane pattern, else a bit of the word index (a sign-extended
+    // one-bit field: 0 or all ones)
     static const uint32_t pat[5] = {0xAAAAAAAAu, 0xCCCCCCCCu, 0xF0F0F0F0u, 0xFF00FF00u, 0xFFFF0000u};
     auto inbit = [&](uint32_t j) -> std::string {
         if (j < 5)
             return hex32(pat[j]);
-        return "(0u - (u32)((gw >> " + std::to_string(j - 5) + ") & 1ull))";
+        if (narrow || j - 5 < 32)
+            return "(u32)__builtin_amdgcn_sbfe((int)(u32)gw, " + std::to_string(j - 5) + "u, 1u)";
+        return "(u32)__builtin_amdgcn_sbfe((int)(u32)((u64)gw >> 32), " + std::to_string(j - 37) +
+               "u, 1u)";
     };
 
     std::vector<Lit> ra, rb;
@@ -997,24 +1062,39 @@ bool codegen_enum(const ts_program *a, const ts_program *b, const int *in_map, c
         const char *T = sc.u32 ? "u32" : "u64";
         for (int l = 0; l < 32; l++)
             o << "    const " << T << " e" << l << " = " << sc.lane_expr[l] << ";\n";
-        o << "    u32 miss = 0u;\n";
+        // mismatches gathered in four accumulators, one v_bitop3_b32 per bit: m | (t ^ o), truth
+        // table 0xF6 (bit 4 S0 + 2 S1 + S2 of the table is the result for inputs S0, S1, S2)
+        o << "    u32 miss0 = 0u, miss1 = 0u, miss2 = 0u, miss3 = 0u;\n";
+        int acc = 0;
+        auto gather = [&](const std::string &t, const std::string &out) {
+            o << "    miss" << acc << " = __builtin_amdgcn_bitop3_b32(miss" << acc << ", " << t
+              << ", " << out << ", 0xF6u);\n";
+            acc = (acc + 1) % 4;
+        };
+        // lane-free low bits: every lane expects bit `bit` of e0
+        for (int bit = 0; bit < sc.lane_free; bit++)
+            gather("(u32)__builtin_amdgcn_sbfe((int)(u32)(e0" + std::string(bit >= 32 ? " >> 32" : "") +
+                       "), " + std::to_string(bit % 32) + "u, 1u)",
+                   oa[bit]);
+        // the rest from the 32 x 32 transpose of the expected values (the compiler drops the
+        // transpose work of rows no compare reads)
         for (int half = 0; half < (el.width > 32 ? 2 : 1); half++) {
+            if (std::max(sc.lane_free, 32 * half) >= std::min(el.width, 32 * half + 32))
+                continue;
             o << "    {\n";
             for (int l = 0; l < 32; l++)
                 o << "    u32 t" << l << " = (u32)(e" << l << (half ? " >> 32" : "") << ");\n";
             emit_transpose(o, "t");
-            for (int bit = 32 * half; bit < el.width && bit < 32 * half + 32; bit++)
-                o << "    miss |= t" << bit - 32 * half << " ^ " << oa[bit] << ";\n";
+            for (int bit = std::max(sc.lane_free, 32 * half); bit < el.width && bit < 32 * half + 32;
+                 bit++)
+                gather("t" + std::to_string(bit - 32 * half), oa[bit]);
             o << "    }\n";
         }
-        o << "    hit = miss & valid;\n";
+        o << "    hit = (miss0 | miss1 | miss2 | miss3) & valid;\n";
     }
-    o << "    if (hit) {\n"
-         "      atomicAdd(&out[0], (u64)__builtin_popcount(hit));\n"
-         "      const u64 k = atomicAdd(&out[1], 1ull);\n"
-         "      if (k < cap) { list[2 * k] = gw; list[2 * k + 1] = hit; }\n"
-         "    }\n"
-         "  }\n}\n";
+    o << "    ts_report_hits(out, list, cap, (u64)gw, hit, &hits, &full);\n"
+         "  }\n"
+         "  ts_count_hits(out, hits);\n}\n";
     ks.names.push_back("ts_enum");
     ks.text = o.str();
     return true;

@@ -13,6 +13,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "lower.hpp"
@@ -92,8 +93,18 @@ gpu_tune defaults(const gpu_engine *e, const gpu_tune *t)
 }  // namespace
 
 // hipRTC: source -> gfx950 code object (needs no device)
+const char *ts_internal::sched_strategy(unsigned sched)
+{
+    switch (sched) {
+    case 2: return "iterative-ilp";
+    case 3: return "max-ilp";
+    case 4: return "iterative-minreg";
+    default: return nullptr;  // the compiler's default
+    }
+}
+
 int ts_internal::rtc_compile(const std::string &src, const std::string &arch_name, std::vector<char> &code,
-                double *seconds, char *err, size_t errlen)
+                double *seconds, char *err, size_t errlen, unsigned sched)
 {
     auto t0 = std::chrono::steady_clock::now();
     hiprtcProgram prog;
@@ -103,9 +114,11 @@ int ts_internal::rtc_compile(const std::string &src, const std::string &arch_nam
         return 0;
     }
     std::string arch = "--offload-arch=" + arch_name;
+    const char *strategy = sched_strategy(sched);
+    const std::string sched_opt = std::string("-amdgpu-sched-strategy=") + (strategy ? strategy : "");
     const char *opts[] = {arch.c_str(), "-O3", "-Xclang", "-target-feature", "-Xclang",
-                          "-packed-fp32-ops"};
-    hiprtcResult r = hiprtcCompileProgram(prog, 6, opts);
+                          "-packed-fp32-ops", "-mllvm", sched_opt.c_str()};
+    hiprtcResult r = hiprtcCompileProgram(prog, strategy ? 8 : 6, opts);
     if (r != HIPRTC_SUCCESS) {
         size_t n = 0;
         hiprtcGetProgramLogSize(prog, &n);
@@ -128,10 +141,10 @@ int ts_internal::rtc_compile(const std::string &src, const std::string &arch_nam
 namespace {
 
 int build_module(gpu_engine *e, const std::string &src, hipModule_t *mod, double *seconds, char *err,
-                 size_t errlen)
+                 size_t errlen, unsigned sched = 0)
 {
     std::vector<char> code;
-    if (!ts_internal::rtc_compile(src, e->arch, code, seconds, err, errlen))
+    if (!ts_internal::rtc_compile(src, e->arch, code, seconds, err, errlen, sched))
         return 0;
     HIPCHK(hipSetDevice(e->device));
     HIPCHK(hipModuleLoadData(mod, code.data()));
@@ -304,7 +317,7 @@ extern "C" int gpu_compile(gpu_engine *e, const ts_program *p, const gpu_tune *t
     if (listing)
         *listing = strdup(ks.text.c_str());
     double secs = 0;
-    if (!build_module(e, ks.text, &c->mod, &secs, err, errlen))
+    if (!build_module(e, ks.text, &c->mod, &secs, err, errlen, c->tune.sched))
         return 0;
     for (const std::string &name : ks.names) {
         hipFunction_t f;
@@ -952,7 +965,106 @@ struct gpu_enum_code {
     uint64_t      len_total = 0;  // instructions per pass of a (+ b for equiv)
     uint64_t     *d_out = nullptr, *d_list = nullptr;
     double        compile_seconds = 0;
+    unsigned      sched = 1;          // the scheduling strategy of the kept kernel
+    double        sched_ms[5] = {0, 0, 0, 0, 0};
+    uint64_t      sample_words = 0;
 };
+
+namespace {
+
+// the hit list of an enumeration code, allocated once and kept with it
+int enum_buffers(gpu_enum_code *c, char *err, size_t errlen)
+{
+    if (c->d_out)
+        return 1;
+    if (hipMalloc(&c->d_out, 2 * sizeof(uint64_t)) != hipSuccess ||
+        hipMalloc(&c->d_list, 2 * kListCap * sizeof(uint64_t)) != hipSuccess) {
+        if (c->d_out)
+            (void)hipFree(c->d_out);
+        c->d_out = nullptr;
+        say(err, errlen, "hipMalloc failed for the hit list");
+        return 0;
+    }
+    return 1;
+}
+
+// Compiles the enumeration kernel with scheduling strategies 1, 2 and 3 (in parallel host
+// threads), times each on the first `sample` words of the share (a warm-up launch, then the best
+// of two) and keeps the fastest module in c.
+int enum_autotune(gpu_enum_code *c, const std::string &src, uint64_t sample, char *err,
+                  size_t errlen)
+{
+    gpu_engine *e = c->e;
+    const unsigned cand[3] = {1, 2, 3};
+    std::vector<char> code[3];
+    std::string msg[3];
+    double secs[3] = {0, 0, 0};
+    int ok[3] = {0, 0, 0};
+    {
+        std::vector<std::thread> th;
+        for (int k = 0; k < 3; k++)
+            th.emplace_back([&, k] {
+                char m[1024] = "";
+                ok[k] = ts_internal::rtc_compile(src, e->arch, code[k], &secs[k], m, sizeof m, cand[k]);
+                msg[k] = m;
+            });
+        for (std::thread &t : th)
+            t.join();
+    }
+    HIPCHK(hipSetDevice(e->device));
+    if (!enum_buffers(c, err, errlen))
+        return 0;
+    hipModule_t mods[3] = {nullptr, nullptr, nullptr};
+    double best = 0;
+    int keep = -1;
+    ShareInfo si = ts_internal::shares(c->n, c->part, c->part_end, c->parts);
+    for (int k = 0; k < 3; k++) {
+        if (!ok[k] || hipModuleLoadData(&mods[k], code[k].data()) != hipSuccess)
+            continue;
+        hipFunction_t fn;
+        if (hipModuleGetFunction(&fn, mods[k], "ts_enum") != hipSuccess)
+            continue;
+        uint64_t *d_out = c->d_out, *d_list = c->d_list, capv = kListCap, i0 = 0, i1 = sample;
+        uint64_t space = si.space;
+        uint32_t parts = (uint32_t)c->parts, part = (uint32_t)c->part, nshare = si.nshare;
+        void *args[] = {&d_out, &d_list, &capv, &i0, &i1, &space, &parts, &part, &nshare};
+        const gpu_tune &tune = c->tune;
+        unsigned grid = (unsigned)std::max<uint64_t>(
+            1, std::min<uint64_t>((sample + tune.block - 1) / tune.block, tune.grid));
+        double ms_best = 0;
+        for (int rep = 0; rep < 3; rep++) {
+            HIPCHK(hipEventRecord(e->ev0, e->stream));
+            HIPCHK(hipModuleLaunchKernel(fn, grid, 1, 1, tune.block, 1, 1, 0, e->stream, args,
+                                         nullptr));
+            HIPCHK(hipEventRecord(e->ev1, e->stream));
+            HIPCHK(hipEventSynchronize(e->ev1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+            if (rep && (ms_best == 0 || ms < ms_best))
+                ms_best = ms;
+        }
+        c->sched_ms[cand[k]] = ms_best;
+        if (keep < 0 || ms_best < best) {
+            best = ms_best;
+            keep = k;
+        }
+    }
+    for (int k = 0; k < 3; k++)
+        if (mods[k] && k != keep)
+            (void)hipModuleUnload(mods[k]);
+    if (keep < 0) {
+        say(err, errlen, "%s", msg[0].empty() ? "no scheduling strategy gave a kernel" : msg[0].c_str());
+        return 0;
+    }
+    c->mod = mods[keep];
+    c->sched = cand[keep];
+    c->sample_words = sample;
+    c->compile_seconds = secs[0] + secs[1] + secs[2];
+    HIPCHK(hipModuleGetFunction(&c->fn, c->mod, "ts_enum"));
+    return 1;
+}
+
+}  // namespace
 
 
 extern "C" int gpu_enum_compile(gpu_engine *e, const gpu_enum *en, gpu_enum_code **out, char *err,
@@ -987,12 +1099,31 @@ extern "C" int gpu_enum_compile(gpu_engine *e, const gpu_enum *en, gpu_enum_code
         KernelSource ks;
         if (!ts_internal::enum_prepare(e, en, c->tune, ks, err, errlen))
             return 0;
-        if (!build_module(e, ks.text, &c->mod, &c->compile_seconds, err, errlen))
-            return 0;
-        if (hipModuleGetFunction(&c->fn, c->mod, "ts_enum") != hipSuccess) {
-            fail();
-            say(err, errlen, "hipModuleGetFunction failed");
-            return 0;
+        c->part = en->part;
+        c->part_end = en->part_end;
+        c->parts = en->parts;
+        c->n = (unsigned)en->a->ninputs;
+        c->ticks = en->ticks;
+        c->len_total = en->a->len + (en->mode == TS_EN_EQUIV ? en->b->len : 0);
+        // long enumerations choose their scheduling strategy by timing a sample of their words
+        const ShareInfo si = ts_internal::shares(c->n, c->part, c->part_end, c->parts);
+        const double work = (double)si.tried * (double)c->len_total * c->ticks;
+        if (c->tune.sched == 0 && work >= 1099511627776.0) {  // 2^40 1-bit ops
+            const uint64_t sample =
+                std::min<uint64_t>(si.nact_words, std::max<uint64_t>(1u << 20, si.nact_words / 16));
+            if (!enum_autotune(c.get(), ks.text, sample, err, errlen)) {
+                fail();
+                return 0;
+            }
+        } else {
+            c->sched = std::max(1u, c->tune.sched);
+            if (!build_module(e, ks.text, &c->mod, &c->compile_seconds, err, errlen, c->sched))
+                return 0;
+            if (hipModuleGetFunction(&c->fn, c->mod, "ts_enum") != hipSuccess) {
+                fail();
+                say(err, errlen, "hipModuleGetFunction failed");
+                return 0;
+            }
         }
     }
     c->part = en->part;
@@ -1061,8 +1192,15 @@ struct EnumExec {
             uint64_t blocks = (i1 - i0 + tune.block - 1) / tune.block;
             unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, tune.grid));
             void *args[] = {&d_out, &d_list, &capv, &i0, &i1, &space, &parts, &part, &nshare};
-            if (!launch(e, c->fn, grid, tune.block, args, err, errlen))
-                return 0;
+            // a cap on resident waves: LDS the kernel does not use, reserved per workgroup so
+            // that only max_waves waves per SIMD fit a CU's LDS
+            unsigned lds = 0;
+            if (tune.max_waves) {
+                const unsigned per_cu = std::max(1u, tune.max_waves * 4 / (tune.block / 64));
+                lds = (unsigned)std::min<size_t>(e->lds_per_block, e->lds_per_block / per_cu + 1024);
+            }
+            HIPCHK(hipModuleLaunchKernel(c->fn, grid, 1, 1, tune.block, 1, 1, lds, e->stream, args,
+                                         nullptr));
         }
         if (timed)
             HIPCHK(hipEventRecord(e->ev1, e->stream));
@@ -1136,20 +1274,25 @@ struct EnumExec {
 
 }  // namespace
 
+extern "C" int gpu_enum_stats(const gpu_enum_code *c, gpu_enum_info *info)
+{
+    memset(info, 0, sizeof *info);
+    info->compile_seconds = c->compile_seconds;
+    info->tape = c->tape;
+    info->sched = c->tape ? 0 : c->sched;
+    for (int k = 0; k < 5; k++)
+        info->sched_ms[k] = c->sched_ms[k];
+    info->sample_words = c->sample_words;
+    return 1;
+}
+
 extern "C" int gpu_enum_exec(gpu_enum_code *c, ts_enum_result *r, char *err, size_t errlen)
 {
     memset(r, 0, sizeof *r);
     HIPCHK(hipSetDevice(c->e->device));
-    if (!c->d_out) {  // the hit list, allocated on the first execution and kept with the code
-        if (hipMalloc(&c->d_out, 2 * sizeof(uint64_t)) != hipSuccess ||
-            hipMalloc(&c->d_list, 2 * kListCap * sizeof(uint64_t)) != hipSuccess) {
-            if (c->d_out)
-                (void)hipFree(c->d_out);
-            c->d_out = nullptr;
-            say(err, errlen, "hipMalloc failed for the hit list");
-            return 0;
-        }
-    }
+    // the hit list, allocated on the first execution (or by the strategy timing) and kept
+    if (!enum_buffers(c, err, errlen))
+        return 0;
     return ts_internal::enum_exec(c, {c->d_out, c->d_list, kListCap}, r, err, errlen);
 }
 

@@ -57,9 +57,11 @@ int enum_device(const gpu_enum_code *c);
 // current, taken once per device when an engine opens
 int static_scratch_init(int device, char *err, size_t errlen);
 
-// HIP source -> code object for arch (no device needed)
+// HIP source -> code object for arch (no device needed). sched picks the compiler's instruction
+// scheduling strategy: 0 or 1 its default, 2 iterative-ilp, 3 max-ilp, 4 iterative-minreg.
 int rtc_compile(const std::string &src, const std::string &arch, std::vector<char> &code,
-                double *seconds, char *err, size_t errlen);
+                double *seconds, char *err, size_t errlen, unsigned sched = 0);
+const char *sched_strategy(unsigned sched);  // the strategy's name, nullptr for the default
 
 // validation with the reference's messages, spec parsing and code generation of an enumeration
 int enum_prepare(const gpu_engine *e, const gpu_enum *en, gpu_tune &tune, ts::KernelSource &ks,

@@ -57,6 +57,57 @@ __device__ inline uint32_t word(const uint4 (&d)[5 * kQ], unsigned f, unsigned j
     return (j & 3) == 0 ? q.x : (j & 3) == 1 ? q.y : (j & 3) == 2 ? q.z : q.w;
 }
 
+// Hits, reported a wave at a time (every lane of the wave takes part). Each lane counts its hits in
+// a register (*acc) and the wave adds its total to out[0] with one atomic at the end (count_hits);
+// a word with hits takes a list entry (word index, 32-bit hit mask), one atomic on out[1]
+// reserving the wave's entries. A wave that finds the list full (out[1] at or past the capacity)
+// makes sure out[1] is past it and stops reserving (*full, the same in every lane). A wave
+// without hits only ballots.
+__device__ inline void report_hits(uint64_t *out, uint64_t *list, uint64_t cap, uint64_t gw,
+                                   uint32_t hit, uint32_t *acc, uint32_t *full)
+{
+    const uint64_t who = __ballot(hit != 0u);
+    if (!who)
+        return;
+    *acc += __builtin_popcount(hit);
+    if (*full)
+        return;
+    const uint64_t n = (uint64_t)__builtin_popcountll(who);
+    const int leader = __builtin_ctzll(who);
+    uint64_t base = 0;
+    if ((int)(threadIdx.x & 63u) == leader) {
+        const uint64_t seen = __atomic_load_n(&out[1], __ATOMIC_RELAXED);
+        if (seen < cap) {
+            base = atomicAdd((unsigned long long *)&out[1], (unsigned long long)n);
+        } else {
+            base = cap;
+            if (seen == cap)
+                atomicAdd((unsigned long long *)&out[1], 1ull);
+        }
+    }
+    base = __shfl(base, leader);
+    if (base + n > cap)
+        *full = 1u;
+    if (hit) {
+        const uint64_t k =
+            base + __builtin_amdgcn_mbcnt_hi((uint32_t)(who >> 32),
+                                             __builtin_amdgcn_mbcnt_lo((uint32_t)who, 0u));
+        if (k < cap) {
+            list[2 * k] = gw;
+            list[2 * k + 1] = hit;
+        }
+    }
+}
+
+__device__ inline void count_hits(uint64_t *out, uint32_t acc)
+{
+    uint64_t sum = acc;
+    for (int off = 32; off > 0; off >>= 1)
+        sum += __shfl_xor(sum, off);
+    if ((threadIdx.x & 63u) == 0u && sum)
+        atomicAdd((unsigned long long *)&out[0], (unsigned long long)sum);
+}
+
 // a wave's entries of one group: every read, then every write
 __device__ inline void run_share(const uint4 (&d)[5 * kQ], char *my)
 {
@@ -82,6 +133,7 @@ __global__ void __launch_bounds__(64 * kWaves)
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     char *const my = (char *)file + lane * 4u;
     const uint32_t lane_zero = lane & zero;
+    uint32_t hits = 0, full = 0;  // this lane's hits (wave 0 only); the hit list is full
     if (wave == 0)
         *(uint32_t *)my = 0u;  // slot 0: the constant
     // The waves run the same rounds (the bounds depend on the workgroup only; a lane past i1
@@ -124,17 +176,12 @@ __global__ void __launch_bounds__(64 * kWaves)
         if (wave == 0) {
             const uint32_t hit =
                 (*(const uint32_t *)(my + (root & 0x7fffffffu)) ^ (0u - (root >> 31))) & valid;
-            if (hit) {
-                atomicAdd((unsigned long long *)&out[0], (unsigned long long)__builtin_popcount(hit));
-                const uint64_t k = atomicAdd((unsigned long long *)&out[1], 1ull);
-                if (k < cap) {
-                    list[2 * k] = gw;
-                    list[2 * k + 1] = hit;
-                }
-            }
+            report_hits(out, list, cap, gw, hit, &hits, &full);
         }
         __syncthreads();  // the root is read before the next round's leaves are written
     }
+    if (wave == 0)
+        count_hits(out, hits);
 }
 
 }  // namespace

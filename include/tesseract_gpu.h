@@ -49,6 +49,11 @@ typedef struct {
                               enumeration of at least 2^40 1-bit ops is compiled with 1, 2 and 3,
                               each timed on a sample of its words, and the fastest kept;
                               otherwise the default */
+    unsigned group;        /* enumerations: 1 = a thread takes one word of 32 assignments at a
+                              time; 2 = groups of 32 words, the part of the program no lane bit
+                              (x bits 0..4) reaches evaluated once per group, bit-sliced over its
+                              words; 0 = auto (groups for one tick, at least 24 inputs and at
+                              least 32 such LUTs) */
 } gpu_tune;
 
 typedef struct {

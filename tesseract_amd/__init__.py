@@ -49,7 +49,7 @@ class EnumResultStruct(C.Structure):
 class GpuTune(C.Structure):
     _fields_ = [("block", C.c_uint), ("slice_nodes", C.c_uint), ("lut3", C.c_uint), ("grid", C.c_uint),
                 ("graph_passes", C.c_uint), ("tape", C.c_uint), ("waves", C.c_uint),
-                ("max_waves", C.c_uint), ("sched", C.c_uint)]
+                ("max_waves", C.c_uint), ("sched", C.c_uint), ("group", C.c_uint)]
 
 
 class GpuEnumInfo(C.Structure):

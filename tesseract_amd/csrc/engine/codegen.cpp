@@ -416,6 +416,8 @@ struct Emitter {
     size_t luts = 0, values = 0;
     // lines written right after a node's definition (stores of outputs as soon as they exist)
     const std::unordered_map<uint32_t, std::vector<std::string>> *after = nullptr;
+    // when set, only the nodes marked here are defined (the rest are defined elsewhere)
+    const std::vector<uint8_t> *only = nullptr;
 
     Emitter(const Aig &g_, const Mapping &m_, std::ostringstream &o_, std::string p)
         : g(g_), m(m_), o(o_), prefix(std::move(p)) {}
@@ -438,7 +440,7 @@ struct Emitter {
     void emit(LeafFn leaf)
     {
         for (size_t n = 1; n < g.nodes.size(); n++) {
-            if (!m.in_cone[n])
+            if (!m.in_cone[n] || (only && !(*only)[n]))
                 continue;
             const Node &nd = g.nodes[n];
             if (nd.kind == N_AND) {
@@ -901,51 +903,7 @@ bool codegen_enum(const ts_program *a, const ts_program *b, const int *in_map, c
         cells_a = el.out_cells_a;
     }
 
-    std::ostringstream o;
-    o << kPrelude << kReportHits;
-    o << "TS_DEV u32 ts_shl32(u32 a, u32 s) { return s < 32u ? a << s : 0u; }\n"
-         "TS_DEV u64 ts_shl64(u64 a, u64 s) { return s < 64ull ? a << s : 0ull; }\n"
-         "TS_DEV u64 ts_shr64(u64 a, u64 s) { return s < 64ull ? a >> s : 0ull; }\n";
-    // Word indices: 32 bits while the space has at most 2^36 assignments (i1 <= 2^31, so a
-    // round's base plus the grid's stride stays below 2^32), else 64. The loop is wave-uniform: every lane of a wave takes the same
-    // rounds (a lane past i1 has no valid assignment), so the hit report's cross-lane steps see
-    // every lane.
-    const bool narrow = n <= 36;
-    const char *W = narrow ? "u32" : "u64";
-    o << "extern \"C\" __global__ void __launch_bounds__(" << opt.block << ")"
-      << (opt.waves > 1 ? " __attribute__((amdgpu_waves_per_eu(" + std::to_string(opt.waves) + ")))"
-                        : std::string())
-      << " ts_enum(u64* __restrict__ out, u64* __restrict__ list, u64 cap, u64 i0_, u64 i1_, "
-         "u64 space, u32 parts, u32 part, u32 nshare) {\n"
-      << "  const " << W << " i0 = (" << W << ")i0_, i1 = (" << W << ")i1_;\n"
-      << "  u32 hits = 0u, full = 0u;  // this lane's hits; the hit list is full\n"
-      << "  for (" << W << " base = i0 + (" << W << ")blockIdx.x * " << opt.block
-      << "u + (threadIdx.x & ~63u); base < i1; base += (" << W << ")gridDim.x * " << opt.block
-      << "u) {\n"
-      << "    const " << W << " i = base + (threadIdx.x & 63u);\n"
-      << "    " << W << " gw = i;\n"
-         "    if (parts > 1u) {  // active chunk (64 passes of 512 = 1024 words) -> chunk index\n"
-      << "      const " << W << " ci = i >> 10, q = ci / nshare, r = part + ci % nshare;\n"
-      << "      gw = ((q * parts + r) << 10) | (i & 1023u);\n"
-         "    }\n"
-         "    const u64 xb = (u64)gw << 5;\n";
-    if (n >= 5)  // the space is a multiple of 32: a word is wholly valid or wholly not
-        o << "    const u32 valid = i < i1 && xb < space ? 0xffffffffu : 0u;\n";
-    else
-        o << "    const u32 valid = i >= i1 || xb >= space ? 0u\n"
-             "                      : space - xb >= 32ull ? 0xffffffffu : ((1u << (u32)(space - xb)) - 1u);\n";
-    // input bit j of x: j < 5 lane pattern, else a bit of the word index (a sign-extended
-    // one-bit field: 0 or all ones)
-    static const uint32_t pat[5] = {0xAAAAAAAAu, 0xCCCCCCCCu, 0xF0F0F0F0u, 0xFF00FF00u, 0xFFFF0000u};
-    auto inbit = [&](uint32_t j) -> std::string {
-        if (j < 5)
-            return hex32(pat[j]);
-        if (narrow || j - 5 < 32)
-            return "(u32)__builtin_amdgcn_sbfe((int)(u32)gw, " + std::to_string(j - 5) + "u, 1u)";
-        return "(u32)__builtin_amdgcn_sbfe((int)(u32)((u64)gw >> 32), " + std::to_string(j - 37) +
-               "u, 1u)";
-    };
-
+    // ---- the network
     std::vector<Lit> ra, rb;
     std::vector<uint8_t> wa(a->ncells + 1, 0), wb;
     std::vector<uint32_t> b_in;
@@ -988,58 +946,202 @@ bool codegen_enum(const ts_program *a, const ts_program *b, const int *in_map, c
     Mapping m;
     map_luts(g, roots, opt.lut3, m);
 
-    auto leaf = [&](uint32_t, const Node &nd) -> std::string {
-        if (nd.kind == N_INBIT)
-            return inbit(nd.a);
-        if (nd.kind == N_LOAD)
-            return "s" + std::to_string(nd.a);
-        return "0u";
-    };
-    std::vector<std::string> oa, ob;  // expressions of the compared output words
-    if (!loop) {
-        Emitter e(g, m, o, "v");
-        e.emit(leaf);
-        ks.bitop3 += e.luts;
-        for (Lit v : out_a)
-            oa.push_back(e.val(v));
-        for (Lit v : out_b)
-            ob.push_back(e.val(v));
-    } else {
-        // state s<c> (a) and s<ncells_a + 1 + c> (b): power-on with the inputs set
-        auto init_state = [&](const ts_program *p, const uint32_t *in_cells, uint32_t base) {
-            for (uint32_t c = 0; c <= p->ncells; c++) {
-                std::string v = "0u";
-                if (c == TS_CELL_ONE)
-                    v = "0xffffffffu";
-                for (unsigned k = 0; k < n; k++)
-                    if (in_cells[k] == c && c >= TS_CELL_FIRST_FREE)
-                        v = inbit(n - 1 - k);
-                o << "    u32 s" << base + c << " = " << v << ";\n";
+    // ---- groups of 32 words. x bits 0..4 are the lanes of a word; a thread that takes the 32
+    // consecutive words of a group sees x bits 5..9 run through all 32 values and the bits above
+    // stay fixed. A LUT whose inputs reach no lane bit is evaluated once per group on group planes
+    // (bit w = its value in word w, the same in all 32 lanes of that word), and a word takes bit w
+    // of each such plane its lane-dependent LUTs read as a sign-extended one-bit field. cls: 0 =
+    // the same in every word of the group (x bits >= 10 only), 1 = varies with x bits 5..9,
+    // 2 = reaches a lane bit.
+    std::vector<uint8_t> cls;
+    size_t group_luts = 0;
+    bool grouped = false;
+    if (!loop && n >= 24 && opt.group != 1) {
+        cls.assign(g.nodes.size(), 0);
+        for (size_t k = 1; k < g.nodes.size(); k++) {
+            if (!m.in_cone[k])
+                continue;
+            const Node &nd = g.nodes[k];
+            if (nd.kind == N_INBIT) {
+                cls[k] = nd.a < 5 ? 2 : nd.a < 10 ? 1 : 0;
+            } else if (nd.kind == N_AND && m.is_lut[k]) {
+                uint8_t c = 0;
+                for (int l = 0; l < m.best[k].n; l++)
+                    c = std::max(c, cls[m.best[k].leaf[l]]);
+                cls[k] = c;
+                if (c < 2)
+                    group_luts++;
             }
+        }
+        grouped = opt.group == 2 || group_luts >= 32;
+    }
+    ks.group = grouped ? 32 : 1;
+
+    std::ostringstream o;
+    o << kPrelude << kReportHits;
+    o << "TS_DEV u32 ts_shl32(u32 a, u32 s) { return s < 32u ? a << s : 0u; }\n"
+         "TS_DEV u64 ts_shl64(u64 a, u64 s) { return s < 64ull ? a << s : 0ull; }\n"
+         "TS_DEV u64 ts_shr64(u64 a, u64 s) { return s < 64ull ? a >> s : 0ull; }\n";
+    // Word indices: 32 bits while the space has at most 2^36 assignments (i1 <= 2^31, so a
+    // round's base plus the grid's stride stays below 2^32), else 64. The loop is wave-uniform:
+    // every lane of a wave takes the same rounds (a lane past i1 has no valid assignment), so the
+    // hit report's cross-lane steps see every lane.
+    const bool narrow = n <= 36;
+    const char *W = narrow ? "u32" : "u64";
+    o << "extern \"C\" __global__ void __launch_bounds__(" << opt.block << ")"
+      << (opt.waves > 1 ? " __attribute__((amdgpu_waves_per_eu(" + std::to_string(opt.waves) + ")))"
+                        : std::string())
+      << " ts_enum(u64* __restrict__ out, u64* __restrict__ list, u64 cap, u64 i0_, u64 i1_, "
+         "u64 space, u32 parts, u32 part, u32 nshare) {\n"
+      << "  const " << W << " i0 = (" << W << ")i0_, i1 = (" << W << ")i1_;\n"
+      << "  u32 hits = 0u, full = 0u;  // this lane's hits; the hit list is full\n";
+    // input bit j of x: j < 5 lane pattern, else a bit of the word index (a sign-extended
+    // one-bit field: 0 or all ones)
+    static const uint32_t pat[5] = {0xAAAAAAAAu, 0xCCCCCCCCu, 0xF0F0F0F0u, 0xFF00FF00u, 0xFFFF0000u};
+    auto wordbit = [&](const char *word, uint32_t j) -> std::string {
+        if (narrow || j - 5 < 32)
+            return "(u32)__builtin_amdgcn_sbfe((int)(u32)" + std::string(word) + ", " +
+                   std::to_string(j - 5) + "u, 1u)";
+        return "(u32)__builtin_amdgcn_sbfe((int)(u32)((u64)" + std::string(word) + " >> 32), " +
+               std::to_string(j - 37) + "u, 1u)";
+    };
+    auto inbit = [&](uint32_t j) -> std::string {
+        return j < 5 ? hex32(pat[j]) : wordbit("gw", j);
+    };
+    // active word index I -> word G of the whole space: a share takes every parts-th chunk of
+    // 64 passes of 512 machines (1024 words)
+    auto emit_chunk_map = [&](const std::string &I, const std::string &G) {
+        o << "    if (parts > 1u) {  // active chunk -> chunk of the space\n"
+          << "      const " << W << " ci = " << I << " >> 10, q = ci / nshare, r = part + ci % nshare;\n"
+          << "      " << G << " = ((q * parts + r) << 10) | (" << I << " & 1023u);\n"
+          << "    }\n";
+    };
+
+    std::vector<std::string> oa, ob;  // expressions of the compared output words
+    if (grouped) {
+        o << "  const " << W << " g0 = i0 >> 5, g1 = (i1 + 31u) >> 5;\n"
+          << "  for (" << W << " gbase = g0 + (" << W << ")blockIdx.x * " << opt.block
+          << "u + (threadIdx.x & ~63u); gbase < g1; gbase += (" << W << ")gridDim.x * "
+          << opt.block << "u) {\n"
+          << "    const " << W << " ib = (gbase + (threadIdx.x & 63u)) << 5;  // the group's first word\n"
+          << "    " << W << " gwb = ib;\n";
+        emit_chunk_map("ib", "gwb");
+        // the group's part of the network on group planes
+        std::vector<uint8_t> in_group(g.nodes.size(), 0), in_word(g.nodes.size(), 0);
+        for (size_t k = 1; k < g.nodes.size(); k++)
+            if (m.in_cone[k]) {
+                const bool def = g.nodes[k].kind == N_INBIT || m.is_lut[k];
+                (cls[k] < 2 ? in_group : in_word)[k] = def;
+            }
+        auto gleaf = [&](uint32_t, const Node &nd) -> std::string {
+            if (nd.kind == N_INBIT)
+                return nd.a < 10 ? hex32(pat[nd.a - 5]) : wordbit("gwb", nd.a);
+            return "0u";
         };
-        init_state(a, a->inputs, 0);
-        if (el.mode == TS_EN_EQUIV)
-            init_state(b, b_in.data(), a->ncells + 1);
-        o << "    for (u32 tick = 0; tick < " << el.ticks << "u; tick++) {\n";
-        Emitter e(g, m, o, "v");
-        e.emit(leaf);
-        ks.bitop3 += e.luts;
-        std::vector<std::pair<uint32_t, std::string>> upd;
-        for (uint32_t c = 0; c <= a->ncells; c++)
-            if (wa[c])
-                upd.push_back({c, e.val(ra[c])});
-        for (uint32_t c = 0; el.mode == TS_EN_EQUIV && c <= b->ncells; c++)
-            if (wb[c])
-                upd.push_back({a->ncells + 1 + c, e.val(rb[c])});
-        for (size_t i = 0; i < upd.size(); i++)
-            o << "      u32 n" << i << " = " << upd[i].second << ";\n";
-        for (size_t i = 0; i < upd.size(); i++)
-            o << "      s" << upd[i].first << " = n" << i << ";\n";
-        o << "    }\n";
-        for (uint32_t c : cells_a)
-            oa.push_back("s" + std::to_string(c));
-        for (uint32_t c : cells_b)
-            ob.push_back("s" + std::to_string(a->ncells + 1 + c));
+        Emitter eg(g, m, o, "g");
+        eg.only = &in_group;
+        eg.emit(gleaf);
+        ks.bitop3 += eg.luts;
+        // the group values a word reads: LUT leaves of its own part and compared outputs
+        std::vector<uint8_t> import(g.nodes.size(), 0);
+        for (size_t k = 1; k < g.nodes.size(); k++)
+            if (in_word[k] && g.nodes[k].kind == N_AND)
+                for (int l = 0; l < m.best[k].n; l++)
+                    if (cls[m.best[k].leaf[l]] < 2)
+                        import[m.best[k].leaf[l]] = 1;
+        for (Lit v : out_a)
+            if (g.nodes[v >> 1].kind != N_CONST && cls[v >> 1] < 2)
+                import[v >> 1] = 1;
+        for (Lit v : out_b)
+            if (g.nodes[v >> 1].kind != N_CONST && cls[v >> 1] < 2)
+                import[v >> 1] = 1;
+        o << "    #pragma unroll 1\n"
+          << "    for (u32 w = 0; w < 32u; w++) {\n"
+          << "    const " << W << " i = ib + w;\n"
+          << "    const " << W << " gw = gwb + w;\n"
+          << "    const u64 xb = (u64)gw << 5;\n"
+          << "    const u32 valid = i >= i0 && i < i1 && xb < space ? 0xffffffffu : 0u;\n";
+        for (size_t k = 1; k < g.nodes.size(); k++)
+            if (import[k])
+                o << "    const u32 v" << k << " = "
+                  << (cls[k] == 1 ? "(u32)__builtin_amdgcn_sbfe((int)g" + std::to_string(k) + ", w, 1u)"
+                                  : "g" + std::to_string(k))
+                  << ";\n";
+        Emitter ew(g, m, o, "v");
+        ew.only = &in_word;
+        ew.emit([&](uint32_t, const Node &nd) -> std::string {
+            return nd.kind == N_INBIT ? hex32(pat[nd.a]) : std::string("0u");
+        });
+        ks.bitop3 += ew.luts;
+        for (Lit v : out_a)
+            oa.push_back(ew.val(v));
+        for (Lit v : out_b)
+            ob.push_back(ew.val(v));
+    } else {
+        o << "  for (" << W << " base = i0 + (" << W << ")blockIdx.x * " << opt.block
+          << "u + (threadIdx.x & ~63u); base < i1; base += (" << W << ")gridDim.x * " << opt.block
+          << "u) {\n"
+          << "    const " << W << " i = base + (threadIdx.x & 63u);\n"
+          << "    " << W << " gw = i;\n";
+        emit_chunk_map("i", "gw");
+        o << "    const u64 xb = (u64)gw << 5;\n";
+        if (n >= 5)  // the space is a multiple of 32: a word is wholly valid or wholly not
+            o << "    const u32 valid = i < i1 && xb < space ? 0xffffffffu : 0u;\n";
+        else
+            o << "    const u32 valid = i >= i1 || xb >= space ? 0u\n"
+                 "                      : space - xb >= 32ull ? 0xffffffffu : ((1u << (u32)(space - xb)) - 1u);\n";
+        auto leaf = [&](uint32_t, const Node &nd) -> std::string {
+            if (nd.kind == N_INBIT)
+                return inbit(nd.a);
+            if (nd.kind == N_LOAD)
+                return "s" + std::to_string(nd.a);
+            return "0u";
+        };
+        if (!loop) {
+            Emitter e(g, m, o, "v");
+            e.emit(leaf);
+            ks.bitop3 += e.luts;
+            for (Lit v : out_a)
+                oa.push_back(e.val(v));
+            for (Lit v : out_b)
+                ob.push_back(e.val(v));
+        } else {
+            // state s<c> (a) and s<ncells_a + 1 + c> (b): power-on with the inputs set
+            auto init_state = [&](const ts_program *p, const uint32_t *in_cells, uint32_t base) {
+                for (uint32_t c = 0; c <= p->ncells; c++) {
+                    std::string v = "0u";
+                    if (c == TS_CELL_ONE)
+                        v = "0xffffffffu";
+                    for (unsigned k = 0; k < n; k++)
+                        if (in_cells[k] == c && c >= TS_CELL_FIRST_FREE)
+                            v = inbit(n - 1 - k);
+                    o << "    u32 s" << base + c << " = " << v << ";\n";
+                }
+            };
+            init_state(a, a->inputs, 0);
+            if (el.mode == TS_EN_EQUIV)
+                init_state(b, b_in.data(), a->ncells + 1);
+            o << "    for (u32 tick = 0; tick < " << el.ticks << "u; tick++) {\n";
+            Emitter e(g, m, o, "v");
+            e.emit(leaf);
+            ks.bitop3 += e.luts;
+            std::vector<std::pair<uint32_t, std::string>> upd;
+            for (uint32_t c = 0; c <= a->ncells; c++)
+                if (wa[c])
+                    upd.push_back({c, e.val(ra[c])});
+            for (uint32_t c = 0; el.mode == TS_EN_EQUIV && c <= b->ncells; c++)
+                if (wb[c])
+                    upd.push_back({a->ncells + 1 + c, e.val(rb[c])});
+            for (size_t i = 0; i < upd.size(); i++)
+                o << "      u32 n" << i << " = " << upd[i].second << ";\n";
+            for (size_t i = 0; i < upd.size(); i++)
+                o << "      s" << upd[i].first << " = n" << i << ";\n";
+            o << "    }\n";
+            for (uint32_t c : cells_a)
+                oa.push_back("s" + std::to_string(c));
+            for (uint32_t c : cells_b)
+                ob.push_back("s" + std::to_string(a->ncells + 1 + c));
+        }
     }
 
     // hit word
@@ -1092,8 +1194,10 @@ bool codegen_enum(const ts_program *a, const ts_program *b, const int *in_map, c
         }
         o << "    hit = (miss0 | miss1 | miss2 | miss3) & valid;\n";
     }
-    o << "    ts_report_hits(out, list, cap, (u64)gw, hit, &hits, &full);\n"
-         "  }\n"
+    o << "    ts_report_hits(out, list, cap, (u64)gw, hit, &hits, &full);\n";
+    if (grouped)
+        o << "    }\n";
+    o << "  }\n"
          "  ts_count_hits(out, hits);\n}\n";
     ks.names.push_back("ts_enum");
     ks.text = o.str();

@@ -875,6 +875,7 @@ int ts_internal::enum_prepare(const gpu_engine *e, const gpu_enum *en, gpu_tune 
     co.block = tune.block;
     co.lut3 = tune.lut3 == 1;
     co.waves = std::min(tune.waves, 8u);
+    co.group = tune.group;
     std::string msg;
     if (!codegen_enum(en->a, en->b, plan.in_map.data(), plan.out_map.data(), plan.el, co, ks, msg,
                       plan.spec)) {
@@ -965,6 +966,7 @@ struct gpu_enum_code {
     uint64_t      len_total = 0;  // instructions per pass of a (+ b for equiv)
     uint64_t     *d_out = nullptr, *d_list = nullptr;
     double        compile_seconds = 0;
+    unsigned      group = 1;          // words a thread of the kernel takes per step
     unsigned      sched = 1;          // the scheduling strategy of the kept kernel
     double        sched_ms[5] = {0, 0, 0, 0, 0};
     uint64_t      sample_words = 0;
@@ -1029,8 +1031,9 @@ int enum_autotune(gpu_enum_code *c, const std::string &src, uint64_t sample, cha
         uint32_t parts = (uint32_t)c->parts, part = (uint32_t)c->part, nshare = si.nshare;
         void *args[] = {&d_out, &d_list, &capv, &i0, &i1, &space, &parts, &part, &nshare};
         const gpu_tune &tune = c->tune;
+        const uint64_t units = (sample + c->group - 1) / c->group;
         unsigned grid = (unsigned)std::max<uint64_t>(
-            1, std::min<uint64_t>((sample + tune.block - 1) / tune.block, tune.grid));
+            1, std::min<uint64_t>((units + tune.block - 1) / tune.block, tune.grid));
         double ms_best = 0;
         for (int rep = 0; rep < 3; rep++) {
             HIPCHK(hipEventRecord(e->ev0, e->stream));
@@ -1099,6 +1102,7 @@ extern "C" int gpu_enum_compile(gpu_engine *e, const gpu_enum *en, gpu_enum_code
         KernelSource ks;
         if (!ts_internal::enum_prepare(e, en, c->tune, ks, err, errlen))
             return 0;
+        c->group = ks.group;
         c->part = en->part;
         c->part_end = en->part_end;
         c->parts = en->parts;
@@ -1189,7 +1193,9 @@ struct EnumExec {
                 return 0;
             }
         } else {
-            uint64_t blocks = (i1 - i0 + tune.block - 1) / tune.block;
+            // a thread takes c->group consecutive words (a group starts at a multiple of it)
+            const uint64_t units = (i1 + c->group - 1) / c->group - i0 / c->group;
+            uint64_t blocks = (units + tune.block - 1) / tune.block;
             unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, tune.grid));
             void *args[] = {&d_out, &d_list, &capv, &i0, &i1, &space, &parts, &part, &nshare};
             // a cap on resident waves: LDS the kernel does not use, reserved per workgroup so

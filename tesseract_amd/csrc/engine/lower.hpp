@@ -119,12 +119,14 @@ struct KernelSource {
     std::string text;
     std::vector<std::string> names;  // kernel names in launch order (one per segment)
     size_t and_nodes = 0, bitop3 = 0, values = 0;
+    unsigned group = 1;  // enumerations: words a thread takes per step of its loop (1 or 32)
 };
 
 struct CodegenOptions {
     unsigned block = 256;
     bool     lut3 = true;  // map 3-input cuts to v_bitop3_b32
     unsigned waves = 0;    // run kernels: minimum waves per SIMD (0 = the compiler's choice)
+    unsigned group = 0;    // enumerations: 1 = a word per thread, 2 = groups of 32 words, 0 = auto
 };
 
 // Streamed form: one kernel per segment, state in HBM slots.

@@ -76,6 +76,7 @@ struct Module {
     int users = 0;
     double last_use = 0;
     unsigned blocks_per_cu = 1;
+    unsigned group = 1;  // words a thread of the kernel takes per step
 };
 
 // AND nodes per second of the tape interpreter with a file of about 230 slots (two workgroups of
@@ -111,6 +112,7 @@ struct Request {
     std::vector<char> code;
     gpu_tune tune{};
     gpu_tune upgrade_tune{};
+    unsigned group = 1, upgrade_group = 1;  // words per thread step of the compiled kernel
     ShareInfo si{};
     Module *mod = nullptr;
     std::vector<Chunk> chunks;  // cut as launched, in increasing words
@@ -281,6 +283,7 @@ void compile_loop(gpu_server *s)
                 r->source = std::move(source);
                 r->code = std::move(code);
                 r->upgrade_tune = tn;
+                r->upgrade_group = ks.group;
                 r->upgrade_ok = ok;
                 r->upgrade_done = true;
                 s->cv_sched.notify_all();
@@ -290,6 +293,7 @@ void compile_loop(gpu_server *s)
         r->source = std::move(source);
         r->code = std::move(code);
         r->tune = tn;
+        r->group = ks.group;
         prepared(s, r, ok, err);
     }
 }
@@ -362,6 +366,7 @@ bool admit_module(gpu_server *s, Request *r)
             blocks < 1)
             blocks = 1;
         m.blocks_per_cu = (unsigned)blocks;
+        m.group = r->group;
     }
     m.users++;
     m.last_use = now_s();
@@ -435,6 +440,7 @@ void upgrade(gpu_server *s, Request *r)
         return;
     gpu_tune keep = r->tune;
     r->tune = r->upgrade_tune;
+    r->group = r->upgrade_group;
     std::string err = r->error;
     if (!admit_module(s, r)) {
         r->tune = keep;
@@ -451,7 +457,8 @@ bool launch_chunk(gpu_server *s, Slot &sl, Request *r, size_t ci)
     Chunk &c = r->chunks[ci];
     const unsigned block = r->tape ? 64 : r->tune.block;
     const unsigned per_cu = r->tape ? r->tape_bpc : r->mod->blocks_per_cu;
-    uint64_t blocks = (c.i1 - c.i0 + block - 1) / block;
+    const uint64_t group = r->tape ? 1 : r->mod->group;
+    uint64_t blocks = ((c.i1 + group - 1) / group - c.i0 / group + block - 1) / block;
     uint64_t share = std::max<uint64_t>(1, (uint64_t)s->e->cus * per_cu / s->slots.size());
     unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, share));
     uint64_t space = r->si.space, cap = kListCap, i0 = c.i0, i1 = c.i1;

@@ -820,6 +820,151 @@ bool lower_spec(const ts_spec *sp, SpecCode &sc, std::string &err)
     return true;
 }
 
+// The expected value as B + sum over the lane bits t of x of (bit t) * C_t, when the spec is
+// linear in them: sums, differences, negation, complement (~v = -v - 1), products with a factor
+// no lane bit reaches and left shifts by such amounts keep the form; &, |, ^ and right shifts
+// keep it only between lane-free operands. B and C_t are C expressions over the per-word var
+// bases vb<k> (lane bits cleared). dep_c: the x bits the C_t read; tz: bits below it are zero in
+// every C_t, so the lanes agree there.
+struct SpecLin {
+    std::string b, c[5];
+    bool has_c[5] = {false, false, false, false, false};
+    uint64_t dep_c = 0;
+    int tz = 64;
+};
+
+bool linear_spec(const ts_spec *sp, bool u32, SpecLin &out)
+{
+    struct Sym {
+        std::string b, c[5];
+        bool has[5] = {false, false, false, false, false};
+        uint64_t dep_b = 0, dep_c = 0;
+        int tz[5] = {64, 64, 64, 64, 64};
+        bool lanes() const { return has[0] || has[1] || has[2] || has[3] || has[4]; }
+    };
+    const int nr = ts_spec_nrpn(sp);
+    const ts_spec_item *rpn = ts_spec_rpn(sp);
+    const std::string T = u32 ? "u32" : "u64";
+    const std::string zero = "((" + T + ")0)";
+    std::vector<Sym> st;
+    for (int i = 0; i < nr; i++) {
+        const ts_spec_item &it = rpn[i];
+        char buf[96];
+        if (it.op == TS_T_NUM) {
+            Sym s;
+            snprintf(buf, sizeof buf, "((%s)0x%llxull)", T.c_str(), (unsigned long long)it.v);
+            s.b = buf;
+            st.push_back(s);
+            continue;
+        }
+        if (it.op == TS_T_VAR) {
+            const int k = (int)it.v;
+            Sym s;
+            s.b = "((" + T + ")vb" + std::to_string(k) + ")";
+            for (int bit = 0; bit < ts_spec_var_bits(sp, k); bit++) {
+                const int p = ts_spec_var_pos(sp, k, bit);
+                if (p < 5) {
+                    snprintf(buf, sizeof buf, "((%s)0x%llxull)", T.c_str(), 1ull << bit);
+                    s.c[p] = buf;
+                    s.has[p] = true;
+                    s.tz[p] = bit;
+                } else {
+                    s.dep_b |= 1ull << p;
+                }
+            }
+            st.push_back(s);
+            continue;
+        }
+        if (it.op == TS_T_NOT || it.op == TS_T_NEG) {
+            if (st.empty())
+                return false;
+            Sym &s = st.back();
+            s.b = it.op == TS_T_NOT ? "(~" + s.b + ")" : "(" + zero + " - " + s.b + ")";
+            for (int t = 0; t < 5; t++)
+                if (s.has[t])
+                    s.c[t] = "(" + zero + " - " + s.c[t] + ")";
+            continue;
+        }
+        if (st.size() < 2)
+            return false;
+        Sym r = st.back();
+        st.pop_back();
+        Sym &x = st.back();
+        switch (it.op) {
+        case TS_T_ADD:
+        case TS_T_SUB: {
+            const char *op = it.op == TS_T_ADD ? " + " : " - ";
+            x.b = "(" + x.b + op + r.b + ")";
+            for (int t = 0; t < 5; t++) {
+                if (r.has[t])
+                    x.c[t] = x.has[t] ? "(" + x.c[t] + op + r.c[t] + ")"
+                                      : it.op == TS_T_ADD ? r.c[t] : "(" + zero + " - " + r.c[t] + ")";
+                x.has[t] = x.has[t] || r.has[t];
+                x.tz[t] = std::min(x.tz[t], r.tz[t]);
+            }
+            x.dep_b |= r.dep_b;
+            x.dep_c |= r.dep_c;
+            break;
+        }
+        case TS_T_MUL: {
+            if (x.lanes() && r.lanes())
+                return false;
+            const Sym &lin = x.lanes() ? x : r, &f = x.lanes() ? r : x;
+            Sym o;
+            o.b = "(" + x.b + " * " + r.b + ")";
+            o.dep_b = x.dep_b | r.dep_b;
+            for (int t = 0; t < 5; t++)
+                if (lin.has[t]) {
+                    o.c[t] = "(" + lin.c[t] + " * " + f.b + ")";
+                    o.has[t] = true;
+                    o.tz[t] = lin.tz[t];
+                }
+            o.dep_c = lin.dep_c | f.dep_b;
+            x = o;
+            break;
+        }
+        case TS_T_SHL: {
+            if (r.lanes())
+                return false;
+            const char *fn = u32 ? "ts_shl32(" : "ts_shl64(";
+            const bool num = i > 0 && rpn[i - 1].op == TS_T_NUM;
+            x.b = std::string(fn) + x.b + ", " + r.b + ")";
+            for (int t = 0; t < 5; t++)
+                if (x.has[t]) {
+                    x.c[t] = std::string(fn) + x.c[t] + ", " + r.b + ")";
+                    x.tz[t] = num ? (int)std::min<uint64_t>(64, x.tz[t] + rpn[i - 1].v) : x.tz[t];
+                }
+            x.dep_b |= r.dep_b;
+            x.dep_c |= r.dep_b;
+            break;
+        }
+        default: {  // &, |, ^, >>: lane-free operands only
+            if (x.lanes() || r.lanes())
+                return false;
+            if (it.op == TS_T_SHR)
+                x.b = "ts_shr64(" + x.b + ", " + r.b + ")";
+            else
+                x.b = "(" + x.b + (it.op == TS_T_AND ? " & " : it.op == TS_T_OR ? " | " : " ^ ") +
+                      r.b + ")";
+            x.dep_b |= r.dep_b;
+        }
+        }
+    }
+    if (st.size() != 1)
+        return false;
+    const Sym &s = st[0];
+    out.b = s.b;
+    out.dep_c = s.dep_c;
+    out.tz = 64;
+    for (int t = 0; t < 5; t++) {
+        out.c[t] = s.has[t] ? s.c[t] : zero;
+        out.has_c[t] = s.has[t];
+        if (s.has[t])
+            out.tz = std::min(out.tz, s.tz[t]);
+    }
+    return true;
+}
+
 // 32 x 32 bit transpose of e0..e31 in place: stage j swaps the j-bit blocks of word pairs
 // (k, k + j). The 16- and 8-bit stages are byte shuffles, one v_perm_b32 per word (a byte
 // selector picks from {S0 = high word, S1 = low word}: 0-3 = bytes of S1, 4-7 = bytes of S0);
@@ -977,6 +1122,18 @@ bool codegen_enum(const ts_program *a, const ts_program *b, const int *in_map, c
     }
     ks.group = grouped ? 32 : 1;
 
+    // verify: the per-lane expected values, and in groups the linear form when its lane terms
+    // read only x bits >= 10 (the same in every word of a group)
+    SpecCode sc;
+    SpecLin sl;
+    bool lin = false;
+    if (el.mode == TS_EN_VERIFY) {
+        if (!lower_spec(spec, sc, err))
+            return false;
+        lin = grouped && linear_spec(spec, sc.u32, sl) && (sl.dep_c & 0x3E0ull) == 0;
+    }
+    const int lin_free = lin ? std::min(sl.tz, el.width) : 0;  // bits every lane of a word shares
+
     std::ostringstream o;
     o << kPrelude << kReportHits;
     o << "TS_DEV u32 ts_shl32(u32 a, u32 s) { return s < 32u ? a << s : 0u; }\n"
@@ -1042,6 +1199,37 @@ bool codegen_enum(const ts_program *a, const ts_program *b, const int *in_map, c
         eg.only = &in_group;
         eg.emit(gleaf);
         ks.bitop3 += eg.luts;
+        if (lin && lin_free < el.width) {
+            // T(k) = sum of C_t over the set bits t of lane k, the same in every word of the
+            // group; tp<i> = bit i of T(k) in lane k
+            const char *T = sc.u32 ? "u32" : "u64";
+            o << "    u32 tp" << lin_free;
+            for (int bit = lin_free + 1; bit < el.width; bit++)
+                o << ", tp" << bit;
+            o << ";\n    {\n"
+              << "    const u64 xb = (u64)gwb << 5;\n"
+              << "    const u64 xr = __builtin_bitreverse64(xb);\n    (void)xr;\n"
+              << sc.prelude;
+            for (int t = 0; t < 5; t++)
+                o << "    const " << T << " c" << t << " = " << sl.c[t] << ";\n";
+            o << "    const " << T << " k0 = 0;\n";
+            for (int k = 1; k < 32; k++)
+                o << "    const " << T << " k" << k << " = k" << (k & (k - 1)) << " + c"
+                  << __builtin_ctz(k) << ";\n";
+            for (int half = 0; half < 2; half++) {
+                const int lo = std::max(lin_free, 32 * half), hi = std::min(el.width, 32 * half + 32);
+                if (lo >= hi)
+                    continue;
+                o << "    {\n";
+                for (int k = 0; k < 32; k++)
+                    o << "    u32 t" << k << " = (u32)(k" << k << (half ? " >> 32" : "") << ");\n";
+                emit_transpose(o, "t");
+                for (int bit = lo; bit < hi; bit++)
+                    o << "    tp" << bit << " = t" << bit - 32 * half << ";\n";
+                o << "    }\n";
+            }
+            o << "    }\n";
+        }
         // the group values a word reads: LUT leaves of its own part and compared outputs
         std::vector<uint8_t> import(g.nodes.size(), 0);
         for (size_t k = 1; k < g.nodes.size(); k++)
@@ -1157,13 +1345,8 @@ bool codegen_enum(const ts_program *a, const ts_program *b, const int *in_map, c
             o << " | (" << oa[k] << " ^ " << ob[k] << ")";
         o << ";\n    hit &= valid;\n";
     } else {
-        SpecCode sc;
-        if (!lower_spec(spec, sc, err))
-            return false;
         o << "    const u64 xr = __builtin_bitreverse64(xb);\n    (void)xr;\n" << sc.prelude;
         const char *T = sc.u32 ? "u32" : "u64";
-        for (int l = 0; l < 32; l++)
-            o << "    const " << T << " e" << l << " = " << sc.lane_expr[l] << ";\n";
         // mismatches gathered in four accumulators, one v_bitop3_b32 per bit: m | (t ^ o), truth
         // table 0xF6 (bit 4 S0 + 2 S1 + S2 of the table is the result for inputs S0, S1, S2)
         o << "    u32 miss0 = 0u, miss1 = 0u, miss2 = 0u, miss3 = 0u;\n";
@@ -1173,26 +1356,57 @@ bool codegen_enum(const ts_program *a, const ts_program *b, const int *in_map, c
               << ", " << out << ", 0xF6u);\n";
             acc = (acc + 1) % 4;
         };
-        // lane-free low bits: every lane expects bit `bit` of e0
-        for (int bit = 0; bit < sc.lane_free; bit++)
-            gather("(u32)__builtin_amdgcn_sbfe((int)(u32)(e0" + std::string(bit >= 32 ? " >> 32" : "") +
-                       "), " + std::to_string(bit % 32) + "u, 1u)",
-                   oa[bit]);
-        // the rest from the 32 x 32 transpose of the expected values (the compiler drops the
-        // transpose work of rows no compare reads)
-        for (int half = 0; half < (el.width > 32 ? 2 : 1); half++) {
-            if (std::max(sc.lane_free, 32 * half) >= std::min(el.width, 32 * half + 32))
-                continue;
-            o << "    {\n";
+        auto bit_of = [](const std::string &v, int bit) {
+            return "(u32)__builtin_amdgcn_sbfe((int)(u32)(" + v + (bit >= 32 ? " >> 32" : "") +
+                   "), " + std::to_string(bit % 32) + "u, 1u)";
+        };
+        if (lin) {
+            // expected value of lane k = B + T(k): B the same in all lanes of the word, T's bit
+            // planes from the group; a bit-sliced ripple add (xor3 0x96, majority 0xE8) gives
+            // each expected bit plane, compared as it is made
+            o << "    const " << T << " bw = " << sl.b << ";\n";
+            for (int bit = 0; bit < lin_free; bit++)
+                gather(bit_of("bw", bit), oa[bit]);
+            for (int bit = lin_free; bit < el.width; bit++) {
+                const std::string tp = "tp" + std::to_string(bit), rb = "r" + std::to_string(bit);
+                o << "    const u32 " << rb << " = " << bit_of("bw", bit) << ";\n";
+                if (bit == lin_free) {
+                    o << "    u32 cy = " << tp << " & " << rb << ";\n";
+                    gather("(" + tp + " ^ " + rb + ")", oa[bit]);
+                } else {
+                    o << "    const u32 s" << bit << " = __builtin_amdgcn_bitop3_b32(" << tp << ", "
+                      << rb << ", cy, 0x96u);\n";
+                    if (bit + 1 < el.width)
+                        o << "    cy = __builtin_amdgcn_bitop3_b32(" << tp << ", " << rb
+                          << ", cy, 0xE8u);\n";
+                    gather("s" + std::to_string(bit), oa[bit]);
+                }
+            }
+            o << "    hit = (miss0 | miss1 | miss2 | miss3) & valid;\n";
+        } else {
             for (int l = 0; l < 32; l++)
-                o << "    u32 t" << l << " = (u32)(e" << l << (half ? " >> 32" : "") << ");\n";
-            emit_transpose(o, "t");
-            for (int bit = std::max(sc.lane_free, 32 * half); bit < el.width && bit < 32 * half + 32;
-                 bit++)
-                gather("t" + std::to_string(bit - 32 * half), oa[bit]);
-            o << "    }\n";
+                o << "    const " << T << " e" << l << " = " << sc.lane_expr[l] << ";\n";
+            // lane-free low bits: every lane expects bit `bit` of e0
+            for (int bit = 0; bit < sc.lane_free; bit++)
+                gather("(u32)__builtin_amdgcn_sbfe((int)(u32)(e0" + std::string(bit >= 32 ? " >> 32" : "") +
+                           "), " + std::to_string(bit % 32) + "u, 1u)",
+                       oa[bit]);
+            // the rest from the 32 x 32 transpose of the expected values (the compiler drops the
+            // transpose work of rows no compare reads)
+            for (int half = 0; half < (el.width > 32 ? 2 : 1); half++) {
+                if (std::max(sc.lane_free, 32 * half) >= std::min(el.width, 32 * half + 32))
+                    continue;
+                o << "    {\n";
+                for (int l = 0; l < 32; l++)
+                    o << "    u32 t" << l << " = (u32)(e" << l << (half ? " >> 32" : "") << ");\n";
+                emit_transpose(o, "t");
+                for (int bit = std::max(sc.lane_free, 32 * half); bit < el.width && bit < 32 * half + 32;
+                     bit++)
+                    gather("t" + std::to_string(bit - 32 * half), oa[bit]);
+                o << "    }\n";
+            }
+            o << "    hit = (miss0 | miss1 | miss2 | miss3) & valid;\n";
         }
-        o << "    hit = (miss0 | miss1 | miss2 | miss3) & valid;\n";
     }
     o << "    ts_report_hits(out, list, cap, (u64)gw, hit, &hits, &full);\n";
     if (grouped)

@@ -56,6 +56,21 @@ def test_mul16_shares_against_the_reference(ref, part, part_end):
         assert hg == (0 if spec == "p = a * b" else odd_b << 23)
 
 
+@pytest.mark.parametrize("spec", ["p = b * a", "p = ~(0 - a * b) + 1",
+                                  "p = a * b + (a << 3) - a * 8", "p = (a + 1) * b - b",
+                                  "p = a * b + 12345", "p = a * b ^ 1"])
+def test_mul16_spec_forms(spec):
+    # specs equal to a * b in several forms, some kept linear in the lane bits (the grouped
+    # kernel's bit-sliced add) and some not (its per-lane values); one differs: a * b ^ 1 in every
+    # assignment, a * b + 12345 in every assignment
+    prog = golden("mul16")
+    plain, grouped = _both(t.EN_VERIFY, prog, spec=spec, part=5, part_end=5, parts=1024)
+    hp, xp = K.run(plain, 32, part=5, part_end=5, parts=1024)
+    hg, xg = K.run(grouped, 32, part=5, part_end=5, parts=1024)
+    assert (hp, xp) == (hg, xg)
+    assert hg == ((1 << 22) if spec in ("p = a * b + 12345", "p = a * b ^ 1") else 0)
+
+
 def test_ranges_inside_groups():
     # launches over word ranges that begin and end inside a group (the list-overflow halving
     # launches such ranges): the grouped kernel lists exactly the one-word kernel's hits

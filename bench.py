@@ -235,20 +235,27 @@ def life_check(life, gens):
 def run_tester1(args):
     """Tester 1 on the device and the reference's t1_run on host threads over the same programs:
     the per-size summaries must agree (the longest cycle's seed aside, which depends on the order
-    the host threads finish in)."""
+    the host threads finish in). With --t1-seconds both run for that long (the reference's
+    --seconds mode) and only their rates compare."""
     import ctypes as C
     import tesseract_amd as t
     eng = t.Engine(0)
     cells = [int(c) for c in args.cells.split(",")]
-    cfg = t.t1_config(cells, args.ratio, args.programs, args.max_passes, args.seed)
+    timed = args.t1_seconds > 0  # the reference's --seconds mode: as many programs as time allows
+    programs = 0 if timed else args.programs
+    cfg = t.t1_config(cells, args.ratio, programs, args.max_passes, args.seed,
+                      seconds=args.t1_seconds)
     # warm-up, untimed: the same programs with cycle searches cut at 2^10 passes (the device
     # buffers and kernels of the timed run, little of its work)
-    eng.t1_run(t.t1_config(cells, args.ratio, args.programs, 1 << 10, args.seed))
+    eng.t1_run(t.t1_config(cells, args.ratio, 64 if timed else args.programs, 1 << 10, args.seed))
     got, gsec = eng.t1_run(cfg)
     ops = sum(d["passes"] * d["len"] * 512 for d in got)  # src/tester.c:302-305
     out = {"workload": "tester1", "cells": cells, "ratio": args.ratio,
-           "programs_per_size": args.programs, "max_passes": args.max_passes, "seed": args.seed,
+           "programs_per_size": [d["programs"] for d in got] if timed else args.programs,
+           "unfinished": [d["unfinished"] for d in got],
+           "max_passes": args.max_passes, "seed": args.seed,
            "seconds": gsec, "ops": ops, "ops_per_s": ops / gsec,
+           "residency": eng.t1_choices(),
            "longest": [d["longest"] for d in got],
            "none_found": [d["hist"][63] for d in got], "cpu_baseline": None}
     if not args.no_cpu:
@@ -258,13 +265,17 @@ def run_tester1(args):
             ref = (t.T1SizeStruct * len(cells))()
             rsec, th = pyref.t1_run(C.byref(cfg), ref)
             ref = [t.t1_size_dict(x) for x in ref]
+            rops = sum(d["passes"] * d["len"] * 512 for d in ref)
             strip = lambda L: [{k: v for k, v in d.items() if k != "longest_seed"} for d in L]
-            out["same_summary_as_reference"] = strip(got) == strip(ref)
-            out["cpu_baseline"] = {"value": ops / rsec, "unit": "1-bit ops/s", "cores": th,
-                                   "kind": "reference",
-                                   "sample": "the reference's t1_run on the same %d programs, "
-                                             "%.2f s" % (len(cells) * args.programs, rsec)}
-            if not out["same_summary_as_reference"]:
+            sample = ("the reference's t1_run for %.1f s (%s programs finished, %s cut off)"
+                      % (rsec, [d["programs"] for d in ref], [d["unfinished"] for d in ref])
+                      if timed else "the reference's t1_run on the same %d programs, %.2f s"
+                      % (len(cells) * args.programs, rsec))
+            out["cpu_baseline"] = {"value": rops / rsec, "unit": "1-bit ops/s", "cores": th,
+                                   "kind": "reference", "cpu": cpu_model(), "sample": sample}
+            if not timed:  # timed runs finish different programs: rates only
+                out["same_summary_as_reference"] = strip(got) == strip(ref)
+            if not timed and not out["same_summary_as_reference"]:
                 print(json.dumps(out), flush=True)
                 sys.exit("tester1: the device summary differs from the reference's")
     out["device"] = eng.describe()
@@ -381,6 +392,9 @@ def main():
                     help="batch: programs of the mix (1000); tester1: programs per size (1024)")
     ap.add_argument("--cells", default="8,16,32,64,128,256", help="tester1: machine sizes")
     ap.add_argument("--ratio", type=float, default=4.0, help="tester1: instructions per cell")
+    ap.add_argument("--t1-seconds", type=float, default=0.0,
+                    help="tester1: run for this long (the reference's --seconds) instead of a "
+                         "fixed number of programs")
     ap.add_argument("--max-passes", type=int, default=1 << 20, help="tester1: cycle search limit")
     ap.add_argument("--groups", type=int, default=2048, help="batch: 512-machine groups each")
     ap.add_argument("--passes", type=int, default=64, help="batch: most passes per group")

@@ -188,6 +188,9 @@ typedef struct {
     const ts_line    *rr;
     unsigned          lam_group, lam_groups;  /* lam covers groups [lam_group, lam_group +
                                lam_groups) only (lam_groups x 512 entries); lam_groups 0 = all */
+    unsigned          residency; /* 0 = by the tune's cell thresholds; 1 = the smallest LDS class
+                               that holds the program (HBM if none does); 2 = HBM */
+    unsigned          win;       /* slots per window: 0 = the cost model's choice, 8 or 16 */
 } gpu_batch_item;
 
 typedef struct {
@@ -195,6 +198,9 @@ typedef struct {
     uint64_t lam_sum;   /* sum of the cycle lengths found */
     uint64_t longest;
     uint64_t passes;    /* passes run, summed over the groups (the reference's T1Size.passes) */
+    uint64_t unfinished; /* 1: stopped by the tune's deadline before its search ended (hist,
+                            lam_sum, longest and lam then hold nothing to use; passes the work
+                            done); a program never started has passes 0 */
 } gpu_batch_stats;
 
 typedef struct {
@@ -209,6 +215,9 @@ typedef struct {
                                their machines in LDS in a class of their own, down to two waves per
                                CU (default and upper bound: what two items per CU allow; a value
                                <= lds_cells turns the class off) */
+    double   seconds;       /* > 0: the run stops this long after its launch; the searches in
+                               flight end at their next 64th pass (the reference's stop rule,
+                               src/tester.c:145-148) as unfinished, the rest never start */
 } gpu_batch_tune;
 
 typedef struct {
@@ -232,11 +241,26 @@ int  gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t n, const g
    the reference's order, and the longest cycle's seed is the first program, in that order, that
    reaches it (the reference's single-thread result). programs > 0: that many per size, in
    batches (gpu_batch_run with one group per program) of at most 4,096 per size; programs = 0:
-   rounds of 64 per size until `seconds` have passed, whole programs only, so unfinished stays 0.
+   rounds of programs enough for two waves of every wave slot of the device, until `seconds`
+   have passed: the searches in flight then stop at their next 64th pass and count as
+   unfinished with the passes they ran, as the reference's workers do (src/tester.c:208-216).
+   Each size's machines stay in LDS or in HBM, whichever ran a sample of the first round's
+   programs of that size (at most 32 passes each) at more 1-bit ops/s (gpu_t1_choices).
    *seconds gets the wall time.
    threads, idle and excluded are ignored. */
 int  gpu_t1_run(gpu_engine *e, const ts_t1_config *cfg, ts_t1_size *out, double *seconds, char *err,
                 size_t errlen);
+
+/* The residency the last gpu_t1_run chose per size, with the sample rates it measured (no
+   counterpart in the reference: its threads have one memory). Returns the number of sizes. */
+typedef struct {
+    uint32_t cells;
+    unsigned residency;   /* 1 = LDS, 2 = HBM */
+    double   rate[2];     /* 1-bit ops/s of the sample in LDS and in HBM (0 = not run: too large
+                             for LDS) */
+    uint64_t sample_programs;
+} gpu_t1_choice;
+int  gpu_t1_choices(const gpu_engine *e, gpu_t1_choice *out, int max);
 
 /* Serving: verify / equiv / search requests submitted over time and run with continuous batching.
    A request is copied, compiled on a pool of host threads (hipRTC), admitted to the device when its

@@ -52,6 +52,11 @@ class GpuTune(C.Structure):
                 ("max_waves", C.c_uint), ("sched", C.c_uint), ("group", C.c_uint)]
 
 
+class GpuT1Choice(C.Structure):
+    _fields_ = [("cells", C.c_uint32), ("residency", C.c_uint), ("rate", C.c_double * 2),
+                ("sample_programs", C.c_uint64)]
+
+
 class GpuEnumInfo(C.Structure):
     _fields_ = [("compile_seconds", C.c_double), ("tape", C.c_uint), ("sched", C.c_uint),
                 ("sched_ms", C.c_double * 5), ("sample_words", C.c_uint64)]
@@ -66,17 +71,19 @@ class GpuCodeInfo(C.Structure):
 class GpuBatchItem(C.Structure):
     _fields_ = [("prog", C.POINTER(ProgramStruct)), ("seed", C.c_uint64),
                 ("lam", C.POINTER(C.c_uint64)), ("mem", C.c_void_p), ("rr", C.c_void_p),
-                ("lam_group", C.c_uint), ("lam_groups", C.c_uint)]
+                ("lam_group", C.c_uint), ("lam_groups", C.c_uint), ("residency", C.c_uint),
+                ("win", C.c_uint)]
 
 
 class GpuBatchStats(C.Structure):
     _fields_ = [("hist", C.c_uint64 * 64), ("lam_sum", C.c_uint64), ("longest", C.c_uint64),
-                ("passes", C.c_uint64)]
+                ("passes", C.c_uint64), ("unfinished", C.c_uint64)]
 
 
 class GpuBatchTune(C.Structure):
     _fields_ = [("groups", C.c_uint), ("max_passes", C.c_uint), ("plain", C.c_uint),
-                ("lds_cells", C.c_uint), ("waves_per_cu", C.c_uint), ("lds_large_cells", C.c_uint)]
+                ("lds_cells", C.c_uint), ("waves_per_cu", C.c_uint), ("lds_large_cells", C.c_uint),
+                ("seconds", C.c_double)]
 
 
 class GpuBatchInfo(C.Structure):
@@ -160,6 +167,7 @@ def lib():
                                     C.c_char_p, sz]),
             "gpu_t1_run": (i32, [vp, C.POINTER(T1ConfigStruct), C.POINTER(T1SizeStruct),
                                  C.POINTER(C.c_double), C.c_char_p, sz]),
+            "gpu_t1_choices": (i32, [vp, C.POINTER(GpuT1Choice), i32]),
             "ts_benes_dim": (i32, [i32]),
             "ts_b_new": (vp, []),
             "ts_b_cell": (u32, [vp, C.c_char_p]),
@@ -674,11 +682,14 @@ class Engine:
         return float(lib().gpu_last_ms(self._e))
 
     def batch_run(self, progs, seeds, groups=2048, max_passes=64, plain=False, lds_cells=0,
-                  waves_per_cu=0, lds_large_cells=0, lams=False, starts=None, lam_groups=None):
-        """gpu_batch_run: per program a dict(hist, lam_sum, longest, passes), with lams also "lam"
-        (machine 512 g + k's cycle length, 0 = none found); plus the run info. starts: per program
-        None (t1_start of its seed) or (mem, rr) as uint64 arrays (groups, ncells, 8), (groups, 8).
-        lam_groups: per program a group g, whose 512 cycle lengths alone come back (lams=True)."""
+                  waves_per_cu=0, lds_large_cells=0, lams=False, starts=None, lam_groups=None,
+                  residency=None, win=None, seconds=0.0):
+        """gpu_batch_run: per program a dict(hist, lam_sum, longest, passes, unfinished), with
+        lams also "lam" (machine 512 g + k's cycle length, 0 = none found); plus the run info.
+        starts: per program None (t1_start of its seed) or (mem, rr) as uint64 arrays
+        (groups, ncells, 8), (groups, 8). lam_groups: per program a group g, whose 512 cycle
+        lengths alone come back (lams=True). residency, win: per program (0 = automatic; 1 = LDS,
+        2 = HBM; 8 or 16 slots per window). seconds > 0 stops the run that long after launch."""
         import numpy as np
         n = len(progs)
         items = (GpuBatchItem * n)()
@@ -699,14 +710,19 @@ class Engine:
                 items[i].lam = lam[i].ctypes.data_as(C.POINTER(C.c_uint64))
                 if lam_groups is not None:
                     items[i].lam_group, items[i].lam_groups = int(lam_groups[i]), 1
-        tune = GpuBatchTune(groups, max_passes, int(plain), lds_cells, waves_per_cu, lds_large_cells)
+            if residency is not None:
+                items[i].residency = int(residency[i])
+            if win is not None:
+                items[i].win = int(win[i])
+        tune = GpuBatchTune(groups, max_passes, int(plain), lds_cells, waves_per_cu, lds_large_cells,
+                            float(seconds))
         stats = (GpuBatchStats * n)()
         info = GpuBatchInfo()
         e = _err()
         if not lib().gpu_batch_run(self._e, items, n, C.byref(tune), stats, C.byref(info), e, len(e)):
             raise RuntimeError("gpu_batch_run: " + e.value.decode())
         out = [{"hist": list(st.hist), "lam_sum": st.lam_sum, "longest": st.longest,
-                "passes": st.passes} for st in stats]
+                "passes": st.passes, "unfinished": st.unfinished} for st in stats]
         if lams:
             for o, a in zip(out, lam):
                 o["lam"] = a
@@ -721,6 +737,14 @@ class Engine:
         if not lib().gpu_t1_run(self._e, C.byref(cfg), out, C.byref(sec), e, len(e)):
             raise ValueError("gpu_t1_run: " + e.value.decode())
         return [t1_size_dict(out[i]) for i in range(cfg.nsizes)], sec.value
+
+    def t1_choices(self):
+        """gpu_t1_choices: the residency the last t1_run chose per size, with the sample rates."""
+        out = (GpuT1Choice * 32)()
+        n = lib().gpu_t1_choices(self._e, out, 32)
+        return [{"cells": c.cells, "residency": "LDS" if c.residency == 1 else "HBM",
+                 "lds_ops_per_s": c.rate[0], "hbm_ops_per_s": c.rate[1],
+                 "sample_programs": c.sample_programs} for c in out[:n]]
 
     def compile(self, prog, tune=None, listing=False):
         return Code(self, prog, tune, listing)

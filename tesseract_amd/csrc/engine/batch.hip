@@ -46,7 +46,8 @@ namespace {
 constexpr unsigned kWinSmall = 8;     // slots per window: 8 or 16
 constexpr unsigned kWinLarge = 16;
 constexpr unsigned kLanes = 64;       // words per work item (one wave)
-constexpr unsigned kStats = 67;       // hist[64], lam_sum, longest, passes (sum over groups)
+constexpr unsigned kStats = 68;       // hist[64], lam_sum, longest, passes (sum over groups),
+                                      // items stopped unfinished
 constexpr unsigned kSample = 64;      // cells compared before a full comparison (reference: 64)
 constexpr int kClasses = 4;           // residency classes: LDS small, LDS, LDS large, HBM
 constexpr int kHbm = 3;
@@ -185,7 +186,8 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
                                                uint64_t *__restrict__ stats, uint32_t max_passes,
                                                uint32_t prog_words, uint32_t item_words,
                                                uint64_t *__restrict__ lams,
-                                               const uint64_t *__restrict__ states)
+                                               const uint64_t *__restrict__ states,
+                                               const uint32_t *stop)
 {
     extern __shared__ uint32_t lds[];
     const uint32_t lane = threadIdx.x;
@@ -196,7 +198,13 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
     // class that keeps it in HBM, nothing when both states are in LDS
     const size_t stride = (size_t)(LDS ? (TORT_LDS ? 0 : 1) : 2) * region_cells * item_words;
     uint8_t *hbm = (uint8_t *)(scratch + (size_t)blockIdx.x * stride);
+    // the host's stop flag (pinned host memory, read through to the host each time)
+    auto stopped = [&]() {
+        return stop && __hip_atomic_load(stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u;
+    };
     for (;;) {
+        if (stopped())  // no new item once the run is stopped
+            break;
         uint32_t item = 0;
         if (lane == 0)
             item = atomicAdd(queue, 1u);
@@ -255,6 +263,7 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
             for (int k = 0; k < 4; k++)
                 open_groups |= ((real >> (16 * k)) & 0xFFFFull) ? 1u << k : 0u;
             uint64_t passes = 0;
+            bool aborted = false;
             for (;;) {
                 // the sample list is padded to a multiple of 8 (repeats of its last cell), so eight
                 // loads of each side are in flight at once
@@ -303,6 +312,11 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
                     passes += steps * (uint64_t)__builtin_popcount(open_groups);
                     break;
                 }
+                if ((steps & 63) == 0 && stopped()) {  // the reference's rule, src/tester.c:145
+                    passes += steps * (uint64_t)__builtin_popcount(open_groups);
+                    aborted = true;
+                    break;
+                }
                 if (power == lam) {
                     for (uint32_t c = 2; c < ncells; c++)
                         tort.st(c, lane, hare.ld(c, lane));
@@ -319,6 +333,8 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
                 atomicAdd((unsigned long long *)&st[63], (unsigned long long)left);
             if (lane == 0)
                 atomicAdd((unsigned long long *)&st[66], (unsigned long long)passes);
+            if (aborted && lane == 0)
+                atomicAdd((unsigned long long *)&st[67], 1ull);
         }
         __syncthreads();  // one wave per block: orders this item's LDS use before the next
     }
@@ -555,9 +571,15 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     // launch sized by what a CU can hold
     const uint32_t limit[kClasses] = {std::min<uint32_t>(32, tune.lds_cells), tune.lds_cells,
                                       tune.lds_large_cells, 0xFFFFFFFFu};
-    auto class_of = [&](const ts_program *p) {
-        uint32_t need = p->ncells + 1;
-        return need <= limit[0] ? 0 : need <= limit[1] ? 1 : need <= limit[2] ? 2 : 3;
+    // a caller's residency: 1 = the smallest LDS class with room (up to what two items per CU
+    // allow), 2 = HBM
+    auto class_of_item = [&](size_t i) {
+        const uint32_t need = items[i].prog->ncells + 1;
+        if (items[i].residency == 2)
+            return kHbm;
+        if (items[i].residency == 1)
+            return need <= limit[0] ? 0 : need <= limit[1] ? 1 : need <= two_fit ? 2 : kHbm;
+        return need <= limit[0] ? 0 : need <= limit[1] ? 1 : need <= limit[2] ? 2 : kHbm;
     };
     const uint32_t per_prog = (prog_words + item_words - 1) / item_words;  // items per program
     std::vector<BatchProg> progs(n);
@@ -567,8 +589,9 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     uint32_t maxcells[kClasses] = {0, 0, 0, 0};
     uint64_t class_items[kClasses] = {0, 0, 0, 0};
     uint64_t ops = 0, lam_total = 0;
+    std::vector<int> cls_of(n);
     for (size_t i = 0; i < n; i++) {
-        const int c = class_of(items[i].prog);
+        const int c = cls_of[i] = class_of_item(i);
         class_items[c] += per_prog;
         maxcells[c] = std::max(maxcells[c], items[i].prog->ncells + 1);
     }
@@ -605,11 +628,13 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
         wait[c] = c == kHbm || per_cu[c] <= 2 || class_items[c] <= 2ull * e->cus ? 150.0 : 20.0;
     std::vector<Encoded> encs(n);  // the records of each program, encoded on host threads
     parallel_for(n, [&](size_t i) {
-        encs[i] = pick_encoding(items[i].prog, row, wait[class_of(items[i].prog)]);
+        const unsigned w = items[i].win;
+        encs[i] = w == kWinSmall || w == kWinLarge ? encode(items[i].prog, row, w)
+                                                   : pick_encoding(items[i].prog, row, wait[cls_of[i]]);
     });
     for (size_t i = 0; i < n; i++) {
         const ts_program *p = items[i].prog;
-        const int cls = class_of(p);
+        const int cls = cls_of[i];
         Encoded enc = std::move(encs[i]);
         BatchProg &bp = progs[i];
         bp.rec_off = (uint32_t)(recs.size() / 4);
@@ -696,6 +721,16 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     if (lam_total)
         BCHK(hipMemsetAsync(d_lams.p, 0, lam_total * 8, s));
 
+    // a deadline: the kernels poll a pinned host word the host sets when time is up
+    uint32_t *stop_dev = nullptr;
+    if (tune.seconds > 0 && !tune.plain) {
+        if (!e->batch_stop)
+            BCHK(hipHostMalloc((void **)&e->batch_stop, sizeof(uint32_t),
+                               hipHostMallocCoherent | hipHostMallocMapped));
+        __atomic_store_n(e->batch_stop, 0u, __ATOMIC_SEQ_CST);
+        BCHK(hipHostGetDevicePointer((void **)&stop_dev, e->batch_stop, 0));
+    }
+
     uint64_t launches = 0;
     auto launch_class = [&](int cls, hipStream_t st, uint32_t *queue, uint32_t first,
                             uint32_t count) -> int {
@@ -706,16 +741,19 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
             const size_t lds_bytes = 2 * (size_t)maxcells[cls] * row;
             hipLaunchKernelGGL((ts_batch<true, true>), dim3(grid), dim3(kLanes), lds_bytes, st,
                                d_progs.p, ip, first, count, d_recs.p, d_samples.p, queue, sc,
-                               maxcells[cls], d_stats.p, tune.max_passes, prog_words, item_words, d_lams.p, d_states.p);
+                               maxcells[cls], d_stats.p, tune.max_passes, prog_words, item_words, d_lams.p, d_states.p,
+                               stop_dev);
         } else if (cls != kHbm) {
             const size_t lds_bytes = (size_t)maxcells[cls] * row;
             hipLaunchKernelGGL((ts_batch<true, false>), dim3(grid), dim3(kLanes), lds_bytes, st,
                                d_progs.p, ip, first, count, d_recs.p, d_samples.p, queue, sc,
-                               maxcells[cls], d_stats.p, tune.max_passes, prog_words, item_words, d_lams.p, d_states.p);
+                               maxcells[cls], d_stats.p, tune.max_passes, prog_words, item_words, d_lams.p, d_states.p,
+                               stop_dev);
         } else {
             hipLaunchKernelGGL((ts_batch<false, false>), dim3(grid), dim3(kLanes), 0, st, d_progs.p,
                                ip, first, count, d_recs.p, d_samples.p, queue, sc, maxcells[cls],
-                               d_stats.p, tune.max_passes, prog_words, item_words, d_lams.p, d_states.p);
+                               d_stats.p, tune.max_passes, prog_words, item_words, d_lams.p, d_states.p,
+                               stop_dev);
         }
         launches++;
         return hipGetLastError() == hipSuccess;
@@ -726,21 +764,21 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     // process's four hardware queues (a fifth stream would share one, and its kernels would wait
     // behind that queue's work); the engine's stream joins the side streams after every launch.
     // Plain: one launch per program on the engine's stream.
-    hipStream_t side[2] = {nullptr, nullptr};
+    // the side streams are the engine's, created once (a stream's creation costs more than a
+    // small batch's whole run)
+    bool used[2] = {false, false};
     hipEvent_t start_ev = nullptr, done[kClasses] = {nullptr, nullptr, nullptr, nullptr};
     struct Cleanup {
-        hipStream_t *s;
         hipEvent_t *st, *d;
         ~Cleanup()
         {
-            for (int k = 0; k < 2; k++)
-                if (s[k]) (void)hipStreamDestroy(s[k]);
             for (int k = 0; k < kClasses; k++)
                 if (d[k]) (void)hipEventDestroy(d[k]);
             if (*st) (void)hipEventDestroy(*st);
         }
-    } cleanup{side, &start_ev, done};
+    } cleanup{&start_ev, done};
     BCHK(hipEventRecord(e->ev0, s));
+    const auto t_launch = std::chrono::steady_clock::now();
     if (!tune.plain) {
         BCHK(hipEventCreateWithFlags(&start_ev, hipEventDisableTiming));
         BCHK(hipEventRecord(start_ev, s));
@@ -750,12 +788,14 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
                 continue;
             hipStream_t st = s;
             if (cls != kHbm) {
-                hipStream_t &sd = side[cls == 2 ? 0 : 1];
-                if (!sd) {
-                    BCHK(hipStreamCreateWithFlags(&sd, hipStreamNonBlocking));
-                    BCHK(hipStreamWaitEvent(sd, start_ev, 0));
+                const int k = cls == 2 ? 0 : 1;
+                if (!e->batch_side[k])
+                    BCHK(hipStreamCreateWithFlags(&e->batch_side[k], hipStreamNonBlocking));
+                if (!used[k]) {
+                    BCHK(hipStreamWaitEvent(e->batch_side[k], start_ev, 0));
+                    used[k] = true;
                 }
-                st = sd;
+                st = e->batch_side[k];
             }
             BCHK(hipEventCreate(&done[cls]));
             if (!launch_class(cls, st, d_queue.p + cls, 0, (uint32_t)item_of[cls].size())) {
@@ -765,14 +805,14 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
             BCHK(hipEventRecord(done[cls], st));
         }
         for (int k = 0; k < 2; k++)
-            if (side[k]) {
+            if (used[k]) {
                 hipEvent_t last = k == 0 ? done[2] : (done[0] ? done[0] : done[1]);
                 BCHK(hipStreamWaitEvent(s, last, 0));
             }
     } else {
         // one launch per program, in program order (the baseline the batch is compared with)
         for (size_t i = 0; i < n; i++) {
-            int cls = class_of(items[i].prog);
+            int cls = cls_of[i];
             if (!launch_class(cls, s, d_queue.p + kClasses + i, progs[i].first_item, per_prog)) {
                 say(err, errlen, "%s", "batch launch failed");
                 return 0;
@@ -780,6 +820,21 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
         }
     }
     BCHK(hipEventRecord(e->ev1, s));
+    if (stop_dev) {
+        const auto t_end = t_launch + std::chrono::duration<double>(tune.seconds);
+        for (;;) {
+            const hipError_t q = hipEventQuery(e->ev1);
+            if (q == hipSuccess)
+                break;
+            if (q != hipErrorNotReady)
+                BCHK(q);
+            if (std::chrono::steady_clock::now() >= t_end) {
+                __atomic_store_n(e->batch_stop, 1u, __ATOMIC_SEQ_CST);
+                break;
+            }
+            std::this_thread::sleep_for(std::chrono::microseconds(500));
+        }
+    }
     BCHK(hipEventSynchronize(e->ev1));
     float ms = 0;
     BCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
@@ -796,6 +851,7 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
         stats_out[i].lam_sum = st[i * kStats + 64];
         stats_out[i].longest = st[i * kStats + 65];
         stats_out[i].passes = st[i * kStats + 66];
+        stats_out[i].unfinished = st[i * kStats + 67] ? 1 : 0;
     }
     // executed work, counted as the reference counts it (src/tester.c:305): passes x len x 512
     ops = 0;
@@ -810,7 +866,7 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
         }
     if (info) {
         for (size_t i = 0; i < n; i++)
-            info->class_ops[class_of(items[i].prog)] += st[i * kStats + 66] * items[i].prog->len * 512ull;
+            info->class_ops[cls_of[i]] += st[i * kStats + 66] * items[i].prog->len * 512ull;
         info->ms = ms;
         info->ops = ops;
         info->launches = launches;

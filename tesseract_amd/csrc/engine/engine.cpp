@@ -217,6 +217,11 @@ extern "C" void gpu_engine_close(gpu_engine *e)
     (void)hipSetDevice(e->device);
     if (e->batch_scratch)
         (void)hipFree(e->batch_scratch);
+    if (e->batch_stop)
+        (void)hipHostFree(e->batch_stop);
+    for (hipStream_t &st : e->batch_side)
+        if (st)
+            (void)hipStreamDestroy(st);
     if (e->ev0)
         (void)hipEventDestroy(e->ev0);
     if (e->ev1)

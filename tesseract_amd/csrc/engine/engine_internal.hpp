@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
+
+#include "tesseract_gpu.h"
 
 struct gpu_engine {
     int         device = 0;
@@ -17,6 +20,9 @@ struct gpu_engine {
     // large buffer waits for the device) and released by gpu_engine_close
     void       *batch_scratch = nullptr;
     size_t      batch_scratch_bytes = 0;
+    uint32_t   *batch_stop = nullptr;  // pinned host word a batch with a deadline polls
+    hipStream_t batch_side[2] = {nullptr, nullptr};  // the LDS classes' streams of a batch
+    std::vector<gpu_t1_choice> t1_choices;  // the last tester run's residency per size
 };
 
 #include <cstdint>

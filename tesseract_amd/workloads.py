@@ -66,21 +66,85 @@ def t1_start_group(prog, seed, g):
     return mem, z[-8:].copy()
 
 
+def autotune_batch(engine, progs, seeds, groups, sample=24, sample_passes=4):
+    """The batch autotuner: per machine size of the mix, the slots per window (8, 16, or the host
+    cost model's choice per program) and where its machines live (LDS when two work items of the
+    program fit a CU's LDS, or HBM), picked by timing on samples (the first `sample` programs of
+    each size, `sample_passes` passes each) at their 1-bit ops/s.
+    Windows are a local choice: each size's sample runs alone under every window for each
+    residency. Residency is not: the classes run side by side and share HBM, so every assignment
+    of the sizes that fit LDS to LDS or HBM is timed on the mixed sample of all sizes together,
+    each size with its best window for that residency, and the fastest whole run wins.
+    Returns per-program residency and window lists for gpu_batch_run and the measured tables."""
+    import itertools
+    item_words = 16 if groups == 1 else 32 if groups == 2 else 64
+    lds_most = 160 * 1024 // (2 * item_words * 4)  # cells (the sink included) two items hold
+    by_cells = {}
+    for i, p in enumerate(progs):
+        by_cells.setdefault(p.ncells, []).append(i)
+    sizes = sorted(by_cells)
+    samples = {c: by_cells[c][:sample] for c in sizes}
+    name = lambda res, w: "%s/%s" % ("LDS" if res == 1 else "HBM", w or "model")
+    rate = lambda info: info["ops"] / (info["ms"] * 1e-3) if info["ms"] else 0.0
+    best_win, table = {}, []
+    for c in sizes:
+        idx = samples[c]
+        sp, ss = [progs[i] for i in idx], [seeds[i] for i in idx]
+        rates = {}
+        for res in ((1, 2) if c + 1 <= lds_most else (2,)):
+            for w in (0, 8, 16):
+                _, info = engine.batch_run(sp, ss, groups=groups, max_passes=sample_passes,
+                                           residency=[res] * len(sp), win=[w] * len(sp))
+                rates[(res, w)] = rate(info)
+            best_win[(c, res)] = max((w for w in (0, 8, 16)), key=lambda w: rates[(res, w)])
+        table.append({"cells": c - 2, "sample_programs": len(idx),
+                      "alone": {name(*k): v for k, v in rates.items()}})
+    lds_able = [c for c in sizes if c + 1 <= lds_most]
+    mixed = [i for c in sizes for i in samples[c]]
+    joint = []
+    for combo in itertools.product((1, 2), repeat=len(lds_able)):
+        res_of = {c: 2 for c in sizes}
+        res_of.update(zip(lds_able, combo))
+        _, info = engine.batch_run([progs[i] for i in mixed], [seeds[i] for i in mixed],
+                                   groups=groups, max_passes=sample_passes,
+                                   residency=[res_of[progs[i].ncells] for i in mixed],
+                                   win=[best_win[(progs[i].ncells, res_of[progs[i].ncells])]
+                                        for i in mixed])
+        joint.append((info["ms"], res_of))
+    ms, res_of = min(joint, key=lambda x: x[0])
+    residency, win = [0] * len(progs), [0] * len(progs)
+    for c in sizes:
+        for i in by_cells[c]:
+            residency[i], win[i] = res_of[c], best_win[(c, res_of[c])]
+    for row in table:
+        c = row["cells"] + 2
+        row["choice"] = name(res_of[c], best_win[(c, res_of[c])])
+    tried = [{"LDS": [x - 2 for x in lds_able if r[x] == 1], "mixed_sample_ms": m} for m, r in joint]
+    return residency, win, {"sizes": table, "residency_assignments": tried}
+
+
 def run_batch(engine, n, groups, passes, seed, plain_too=True, lds_cells=0, waves_per_cu=0,
-              lds_large_cells=0, lam_groups=None, keep=None):
-    """The mix batched (and, with plain_too, one launch per program). lam_groups: per program
-    the group whose 512 cycle lengths the batched run returns (for checks); keep, a dict, then
-    receives the programs, seeds and per-program results."""
+              lds_large_cells=0, lam_groups=None, keep=None, tuned=True):
+    """The mix batched (and, with plain_too, one launch per program), with the autotuner's
+    per-size choices when tuned (both forms run with them). lam_groups: per program the group
+    whose 512 cycle lengths the batched run returns (for checks); keep, a dict, then receives
+    the programs, seeds and per-program results."""
     progs, seeds = batch_mix(n, seed)
     if callable(lam_groups):
         lam_groups = lam_groups(progs)
     out = {"workload": "batch", "programs": n, "machines_per_program": groups * 512,
            "passes": passes, "instructions": sum(p.len for p in progs)}
+    residency = win = None
+    if tuned:
+        t0 = time.perf_counter()
+        residency, win, out["autotuner"] = autotune_batch(engine, progs, seeds, groups)
+        out["autotune_s"] = time.perf_counter() - t0
     t0 = time.perf_counter()
     stats, info = engine.batch_run(progs, seeds, groups=groups, max_passes=passes,
                                    lds_cells=lds_cells, waves_per_cu=waves_per_cu,
                                    lds_large_cells=lds_large_cells,
-                                   lams=lam_groups is not None, lam_groups=lam_groups)
+                                   lams=lam_groups is not None, lam_groups=lam_groups,
+                                   residency=residency, win=win)
     if keep is not None:
         keep.update(progs=progs, seeds=seeds, stats=stats, lam_groups=lam_groups)
     out["batched_wall_s"] = time.perf_counter() - t0
@@ -93,7 +157,8 @@ def run_batch(engine, n, groups, passes, seed, plain_too=True, lds_cells=0, wave
     if plain_too:
         stats_p, info_p = engine.batch_run(progs, seeds, groups=groups, max_passes=passes, plain=True,
                                            lds_cells=lds_cells, waves_per_cu=waves_per_cu,
-                                           lds_large_cells=lds_large_cells)
+                                           lds_large_cells=lds_large_cells,
+                                           residency=residency, win=win)
         out["plain_ms"] = info_p["ms"]
         out["plain_ops_per_s"] = info_p["ops"] / (info_p["ms"] * 1e-3)
         out["launches_plain"] = info_p["launches"]
@@ -322,6 +387,9 @@ def main():
     ap.add_argument("--lds-large-cells", type=int, default=0,
                     help="batch: largest program (cells + 1) of the large LDS class "
                          "(0 = what two items per CU allow, 1 = none)")
+    ap.add_argument("--untuned", action="store_true",
+                    help="batch: the fixed cell thresholds and the window cost model instead of "
+                         "the autotuner's measured per-size choices")
     ap.add_argument("--tape-race-ms", type=float, default=0.0,
                     help="serve: tape time up to which a compiled request races on the tape "
                          "(0 = the server's default, negative = never)")
@@ -329,7 +397,7 @@ def main():
     eng = t.Engine(0)
     if a.workload == "batch":
         out = run_batch(eng, a.programs, a.groups, a.passes, a.seed, not a.no_plain, a.lds_cells,
-                        a.waves_per_cu, a.lds_large_cells)
+                        a.waves_per_cu, a.lds_large_cells, tuned=not a.untuned)
     elif a.workload == "life":
         out = run_life(eng, a.gens)
     elif a.workload == "serve":

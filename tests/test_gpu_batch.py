@@ -32,7 +32,7 @@ def expected(prog, seed, groups, max_passes):
                 hist[bucket(lam)] += 1
                 total += lam
                 longest = max(longest, lam)
-    return {"hist": hist, "lam_sum": total, "longest": longest, "passes": passes}
+    return {"hist": hist, "lam_sum": total, "longest": longest, "passes": passes, "unfinished": 0}
 
 
 def programs():
@@ -111,7 +111,8 @@ def test_batch_given_start_states(engine, groups):
                     total += x
                     longest = max(longest, x)
             assert np.array_equal(g.pop("lam"), np.concatenate(lams)), (p.ncells, plain)
-            assert g == {"hist": hist, "lam_sum": total, "longest": longest, "passes": passes}
+            assert g == {"hist": hist, "lam_sum": total, "longest": longest, "passes": passes,
+                         "unfinished": 0}
 
 
 @pytest.mark.parametrize("lds_large_cells", [0, 1])
@@ -170,6 +171,38 @@ def counter(bits):
     return b.finish()
 
 
+def test_batch_2_24_passes(engine):
+    # the reference tester's largest search, 2^24 passes: a 2^21 counter's cycle is found after
+    # 2^22 - 1 passes, a 2^23 counter's not within 2^24 (the whole search runs)
+    progs = [(counter(21), 41), (counter(23), 42)]
+    print("2^24 passes", flush=True)
+    got, _ = engine.batch_run([p for p, _ in progs], [s for _, s in progs], groups=1,
+                              max_passes=1 << 24)
+    for g, bits in zip(got, (21, 23)):
+        found = (1 << (bits + 1)) - 1 <= 1 << 24
+        hist = [0] * 64
+        hist[bits if found else 63] = 512
+        assert g == {"hist": hist, "lam_sum": 512 << bits if found else 0,
+                     "longest": 1 << bits if found else 0,
+                     "passes": (1 << (bits + 1)) - 1 if found else 1 << 24, "unfinished": 0}
+
+
+def test_batch_deadline(engine):
+    # a deadline of 0.5 s: the small programs finish, the long counter search is cut at a 64th
+    # pass and reports unfinished with the passes it ran; the same small programs alone agree
+    small = [(t.random_program(8, 32, 5), 5), (t.random_program(12, 48, 6), 6)]
+    progs = small + [(counter(24), 7)]
+    print("batch with a 0.5 s deadline", flush=True)
+    got, info = engine.batch_run([p for p, _ in progs], [s for _, s in progs], groups=1,
+                                 max_passes=1 << 30, seconds=0.5)
+    assert info["ms"] < 5000
+    long = got[2]
+    assert long["unfinished"] == 1 and long["passes"] > 0 and long["passes"] % 64 == 0
+    alone, _ = engine.batch_run([p for p, _ in small], [s for _, s in small], groups=1,
+                                max_passes=1 << 30)
+    assert got[:2] == alone
+
+
 def test_batch_reference_scale_passes(engine):
     # the reference's tester searches cycles over up to 2^24 passes: 2^20 here, with counters
     # whose cycle (2^12, 2^18) is found and one (2^21) that is not. The 2^12 counter is checked
@@ -188,7 +221,7 @@ def test_batch_reference_scale_passes(engine):
         hist[bits if found else 63] = 512
         assert g == {"hist": hist, "lam_sum": 512 << bits if found else 0,
                      "longest": 1 << bits if found else 0,
-                     "passes": (1 << (bits + 1)) - 1 if found else 1 << 20}
+                     "passes": (1 << (bits + 1)) - 1 if found else 1 << 20, "unfinished": 0}
 
 
 def test_batch_program_without_instructions(engine):
@@ -219,7 +252,7 @@ def test_batch_matches_reference_t1_cycles(engine, ref):
             hist[63 if x == 0 else bucket(x)] += 1
         found = [int(x) for x in lam if x]
         assert g == {"hist": hist, "lam_sum": sum(found), "longest": max(found, default=0),
-                     "passes": ran}, (p.ncells, p.len)
+                     "passes": ran, "unfinished": 0}, (p.ncells, p.len)
 
 
 def test_batch_mix_10000_matches_reference(engine, ref):

@@ -82,14 +82,28 @@ def test_t1_ratio_and_long_search(engine):
 
 
 def test_t1_seconds_mode(engine):
-    cfg = t.t1_config([8, 16], ratio=4.0, programs=0, max_passes=1 << 10, seed=11, seconds=0.2)
-    print("gpu_t1_run: rounds for 0.2 s", flush=True)
+    # small programs: rounds until 0.3 s are up; every finished program adds 512 starts, the
+    # searches cut off add their passes and count as unfinished
+    cfg = t.t1_config([8, 16], ratio=4.0, programs=0, max_passes=1 << 10, seed=11, seconds=0.3)
+    print("gpu_t1_run: rounds for 0.3 s", flush=True)
     got, sec = engine.t1_run(cfg)
-    assert sec >= 0.2
-    n = got[0]["programs"]
-    assert n > 0 and n % 64 == 0 and got[1]["programs"] == n
-    fixed, _ = engine.t1_run(t.t1_config([8, 16], 4.0, n, 1 << 10, 11))
-    assert got == fixed  # whole programs in the same order: the same summary
+    assert 0.3 <= sec < 5.0
+    for d in got:
+        assert d["programs"] > 0 and sum(d["hist"]) == d["starts"] == 512 * d["programs"]
+        assert d["passes"] >= d["programs"]
+    assert [c["cells"] for c in engine.t1_choices()] == [8, 16]
+
+
+def test_t1_seconds_mode_stops_long_searches(engine):
+    # 600-cell programs searched up to 2^30 passes: the stop after 1 s cuts the searches in
+    # flight at a 64th pass; they add their passes but no cycle lengths
+    cfg = t.t1_config([600], ratio=4.0, programs=0, max_passes=1 << 30, seed=12, seconds=1.0)
+    print("gpu_t1_run: 600 cells for 1 s", flush=True)
+    got, sec = engine.t1_run(cfg)
+    d = got[0]
+    assert 1.0 <= sec < 10.0
+    assert d["unfinished"] > 0 and d["passes"] > 0
+    assert sum(d["hist"]) == d["starts"] == 512 * d["programs"]
 
 
 def test_t1_errors(engine):

@@ -377,7 +377,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-parts", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")

@@ -104,6 +104,29 @@ def lib():
         L.t1_merge.argtypes = [C.POINTER(C.c_char_p), C.c_int, vp, C.POINTER(C.c_double),
                                C.POINTER(C.c_int), C.POINTER(C.c_uint64)]
         L.t1_merge.restype = C.c_int
+        # beta (src/beta.h) and gamma (src/gamma.h), the large cube clients
+        L.gen_beta.argtypes = [vp, vp]
+        L.beta_reset.argtypes = [vp]
+        L.beta_step.argtypes = [vp]
+        L.beta_to_lane.argtypes = [vp, vp, C.c_uint, vp]
+        L.beta_from_lane.argtypes = [vp, vp, C.c_uint, vp]
+        L.gen_gamma.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp]
+        L.gen_gamma.restype = C.c_int
+        L.gamma_world_init.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.gamma_world_init.restype = C.c_int
+        L.gamma_world_free.argtypes = [vp]
+        L.gamma_soup.argtypes = [vp, C.c_double, C.c_uint, C.c_uint64]
+        L.gamma_soup.restype = C.c_size_t
+        L.gamma_predators.argtypes = [vp, C.c_size_t, C.c_uint64]
+        L.gamma_predators.restype = C.c_size_t
+        L.gamma_step.argtypes = [vp]
+        L.gamma_to_cube.argtypes = [vp, vp, vp]
+        L.gamma_from_cube.argtypes = [vp, vp, vp]
+        L.gamma_compare.argtypes = [vp, vp, C.POINTER(C.c_char_p)]
+        L.gamma_compare.restype = C.c_long
+        L.lane_cube_init.argtypes = [vp, vp]
+        L.lane_cube_init.restype = C.c_int
+        L.lane_cube_free.argtypes = [vp]
         _r = L
     return _r
 
@@ -303,3 +326,129 @@ def cube_run(prog_ptr, ncells, mem, rr, passes, steps=0):
     mem2 = np.packbits(out, axis=-1, bitorder="little").view(np.uint64).reshape(mem.shape)
     rr2 = np.packbits(outr, axis=-1, bitorder="little").view(np.uint64).reshape(rr.shape)
     return mem2, rr2
+
+
+# ---- beta and gamma state through the reference's own packers ---------------------------------
+
+class Beta(C.Structure):
+    """src/beta.h Beta: one beta computer's state."""
+    _fields_ = [("mem", C.c_uint8 * 256), ("a", C.c_uint8), ("x", C.c_uint8), ("pc", C.c_uint8),
+                ("c", C.c_uint8), ("inp", C.c_uint8), ("in_valid", C.c_uint8), ("out", C.c_uint8),
+                ("out_strobe", C.c_uint8), ("halted", C.c_uint8), ("waiting", C.c_uint8)]
+
+
+# src/lanes.h: LaneMachine (prog, mem, then the 64-byte-aligned rr line at offset 64; 192 bytes in
+# all) and LaneCube (prog, dims, groups, g)
+_LM_SIZE, _LM_MEM, _LM_RR = 192, 8, 64
+
+
+def _machine_lines(base, ncells):
+    """(mem[ncells, 8], rr[8]) of the LaneMachine at address base, copied."""
+    mem_ptr = C.c_void_p.from_address(base + _LM_MEM).value
+    mem = np.ctypeslib.as_array(C.cast(mem_ptr, C.POINTER(C.c_uint64)), (ncells * 8,))
+    rr = np.ctypeslib.as_array(C.cast(base + _LM_RR, C.POINTER(C.c_uint64)), (8,))
+    return mem.reshape(ncells, 8).copy(), rr.copy()
+
+
+def _set_machine_lines(base, ncells, mem, rr):
+    mem_ptr = C.c_void_p.from_address(base + _LM_MEM).value
+    dst = np.ctypeslib.as_array(C.cast(mem_ptr, C.POINTER(C.c_uint64)), (ncells * 8,))
+    dst[:] = np.asarray(mem, np.uint64).reshape(-1)
+    np.ctypeslib.as_array(C.cast(base + _LM_RR, C.POINTER(C.c_uint64)), (8,))[:] = rr
+
+
+def beta_lines(prog_ptr, ncells, cells_ptr, betas):
+    """Lines (groups, ncells, 8), (groups, 8) holding the Beta states `betas` (512 per group) in a
+    LaneMachine of the program, through the reference's beta_to_lane."""
+    L = lib()
+    groups = (len(betas) + 511) // 512
+    mem = np.zeros((groups, ncells, 8), np.uint64)
+    rr = np.zeros((groups, 8), np.uint64)
+    for g in range(groups):  # a fresh machine per group: power-on state, then the betas
+        m = (C.c_uint8 * _LM_SIZE)()
+        if not L.lane_init(m, prog_ptr):
+            raise MemoryError
+        try:
+            for lane in range(512):
+                k = 512 * g + lane
+                if k < len(betas):
+                    L.beta_to_lane(m, cells_ptr, lane, C.byref(betas[k]))
+            mem[g], rr[g] = _machine_lines(C.addressof(m), ncells)
+        finally:
+            L.lane_free(m)
+    return mem, rr
+
+
+def beta_states(prog_ptr, ncells, cells_ptr, mem, rr, count):
+    """The Beta states of lines (groups, ncells, 8) through the reference's beta_from_lane."""
+    L = lib()
+    m = (C.c_uint8 * _LM_SIZE)()
+    if not L.lane_init(m, prog_ptr):
+        raise MemoryError
+    out = []
+    try:
+        for g in range(rr.shape[0]):
+            _set_machine_lines(C.addressof(m), ncells, mem[g], rr[g])
+            for lane in range(512):
+                if len(out) < count:
+                    b = Beta()
+                    L.beta_from_lane(m, cells_ptr, lane, C.byref(b))
+                    out.append(b)
+    finally:
+        L.lane_free(m)
+    return out
+
+
+class GammaWorld:
+    """A reference GammaWorld (src/gamma.h), held in a buffer the reference fills."""
+
+    def __init__(self, xbits, ybits, zbits):
+        self.buf = (C.c_uint8 * 4096)()
+        if not lib().gamma_world_init(self.buf, xbits, ybits, zbits):
+            raise MemoryError
+
+    def __del__(self):
+        if _r is not None:
+            _r.gamma_world_free(self.buf)
+
+
+def gamma_lines(prog_ptr, ncells, cells_ptr, world):
+    """The world's cube state as lines (groups, ncells, 8), (groups, 8): gamma_to_cube into the
+    reference's LaneCube of the program."""
+    L = lib()
+    lc = (C.c_uint8 * 64)()
+    if not L.lane_cube_init(lc, prog_ptr):
+        raise MemoryError
+    try:
+        L.gamma_to_cube(lc, cells_ptr, world.buf)
+        groups = C.c_uint.from_address(C.addressof(lc) + 12).value
+        g = C.c_void_p.from_address(C.addressof(lc) + 16).value
+        mem = np.zeros((groups, ncells, 8), np.uint64)
+        rr = np.zeros((groups, 8), np.uint64)
+        for k in range(groups):
+            mem[k], rr[k] = _machine_lines(g + k * _LM_SIZE, ncells)
+    finally:
+        L.lane_cube_free(lc)
+    return mem, rr
+
+
+def gamma_world_of(prog_ptr, ncells, cells_ptr, mem, rr, world):
+    """gamma_from_cube of lines (groups, ncells, 8) into `world`."""
+    L = lib()
+    lc = (C.c_uint8 * 64)()
+    if not L.lane_cube_init(lc, prog_ptr):
+        raise MemoryError
+    try:
+        g = C.c_void_p.from_address(C.addressof(lc) + 16).value
+        for k in range(rr.shape[0]):
+            _set_machine_lines(g + k * _LM_SIZE, ncells, mem[k], rr[k])
+        L.gamma_from_cube(lc, cells_ptr, world.buf)
+    finally:
+        L.lane_cube_free(lc)
+
+
+def gamma_compare(a, b):
+    """(first differing voxel or -1, the part's name) of two worlds."""
+    what = C.c_char_p()
+    v = lib().gamma_compare(a.buf, b.buf, C.byref(what))
+    return v, (what.value.decode() if what.value else "")

@@ -295,29 +295,31 @@ static void cone(const Aig &g, const std::vector<Lit> &roots, std::vector<uint8_
 }
 
 // Splits a segment's stored outputs into slices of about `target` AIG nodes each, in output order
-// (program order keeps neighbouring cells, which share inputs, together).
-static void slice_outputs(Segment &s, std::vector<uint32_t> stored, unsigned target);
+// (program order keeps neighbouring cells, which share inputs, together). Returns the nodes the
+// slices hold together (a node in the cones of several slices counts in each).
+static uint64_t slice_outputs(Segment &s, std::vector<uint32_t> stored, unsigned target);
 
-// Slices of the outputs every pass stores, then slices of the final_only outputs.
-static void make_slices(Segment &s, unsigned target)
+// Slices of the outputs every pass stores, then slices of the final_only outputs; the nodes they
+// hold together.
+static uint64_t make_slices(Segment &s, unsigned target)
 {
     std::vector<uint32_t> always, last;
     for (uint32_t i = 0; i < s.outs.size(); i++)
         if (s.outs[i].slot >= 0)
             (s.outs[i].final_only ? last : always).push_back(i);
     s.slices.clear();
-    slice_outputs(s, always, target);
+    uint64_t held = slice_outputs(s, always, target);
     s.always = s.slices.size();
-    slice_outputs(s, last, target);
+    return held + slice_outputs(s, last, target);
 }
 
-static void slice_outputs(Segment &s, std::vector<uint32_t> stored, unsigned target)
+static uint64_t slice_outputs(Segment &s, std::vector<uint32_t> stored, unsigned target)
 {
     if (stored.empty())
-        return;
+        return 0;
     if (!target) {
         s.slices.push_back(stored);
-        return;
+        return 0;
     }
     // visit order: by the output's node index, so cells computed together stay together
     std::stable_sort(stored.begin(), stored.end(), [&](uint32_t x, uint32_t y) {
@@ -327,6 +329,7 @@ static void slice_outputs(Segment &s, std::vector<uint32_t> stored, unsigned tar
     std::vector<uint32_t> touched, stack;
     std::vector<uint32_t> slice;
     size_t size = 0;
+    uint64_t held = 0;
     for (uint32_t oi : stored) {
         stack.push_back(s.outs[oi].v >> 1);
         size_t added = 0;
@@ -349,6 +352,7 @@ static void slice_outputs(Segment &s, std::vector<uint32_t> stored, unsigned tar
         }
         slice.push_back(oi);
         size += added;
+        held += added;
         if (size >= target) {
             s.slices.push_back(slice);
             slice.clear();
@@ -360,6 +364,7 @@ static void slice_outputs(Segment &s, std::vector<uint32_t> stored, unsigned tar
     }
     if (!slice.empty())
         s.slices.push_back(slice);
+    return held;
 }
 
 // Slice size when the caller leaves it open. A big cube (2^14 machines and more) is usually one
@@ -392,15 +397,30 @@ static unsigned auto_slice_nodes(const Segment &s, unsigned dims)
 // once, the size is scaled so that the count comes near the wanted one. Life 8192^2 (256
 // wanted): 516 slices ran 6.9 us per generation, 272 6.6 us (the time is not smooth in the
 // slice size: a caller that knows its program passes slice_nodes).
+// Slices recompute the logic their cones share; where outputs' cones overlap so much that the
+// slices would hold more than kMostHeld times the segment's nodes (a program whose every output
+// reads most of the network, like the gamma world's tick: 127 slices, 37 x the nodes at 1,500),
+// the slices grow until they hold at most that.
+static constexpr uint64_t kMostHeld = 3;
+
 static void auto_slices(Segment &s, unsigned dims)
 {
     unsigned target = auto_slice_nodes(s, dims);
-    make_slices(s, target);
-    if (dims < 14)
-        return;
-    const uint64_t want = slices_wanted(dims);
-    if (s.slices.size() > want + want / 2)
-        make_slices(s, (unsigned)std::min<uint64_t>(kSliceLarge, target * s.slices.size() / want));
+    uint64_t held = make_slices(s, target);
+    if (dims >= 14) {
+        const uint64_t want = slices_wanted(dims);
+        if (s.slices.size() > want + want / 2) {
+            target = (unsigned)std::min<uint64_t>(kSliceLarge, target * s.slices.size() / want);
+            held = make_slices(s, target);
+        }
+    }
+    uint64_t nodes = 0;
+    for (const Node &n : s.g.nodes)
+        nodes += n.kind == N_AND || n.kind == N_PERM;
+    while (held > kMostHeld * nodes && target < nodes) {
+        target = (unsigned)std::min<uint64_t>(nodes, 2ull * target);
+        held = make_slices(s, target);
+    }
 }
 
 }  // namespace

@@ -17,6 +17,9 @@ as compiled into oracle/_ref (`make -C oracle ref`, from the sources under /root
   adder8, xor, full_adder, counter4, mux, cube_any, cube_rank, cube_max, rule110
                       examples/*.asm through the reference's assembler (asm_file)
   c17                 examples/c17.bench through the reference's load_program
+  beta                gen_beta() + its BetaCells (x_cells, the struct's words)     src/beta.c:678
+  gamma_<x>_<y>_<z>   gen_gamma(x, y, z) + its GammaCells (x_cells, the struct's bytes) and
+                      x_geometry (x, y, z bits)                                   src/gamma.c:703
 
 Each file is a numpy .npz (load with allow_pickle=False): code (u32, addr | op << 31), ncells,
 names (the cell names, each NUL-terminated), inputs, outputs, kind/dim/src (only for programs with
@@ -142,6 +145,17 @@ def main():
                          "examples/%s.asm" % ex))
     made.append(save("c17", load(L, os.path.join(REF, "examples", "c17.bench")),
                      "examples/c17.bench"))
+    bc = np.zeros(2048, np.uint32)  # BetaCells: 1,325 words, the rest zero
+    p = pyref.Prog()
+    L.gen_beta(p.ptr, bc.ctypes.data)
+    made.append(save("beta", p, "gen_beta() and its BetaCells", cells=bc))
+    for geo in ((3, 3, 3), (4, 4, 4)):
+        gc = np.zeros(8192, np.uint8)  # GammaCells: 2,224 bytes, the rest zero
+        p = pyref.Prog()
+        if not L.gen_gamma(p.ptr, *geo, gc.ctypes.data):
+            raise RuntimeError("gen_gamma%r" % (geo,))
+        made.append(save("gamma_%d_%d_%d" % geo, p, "gen_gamma(%d, %d, %d) and its GammaCells" % geo,
+                         cells=gc, geometry=np.array(geo, np.uint32)))
     total = sum(os.path.getsize(p) for p in made)
     print("%d programs, %.1f KiB" % (len(made), total / 1024))
 

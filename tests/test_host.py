@@ -107,6 +107,14 @@ def _ref_program(ref, made_by):
         bd = np.zeros(args[0] * args[1], np.uint32)
         L.gen_life(p.ptr, *args, bd.ctypes.data)
         extra["board"] = bd
+    elif fn == "gen_beta":
+        bc = np.zeros(2048, np.uint32)
+        L.gen_beta(p.ptr, bc.ctypes.data)
+        extra["cells"] = bc
+    elif fn == "gen_gamma":
+        gc = np.zeros(8192, np.uint8)
+        assert L.gen_gamma(p.ptr, *args, gc.ctypes.data)
+        extra["cells"] = gc
     elif fn == "gen_polymul":
         L.gen_polymul(p.ptr)
         zf = np.zeros((9, 512), np.uint16)

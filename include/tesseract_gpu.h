@@ -48,7 +48,7 @@ typedef struct {
                               2 = iterative-ilp, 3 = max-ilp, 4 = iterative-minreg; 0 = auto: an
                               enumeration of at least 2^40 1-bit ops is compiled with 1, 2 and 3,
                               each timed on a sample of its words, and the fastest kept;
-                              otherwise the default */
+                              otherwise 2, which compiles fastest on large kernels */
     unsigned group;        /* enumerations: 1 = a thread takes one word of 32 assignments at a
                               time; 2 = groups of 32 words, the part of the program no lane bit
                               (x bits 0..4) reaches evaluated once per group, bit-sliced over its
@@ -276,7 +276,9 @@ typedef struct gpu_server gpu_server;
 typedef struct {
     unsigned streams;          /* device slots (default 4) */
     unsigned compile_threads;  /* host compile threads (default 8) */
-    uint64_t memory_budget;    /* bytes of resident compiled code (default 1 GiB) */
+    uint64_t memory_budget;    /* device bytes the admitted requests may hold: resident code
+                                  objects, tapes and the scratch of kernels that spill
+                                  (default 1 GiB) */
     uint64_t chunk_words;      /* words (32 assignments each) per launch (default 2^24) */
     unsigned max_active;       /* admitted requests at once (default 64) */
     double   tape_max_ms;      /* equiv/search requests whose estimated device time on the tape
@@ -292,6 +294,11 @@ typedef struct {
     uint64_t launches;
     uint64_t tier;             /* 0 = compiled kernel, 1 = tape interpreter, 2 = started on the
                                   tape while its kernel compiled, finished compiled */
+    /* its footprint when admitted: the kernel's VGPRs and LDS bytes per workgroup, the waves of
+       it a CU holds at once (which sizes its chunks' grids), and the device bytes it held (code
+       object if loaded for it, tape, scratch of a spilling kernel for its grid share) */
+    uint32_t vgprs, lds_bytes, waves_per_cu;
+    uint64_t hbm_bytes;
 } gpu_request_times;
 /* state: 0 queued, 1 compiling, 2 waiting for admission, 3 running, 4 done, 5 failed */
 int  gpu_server_open(gpu_engine *e, const gpu_serve_tune *tune, gpu_server **out, char *err,

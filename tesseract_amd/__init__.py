@@ -100,7 +100,8 @@ class GpuServeTune(C.Structure):
 class GpuRequestTimes(C.Structure):
     _fields_ = [("submitted_s", C.c_double), ("compiled_s", C.c_double), ("admitted_s", C.c_double),
                 ("started_s", C.c_double), ("done_s", C.c_double), ("launches", C.c_uint64),
-                ("tier", C.c_uint64)]
+                ("tier", C.c_uint64), ("vgprs", C.c_uint32), ("lds_bytes", C.c_uint32),
+                ("waves_per_cu", C.c_uint32), ("hbm_bytes", C.c_uint64)]
 
 
 class GpuTapeStruct(C.Structure):

@@ -899,7 +899,7 @@ int ts_internal::enum_tape(const gpu_engine *e, const gpu_enum *en, gpu_tune &tu
     tune = defaults(e, en->tune);
     std::string msg;
     if (!build_tape(en->a, en->b, plan.in_map.data(), plan.out_map.data(), plan.el, max_slots, t,
-                    msg)) {
+                    msg, plan.spec)) {
         say(err, errlen, "%s", msg.c_str());
         return 0;
     }
@@ -1125,7 +1125,7 @@ extern "C" int gpu_enum_compile(gpu_engine *e, const gpu_enum *en, gpu_enum_code
                 return 0;
             }
         } else {
-            c->sched = std::max(1u, c->tune.sched);
+            c->sched = c->tune.sched ? c->tune.sched : ts_internal::kEnumSched;
             if (!build_module(e, ks.text, &c->mod, &c->compile_seconds, err, errlen, c->sched))
                 return 0;
             if (hipModuleGetFunction(&c->fn, c->mod, "ts_enum") != hipSuccess) {

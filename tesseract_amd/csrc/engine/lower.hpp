@@ -178,9 +178,17 @@ struct Tape {
     size_t   and_nodes = 0;
 };
 
-// Builds the tape of an equiv or search enumeration; false (with a message) if it needs more than
-// max_slots slots or the mode has no tape form (verify, ticks > 1).
+// Builds the tape of an enumeration with one tick; false (with a message) if it needs more than
+// max_slots slots or has no tape form (ticks > 1; a verify spec with a right shift or a shift by a
+// non-constant). A verify's spec is bit-blasted into the graph (spec_bits).
 bool build_tape(const ts_program *a, const ts_program *b, const int *in_map, const int *out_map,
-                const EnumLowering &el, unsigned max_slots, Tape &t, std::string &err);
+                const EnumLowering &el, unsigned max_slots, Tape &t, std::string &err,
+                const ts_spec *spec = nullptr);
+
+// The low `width` bits of the spec's value as AIG literals over the input bits (x bit p =
+// g.InBit(p)): ripple adders, shift-and-add products, bitwise logic, left shifts by constants, all
+// mod 2^width (the low bits of those depend only on the operands' low bits). False for a right
+// shift or a shift by a non-constant, whose low bits need the high ones.
+bool spec_bits(Aig &g, const ts_spec *sp, int width, std::vector<Lit> &bits, std::string &err);
 
 }  // namespace ts

@@ -17,8 +17,9 @@
 // kernel. An equiv/search request whose estimated tape time is at most tape_max_ms runs on the
 // tape; one of up to 0.3 s of tape starts there too, in chunks of about 2 ms, while its kernel
 // compiles, and its remaining chunks run compiled once the module is in (both tiers list the same
-// hits for the same words, so which tier ran a chunk changes nothing in the result). Larger
-// requests, and verify requests, which have no tape, wait for their compile. Chunks are cut as
+// hits for the same words, so which tier ran a chunk changes nothing in the result). A verify's
+// spec is bit-blasted into the tape's graph (a spec with a right shift has no tape form). Larger
+// requests wait for their compile. Chunks are cut as
 // they are launched, each to the size its tier wants. Tape buffers are recycled through a pool,
 // never freed while the server runs (hipFree would wait for the whole device).
 #include <hip/hip_runtime.h>
@@ -77,6 +78,8 @@ struct Module {
     double last_use = 0;
     unsigned blocks_per_cu = 1;
     unsigned group = 1;  // words a thread of the kernel takes per step
+    unsigned vgprs = 0, lds_bytes = 0;  // per thread, per workgroup (function attributes)
+    uint64_t scratch_per_thread = 0;    // private memory of a kernel that spills
 };
 
 // AND nodes per second of the tape interpreter with a file of about 230 slots (two workgroups of
@@ -125,6 +128,7 @@ struct Request {
     uint64_t len_total = 0;
     bool in_compile = false;       // a compile thread reads the request's programs
     bool release_wanted = false;   // released while in_compile: dropped when the compile ends
+    uint64_t held_bytes = 0;       // device bytes counted for it while admitted
 };
 
 struct Slot {
@@ -155,6 +159,7 @@ struct gpu_server {
     std::atomic<bool> stop{false};
     uint64_t next_id = 1;
     size_t module_bytes = 0;
+    uint64_t held_bytes = 0;  // tapes and scratch of the admitted requests (codes: module_bytes)
     std::string fatal;
     std::vector<TapeBuffer> tape_pool;  // free device buffers for tapes
     std::vector<TapeBuffer> tape_all;   // every tape buffer, freed at close
@@ -211,7 +216,7 @@ void plan_loop(gpu_server *s)
             r->state = ST_COMPILING;
             s->preparing++;
         }
-        if (s->tune.tape_max_ms >= 0 && r->en.mode != TS_EN_VERIFY && r->en.ticks == 1) {
+        if (s->tune.tape_max_ms >= 0 && r->en.ticks == 1) {
             char err[1024] = "";
             ts::Tape tp;
             gpu_tune tn{};
@@ -270,7 +275,7 @@ void compile_loop(gpu_server *s)
                 cached = s->modules.count(source) != 0;
             }
             if (!cached)
-                ok = rtc_compile(source, s->e->arch, code, nullptr, err, sizeof err) != 0;
+                ok = rtc_compile(source, s->e->arch, code, nullptr, err, sizeof err, kEnumSched) != 0;
         }
         std::lock_guard<std::mutex> lk(s->mu);
         r->in_compile = false;
@@ -346,7 +351,7 @@ bool admit_module(gpu_server *s, Request *r)
     if (!m.mod) {
         if (r->code.empty()) {  // its module was evicted after the compile found it resident
             char err[512];
-            if (!rtc_compile(r->source, s->e->arch, r->code, nullptr, err, sizeof err)) {
+            if (!rtc_compile(r->source, s->e->arch, r->code, nullptr, err, sizeof err, kEnumSched)) {
                 r->error = err;
                 s->modules.erase(r->source);
                 return false;
@@ -367,6 +372,13 @@ bool admit_module(gpu_server *s, Request *r)
             blocks = 1;
         m.blocks_per_cu = (unsigned)blocks;
         m.group = r->group;
+        int v = 0;
+        if (hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_NUM_REGS, m.fn) == hipSuccess)
+            m.vgprs = (unsigned)v;
+        if (hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_SHARED_SIZE_BYTES, m.fn) == hipSuccess)
+            m.lds_bytes = (unsigned)v;
+        if (hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, m.fn) == hipSuccess)
+            m.scratch_per_thread = (uint64_t)v;
     }
     m.users++;
     m.last_use = now_s();
@@ -376,11 +388,53 @@ bool admit_module(gpu_server *s, Request *r)
     return true;
 }
 
+// the device bytes a compiled request's chunks hold beyond the code: the scratch of a kernel
+// that spills, for the resident threads of its grid share (a 1/streams share of the device)
+uint64_t scratch_bytes(const gpu_server *s, const Module &m, unsigned block)
+{
+    return m.scratch_per_thread * s->e->cus * (uint64_t)m.blocks_per_cu * block /
+           std::max<size_t>(1, s->slots.size());
+}
+
+// the request's footprint, recorded in its times and counted while it is admitted
+void hold(gpu_server *s, Request *r, uint64_t code_bytes)
+{
+    uint64_t bytes = code_bytes;
+    if (r->tape) {
+        bytes += r->tbuf.bytes;
+        r->times.lds_bytes = (uint32_t)tape_lds_bytes(r->td);
+        r->times.waves_per_cu = r->tape_bpc * 4;
+    } else if (r->mod) {
+        const uint64_t scratch = scratch_bytes(s, *r->mod, r->tune.block);
+        bytes += scratch;
+        r->held_bytes += scratch;
+        s->held_bytes += scratch;
+        r->times.vgprs = r->mod->vgprs;
+        r->times.lds_bytes = r->mod->lds_bytes;
+        r->times.waves_per_cu = r->mod->blocks_per_cu * (r->tune.block / 64);
+    }
+    r->times.hbm_bytes = bytes;
+}
+
+// returns what a finished request held
+void unhold(gpu_server *s, Request *r)
+{
+    s->held_bytes -= std::min(s->held_bytes, r->held_bytes);
+    r->held_bytes = 0;
+}
+
 // device side of one request, then its share cut into chunks
 bool admit(gpu_server *s, Request *r)
 {
+    const bool loads = !r->tape && !s->modules.count(r->source);
+    const uint64_t code_bytes = loads ? r->code.size() : 0;
     if (!(r->tape ? admit_tape(s, r) : admit_module(s, r)))
         return false;
+    if (r->tape) {
+        r->held_bytes += r->tbuf.bytes;
+        s->held_bytes += r->tbuf.bytes;
+    }
+    hold(s, r, code_bytes);
     r->si = shares((unsigned)r->en.a->ninputs, r->en.part, r->en.part_end, r->en.parts);
     r->result.space = r->si.space;
     r->result.tried = r->si.tried;
@@ -436,8 +490,9 @@ void upgrade(gpu_server *s, Request *r)
         return;
     size_t need = s->modules.count(r->source) ? 0 : r->code.size();
     evict(s, need);
-    if (need && s->module_bytes + need > s->tune.memory_budget)
+    if (need && s->module_bytes + s->held_bytes + need > s->tune.memory_budget)
         return;
+    const uint64_t code_bytes = need;
     gpu_tune keep = r->tune;
     r->tune = r->upgrade_tune;
     r->group = r->upgrade_group;
@@ -450,6 +505,8 @@ void upgrade(gpu_server *s, Request *r)
     r->tape = false;
     r->upgrade = false;
     r->times.tier = 2;
+    hold(s, r, code_bytes);
+    r->times.hbm_bytes += r->tbuf.bytes;  // the tape's buffer stays with it until it finishes
 }
 
 bool launch_chunk(gpu_server *s, Slot &sl, Request *r, size_t ci)
@@ -572,9 +629,15 @@ void schedule_loop(gpu_server *s)
         // admission: budget and active-request limit
         while (!s->ready.empty() && s->active.size() < s->tune.max_active) {
             Request *r = s->ready.front();
-            size_t need = r->tape || s->modules.count(r->source) ? 0 : r->code.size();
+            // its footprint: a code object not yet resident, a tape, a spilling kernel's scratch
+            // (known once its module was loaded for an earlier request)
+            size_t need = r->tape ? tape_bytes(r->host_tape)
+                                  : s->modules.count(r->source) ? 0 : r->code.size();
+            auto m = s->modules.find(r->source);
+            if (!r->tape && m != s->modules.end())
+                need += scratch_bytes(s, m->second, r->tune.block);
             evict(s, need);
-            if (need && s->module_bytes + need > s->tune.memory_budget && !s->active.empty())
+            if (s->module_bytes + s->held_bytes + need > s->tune.memory_budget && !s->active.empty())
                 break;  // wait for memory to free up
             s->ready.pop_front();
             if (!admit(s, r)) {
@@ -606,6 +669,7 @@ void schedule_loop(gpu_server *s)
             Request *r = s->active[k];
             if (r->chunks_done == r->chunks.size() && r->pos == r->si.nact_words) {
                 bool ok = finalize(s, r);
+                unhold(s, r);
                 if (r->mod)
                     r->mod->users--;
                 if (r->tbuf.ptr) {  // every chunk and re-listing of this tape has completed

@@ -8,25 +8,147 @@
 
 namespace ts {
 
-bool build_tape(const ts_program *a, const ts_program *b, const int *in_map, const int *out_map,
-                const EnumLowering &el, unsigned max_slots, Tape &t, std::string &err)
+bool spec_bits(Aig &g, const ts_spec *sp, int width, std::vector<Lit> &bits, std::string &err)
 {
-    if (el.mode == TS_EN_VERIFY || el.ticks != 1) {
-        err = "only equiv and search with one tick have a tape form";
+    typedef std::vector<Lit> Word;
+    const int W = width;
+    auto add = [&](const Word &x, const Word &y, Lit carry) {
+        Word r(W);
+        for (int i = 0; i < W; i++) {
+            const Lit t = g.Xor(x[i], y[i]);
+            r[i] = g.Xor(t, carry);
+            if (i + 1 < W)
+                carry = g.Or(g.And(x[i], y[i]), g.And(carry, t));
+        }
+        return r;
+    };
+    auto inv = [&](Word x) {
+        for (Lit &l : x)
+            l = lit_not(l);
+        return x;
+    };
+    const int nr = ts_spec_nrpn(sp);
+    const ts_spec_item *rpn = ts_spec_rpn(sp);
+    std::vector<Word> st;
+    for (int i = 0; i < nr; i++) {
+        const ts_spec_item &it = rpn[i];
+        if (it.op == TS_T_NUM) {
+            Word w(W);
+            for (int b = 0; b < W; b++)
+                w[b] = (Lit)((it.v >> b) & 1);
+            st.push_back(w);
+            continue;
+        }
+        if (it.op == TS_T_VAR) {
+            const int k = (int)it.v;
+            Word w(W, 0);
+            for (int b = 0; b < W && b < ts_spec_var_bits(sp, k); b++)
+                w[b] = g.InBit((uint32_t)ts_spec_var_pos(sp, k, b));
+            st.push_back(w);
+            continue;
+        }
+        if (st.empty()) {
+            err = "internal: bad spec program";
+            return false;
+        }
+        if (it.op == TS_T_NOT) {
+            st.back() = inv(st.back());
+            continue;
+        }
+        if (it.op == TS_T_NEG) {  // -x = ~x + 1
+            st.back() = add(inv(st.back()), Word(W, 0), 1);
+            continue;
+        }
+        if (st.size() < 2) {
+            err = "internal: bad spec program";
+            return false;
+        }
+        const Word y = st.back();
+        st.pop_back();
+        Word &x = st.back();
+        switch (it.op) {
+        case TS_T_ADD:
+            x = add(x, y, 0);
+            break;
+        case TS_T_SUB:  // x + ~y + 1
+            x = add(x, inv(y), 1);
+            break;
+        case TS_T_MUL: {  // the rows y_j & (x << j), added
+            Word acc(W, 0);
+            for (int j = 0; j < W; j++) {
+                if (y[j] == 0)
+                    continue;
+                Word row(W, 0);
+                for (int b = j; b < W; b++)
+                    row[b] = g.And(x[b - j], y[j]);
+                acc = add(acc, row, 0);
+            }
+            x = acc;
+            break;
+        }
+        case TS_T_AND:
+        case TS_T_OR:
+        case TS_T_XOR:
+            for (int b = 0; b < W; b++)
+                x[b] = it.op == TS_T_AND ? g.And(x[b], y[b])
+                       : it.op == TS_T_OR ? g.Or(x[b], y[b])
+                                          : g.Xor(x[b], y[b]);
+            break;
+        case TS_T_SHL: {
+            if (i == 0 || rpn[i - 1].op != TS_T_NUM) {
+                err = "a shift by a non-constant has no tape form";
+                return false;
+            }
+            const uint64_t s = rpn[i - 1].v;
+            Word r(W, 0);
+            for (int b = 0; b < W; b++)
+                if ((uint64_t)b >= s)
+                    r[b] = x[b - s];
+            x = r;
+            break;
+        }
+        default:
+            err = "a right shift has no tape form";
+            return false;
+        }
+    }
+    if (st.size() != 1) {
+        err = "internal: bad spec program";
+        return false;
+    }
+    bits = st[0];
+    return true;
+}
+
+bool build_tape(const ts_program *a, const ts_program *b, const int *in_map, const int *out_map,
+                const EnumLowering &el, unsigned max_slots, Tape &t, std::string &err,
+                const ts_spec *spec)
+{
+    if (el.ticks != 1 || (el.mode == TS_EN_VERIFY && !spec)) {
+        err = "only enumerations with one tick have a tape form";
         return false;
     }
     Aig g;
     std::vector<Lit> oa, ob;
     uint32_t a_end = 0, b_end = 0;
     enum_outputs(a, b, in_map, out_map, el, g, oa, ob, &a_end, &b_end);
+    std::vector<Lit> want;  // verify: the spec's bits, after the program's nodes
+    if (el.mode == TS_EN_VERIFY && !spec_bits(g, spec, el.width, want, err))
+        return false;
+    if (el.mode == TS_EN_VERIFY)
+        b_end = (uint32_t)g.nodes.size();
 
-    // the hit word: equiv = OR_k (a_k ^ b_k), search = AND_bit (bit == target bit)
-    // (src/enum.c:150-213)
+    // the hit word: equiv = OR_k (a_k ^ b_k), verify = OR_bit (out ^ spec bit), search =
+    // AND_bit (bit == target bit) (src/enum.c:150-213)
     Lit hit;
     if (el.mode == TS_EN_EQUIV) {
         hit = 0;
         for (size_t k = 0; k < oa.size(); k++)
             hit = g.Or(hit, g.Xor(oa[k], ob[k]));
+    } else if (el.mode == TS_EN_VERIFY) {
+        hit = 0;
+        for (int bit = 0; bit < el.width; bit++)
+            hit = g.Or(hit, g.Xor(oa[bit], want[bit]));
     } else {
         hit = 1;
         for (int bit = 0; bit < el.width; bit++)

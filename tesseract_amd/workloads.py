@@ -349,6 +349,13 @@ def run_serve(engine, n, rate, seed, streams=4, chunk_words=1 << 24, check=True,
                 "n": int(m.sum()), "p50_ms": float(np.percentile(lat[m], 50) * 1e3),
                 "p99_ms": float(np.percentile(lat[m], 99) * 1e3)}
     out["latency_by_input_space"] = classes
+    compiled = [tm for tm in times if tm["tier"] != 1]
+    out["footprint"] = {  # what admission counted (gpu_request_times)
+        "vgprs_max": int(max((tm["vgprs"] for tm in compiled), default=0)),
+        "waves_per_cu_min": int(min((tm["waves_per_cu"] for tm in times), default=0)),
+        "lds_bytes_max": int(max(tm["lds_bytes"] for tm in times)),
+        "hbm_bytes_max": int(max(tm["hbm_bytes"] for tm in times)),
+        "hbm_bytes_sum": int(sum(tm["hbm_bytes"] for tm in times))}
     if keep is not None:
         keep["requests"] = reqs
         keep["results"] = results

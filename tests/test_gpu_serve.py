@@ -61,9 +61,9 @@ def test_server_mix_equals_alone(engine):
     assert all(tm["done_s"] <= big_times["done_s"] for tm in inside)
     adder_res = results.pop(6)
     assert adder_res["hits"] == 65536 and adder_res["hit"] == [2 * h + 1 for h in range(16)]
-    # both tiers served: small requests on the tape interpreter, the big one (far more than 2 s
-    # of tape) and verify compiled
-    assert big_times["tier"] == 0 and times[6]["tier"] == 0
+    # both tiers served: small requests on the tape interpreter, the small verify too (its spec
+    # bit-blasted into the tape), the big one (far more than 2 s of tape) compiled
+    assert big_times["tier"] == 0 and times[6]["tier"] == 1
     assert sum(tm["tier"] == 1 for tm in times) >= 10
     key = lambda r: (r["hits"], r["hit"], r["tried"], r["ops"], r["space"])
     for (a, b, c, r, faulty), got in zip(reqs, results):
@@ -117,10 +117,17 @@ def test_server_tape_then_compiled(engine):
     srv = t.Server(engine, streams=4, tape_max_ms=1e-6, tape_race_ms=1e6)
     tks = [srv.submit(t.EN_EQUIV, a, b) for a, b, _, _, _ in reqs]
     got = [srv.wait(tk) for tk in tks]
-    tiers = [srv.times(tk)["tier"] for tk in tks]
+    times = [srv.times(tk) for tk in tks]
+    tiers = [tm["tier"] for tm in times]
     srv.close()
     print("tiers", tiers, flush=True)
     assert set(tiers) <= {1, 2} and 2 in tiers
+    # the footprint admission counted: a compiled kernel's registers and resident waves, the
+    # code and the tape it held
+    for tm in times:
+        assert tm["waves_per_cu"] > 0 and tm["hbm_bytes"] > 0
+        if tm["tier"] == 2:
+            assert 0 < tm["vgprs"] <= 512
     key = lambda r: (r["hits"], r["hit"], r["tried"], r["ops"], r["space"])
     for (a, b, c, r, faulty), res in zip(reqs, got):
         assert key(res) == key(engine.enum_run(t.EN_EQUIV, a, b))

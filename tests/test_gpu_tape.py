@@ -97,3 +97,13 @@ def test_tape_partial_rounds(engine, n_in):
     o = O.enum(t.EN_EQUIV, a, b, b_in=b.inputs, b_out=b.outputs)
     assert (got["hits"], got["hit"], got["tried"]) == (o["hits"], o["hit"], o["tried"])
     assert _same(got, engine.enum_run(t.EN_EQUIV, a, b))
+
+
+@pytest.mark.parametrize("name,spec", [("mul8", "p = a * b"), ("mul8", "p = a * b + (a & b & 1)"),
+                                       ("mul10", "p = a * b + 3"), ("adder8", "c:s = a + b")])
+def test_tape_verify(engine, name, spec):
+    # a verify's spec bit-blasted into the tape: the compiled kernel's hits, count and 16 smallest
+    prog = golden(name)
+    print("tape verify", name, spec, flush=True)
+    got = engine.enum_run(t.EN_VERIFY, prog, spec=spec, tune=TAPE)
+    assert _same(got, engine.enum_run(t.EN_VERIFY, prog, spec=spec))

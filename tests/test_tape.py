@@ -105,10 +105,27 @@ def test_tape_constant_hit_words():
     assert run_tape(tp, 9) == (512, list(range(16)))
 
 
+@pytest.mark.parametrize("name,spec", [
+    ("adder8", "c:s = a + b + c"), ("adder8", "c:s = a + b"), ("adder8", "c:s = b - ~a - 1 + c"),
+    ("mul6", "p = a * b"), ("mul6", "p = a * b + (a & b & 1)"), ("mul6", "p = (a << 2) * b"),
+    ("mul6", "p = -(a * b) ^ 5"), ("mul8", "p = a * b"), ("mul8", "p = a * b | 0x100")])
+def test_tape_verify_matches_oracle(name, spec):
+    # a verify's spec bit-blasted into the tape (adders, shift-and-add products, logic, shifts):
+    # the same hits as the oracle's enumeration against the spec's values
+    prog = golden(name)
+    tp = t.enum_tape(t.EN_VERIFY, prog, spec=spec)
+    sp = t.Spec(spec, prog)
+    n = len(prog.inputs)
+    expect = np.array([sp.value(x) for x in range(1 << n)], np.uint64)
+    cells = [sp.out_cell(i) for i in range(sp.width)]
+    o = O.enum(t.EN_VERIFY, prog, out_cells=cells, expect=expect)
+    assert run_tape(tp, n) == (o["hits"], o["hit"])
+
+
 def test_tape_refuses_what_it_cannot_run():
     prog = adder(4)
     with pytest.raises(ValueError, match="tape form"):
-        t.enum_tape(t.EN_VERIFY, prog, spec="c:s = a + b")
+        t.enum_tape(t.EN_VERIFY, prog, spec="c:s = (a + b) >> 1")
     rng = np.random.default_rng(5)
     c = N.circuit(12, 8, 3000, rng)
     with pytest.raises(ValueError, match="slots"):

@@ -175,7 +175,8 @@ int  gpu_enum_exec_noalloc(gpu_enum_code *c, unsigned list_words, ts_enum_result
    work queue, not compiled one by one. stats[i] is program i's histogram (bucket b: cycle length
    in (2^(b-1), 2^b], src/tester.c:42-48; bucket 63: none within max_passes). plain = 1 runs the same
    kernel one launch per program, the baseline a batch is compared with. A work item is one wave
-   of 64 words (2,048 machines); a program of one or two groups gets items of 16 or 32 words. */
+   of 64 words (2,048 machines); a program of one or two groups gets items of 16 or 32 words, and
+   a caller may ask for narrower items per program. */
 typedef struct {
     const ts_program *prog;
     uint64_t          seed;
@@ -189,8 +190,15 @@ typedef struct {
     unsigned          lam_group, lam_groups;  /* lam covers groups [lam_group, lam_group +
                                lam_groups) only (lam_groups x 512 entries); lam_groups 0 = all */
     unsigned          residency; /* 0 = by the tune's cell thresholds; 1 = the smallest LDS class
-                               that holds the program (HBM if none does); 2 = HBM */
+                               that holds the program (HBM if none does); 2 = HBM; 3, 4, 5 = LDS
+                               class 0, 1, 2 (small, medium, large), whose launch sizes its LDS for
+                               its largest item: a caller's own grouping (at most one CU's LDS) */
     unsigned          win;       /* slots per window: 0 = the cost model's choice, 8 or 16 */
+    unsigned          words;     /* words (of 32 machines) per work item: 16, 32 or 64; 0 = by
+                               groups (16 for one group, 32 for two, else 64). A narrower item
+                               needs less LDS, so more waves of an LDS class share a CU (each with
+                               idle lanes); the classes' cell thresholds count cells of the
+                               default width, a program's need is its bytes */
 } gpu_batch_item;
 
 typedef struct {
@@ -218,6 +226,9 @@ typedef struct {
     double   seconds;       /* > 0: the run stops this long after its launch; the searches in
                                flight end at their next 64th pass (the reference's stop rule,
                                src/tester.c:145-148) as unfinished, the rest never start */
+    unsigned sink_branch;   /* bit c set: residency class c (LDS small, LDS, LDS large, HBM)
+                               branches around the loads and stores of slots without a NAND or a
+                               STO; clear: such slots load and store a sink cell (no branch) */
 } gpu_batch_tune;
 
 typedef struct {
@@ -339,9 +350,11 @@ int  gpu_enum_tape(const gpu_enum *en, unsigned max_slots, gpu_tape *out, char *
 void gpu_tape_free(gpu_tape *t);
 /* The batch interpreter's records of a program (no device; no reference counterpart: the
    reference compiles each program, this lists what gpu_batch_run runs instead): windows of `win`
-   slots (8 or 16), each 2 x win words, the load offsets (bit 31 = NAND) then the store offsets, in
-   bytes of cell x row with row = 4 x item_words; the sink is cell ncells. Writes up to cap words
-   to out and the total to *nwords; fails for a window size other than 8 or 16. */
+   slots (8 or 16), each 2 x win words, the load offsets (bit 31 = NAND) then the store offsets
+   (bit 31 = the slot stores nothing), in bytes of cell x row with row = 4 x item_words; the sink
+   is cell ncells, named by slots without a NAND or a STO. Writes up to cap words to out and the
+   total to *nwords; fails for a window size other than 8 or 16 or item words other than 16, 32
+   or 64. */
 int  gpu_batch_records(const ts_program *p, unsigned item_words, unsigned win, uint32_t *out,
                        size_t cap, size_t *nwords, char *err, size_t errlen);
 int  gpu_rtc_check(const char *source, const char *arch, double *seconds, size_t *code_bytes,

@@ -72,7 +72,7 @@ class GpuBatchItem(C.Structure):
     _fields_ = [("prog", C.POINTER(ProgramStruct)), ("seed", C.c_uint64),
                 ("lam", C.POINTER(C.c_uint64)), ("mem", C.c_void_p), ("rr", C.c_void_p),
                 ("lam_group", C.c_uint), ("lam_groups", C.c_uint), ("residency", C.c_uint),
-                ("win", C.c_uint)]
+                ("win", C.c_uint), ("words", C.c_uint)]
 
 
 class GpuBatchStats(C.Structure):
@@ -83,7 +83,7 @@ class GpuBatchStats(C.Structure):
 class GpuBatchTune(C.Structure):
     _fields_ = [("groups", C.c_uint), ("max_passes", C.c_uint), ("plain", C.c_uint),
                 ("lds_cells", C.c_uint), ("waves_per_cu", C.c_uint), ("lds_large_cells", C.c_uint),
-                ("seconds", C.c_double)]
+                ("seconds", C.c_double), ("sink_branch", C.c_uint)]
 
 
 class GpuBatchInfo(C.Structure):
@@ -599,7 +599,8 @@ def enum_tape(mode, a, b=None, spec=None, max_slots=0):
 
 def batch_records(prog, item_words=64, win=16):
     """The batch interpreter's window records of a program (no device): an array of windows x 2 x
-    win (load offsets with bit 31 = NAND, then store offsets; bytes of cell x 4 x item_words)."""
+    win (load offsets with bit 31 = NAND, then store offsets with bit 31 = nothing stored; bytes of
+    cell x 4 x item_words)."""
     import numpy as np
     n = C.c_size_t()
     e = _err()
@@ -684,13 +685,16 @@ class Engine:
 
     def batch_run(self, progs, seeds, groups=2048, max_passes=64, plain=False, lds_cells=0,
                   waves_per_cu=0, lds_large_cells=0, lams=False, starts=None, lam_groups=None,
-                  residency=None, win=None, seconds=0.0):
+                  residency=None, win=None, seconds=0.0, words=None, sink_branch=0):
         """gpu_batch_run: per program a dict(hist, lam_sum, longest, passes, unfinished), with
         lams also "lam" (machine 512 g + k's cycle length, 0 = none found); plus the run info.
         starts: per program None (t1_start of its seed) or (mem, rr) as uint64 arrays
         (groups, ncells, 8), (groups, 8). lam_groups: per program a group g, whose 512 cycle
         lengths alone come back (lams=True). residency, win: per program (0 = automatic; 1 = LDS,
-        2 = HBM; 8 or 16 slots per window). seconds > 0 stops the run that long after launch."""
+        2 = HBM; 8 or 16 slots per window). seconds > 0 stops the run that long after launch.
+        words: per program the words (of 32 machines) of its work items, 16, 32 or 64 (0 = by
+        groups). sink_branch: bit c set = class c branches around slots' missing loads and
+        stores."""
         import numpy as np
         n = len(progs)
         items = (GpuBatchItem * n)()
@@ -715,8 +719,10 @@ class Engine:
                 items[i].residency = int(residency[i])
             if win is not None:
                 items[i].win = int(win[i])
+            if words is not None:
+                items[i].words = int(words[i])
         tune = GpuBatchTune(groups, max_passes, int(plain), lds_cells, waves_per_cu, lds_large_cells,
-                            float(seconds))
+                            float(seconds), sink_branch)
         stats = (GpuBatchStats * n)()
         info = GpuBatchInfo()
         e = _err()

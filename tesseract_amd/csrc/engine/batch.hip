@@ -51,15 +51,19 @@ constexpr unsigned kStats = 68;       // hist[64], lam_sum, longest, passes (sum
 constexpr unsigned kSample = 64;      // cells compared before a full comparison (reference: 64)
 constexpr int kClasses = 4;           // residency classes: LDS small, LDS, LDS large, HBM
 constexpr int kHbm = 3;
-
 struct BatchProg {
     uint32_t rec_off;    // first window record (16-byte units)
     uint32_t nrec;       // windows per pass (2 x win words each)
     uint32_t win;        // slots per window (8 or 16)
     uint32_t ncells;     // cells incl. ZERO and ONE (the sink is cell ncells)
-    uint32_t nsample;    // sampled cells (<= 64)
-    uint32_t sample_off; // into the sample list
+    uint32_t nsample;    // sampled stored cells (<= 64; the list is padded to a multiple of 8)
+    uint32_t sample_off; // into the cell lists: the sample
+    uint32_t nstored;    // distinct cells a pass stores (live STOs)
+    uint32_t cells_off;  // into the cell lists: the stored cells, padded to a multiple of 8 with
+    uint32_t nstored8;   //   repeats (nstored8 entries), then the cells read but never stored
+    uint32_t ncells_used;//   (ncells_used entries in all)
     uint32_t first_item; // global index of the program's first work item
+    uint32_t words;      // words (of 32 machines) per work item: 16, 32 or 64
     uint64_t seed;       // t1_start stream seed
     uint64_t lam_off;    // first of the program's per-machine cycle lengths, or kNoLam
     uint64_t state_off;  // first u64 of the program's start state, or kNoLam (t1_start stream)
@@ -98,24 +102,31 @@ struct Cells {
     }
 };
 
-// One window: its W loads issue together, then the W slots run in order.
-template <unsigned W>
+// One window: its W loads issue together, then the W slots run in order. A load word has bit 31
+// set for a NAND; a store word has bit 31 set when its slot stores nothing. Both then name the
+// sink, which SKIP = false touches like any cell (no branch at all) and SKIP = true branches
+// around on the scalar unit (the words are wave-uniform), saving the memory instructions.
+template <unsigned W, bool SKIP>
 __device__ __forceinline__ void run_window(const Cells &h, uint32_t lane4, const uint32_t (&w)[2 * W],
                                            uint32_t &rr)
 {
     uint32_t v[W];
 #pragma unroll
-    for (unsigned k = 0; k < W; k++)
-        v[k] = h.ld_at(w[k] & 0x7FFFFFFFu, lane4);
+    for (unsigned k = 0; k < W; k++) {
+        v[k] = 0u;
+        if (!SKIP || (int32_t)w[k] < 0)
+            v[k] = h.ld_at(w[k] & 0x7FFFFFFFu, lane4);
+    }
 #pragma unroll
     for (unsigned k = 0; k < W; k++) {
         const uint32_t m = (uint32_t)((int32_t)w[k] >> 31);  // all ones for a NAND
         rr = __builtin_amdgcn_bitop3_b32(m, rr, v[k], 0x7Cu); // m ? !(rr & v) : rr
-        h.st_at(w[W + k], lane4, rr);
+        if (!SKIP || (int32_t)w[W + k] >= 0)
+            h.st_at(w[W + k] & 0x7FFFFFFFu, lane4, rr);
     }
 }
 
-template <unsigned W>
+template <unsigned W, bool SKIP>
 __device__ __forceinline__ void run_windows(const Cells &h, uint32_t lane4, const uint4 *rec,
                                             uint32_t nwin, uint32_t &rr)
 {
@@ -130,7 +141,7 @@ __device__ __forceinline__ void run_windows(const Cells &h, uint32_t lane4, cons
             w[4 * q + 2] = x.z;
             w[4 * q + 3] = x.w;
         }
-        run_window<W>(h, lane4, w, rr);
+        run_window<W, SKIP>(h, lane4, w, rr);
     }
 }
 
@@ -138,12 +149,19 @@ __device__ __forceinline__ void run_windows(const Cells &h, uint32_t lane4, cons
 // (a search of few programs) waits about one memory latency per window (its loads follow the
 // previous window's stores) and then runs the window's dependent chain of W slots.
 __device__ __forceinline__ void run_pass(const Cells &h, uint32_t lane, const uint4 *rec,
-                                         uint32_t nwin, uint32_t win, uint32_t &rr)
+                                         uint32_t nwin, uint32_t win, bool skip, uint32_t &rr)
 {
-    if (win == kWinLarge)
-        run_windows<kWinLarge>(h, lane * 4, rec, nwin, rr);
-    else
-        run_windows<kWinSmall>(h, lane * 4, rec, nwin, rr);
+    if (skip) {
+        if (win == kWinLarge)
+            run_windows<kWinLarge, true>(h, lane * 4, rec, nwin, rr);
+        else
+            run_windows<kWinSmall, true>(h, lane * 4, rec, nwin, rr);
+    } else {
+        if (win == kWinLarge)
+            run_windows<kWinLarge, false>(h, lane * 4, rec, nwin, rr);
+        else
+            run_windows<kWinSmall, false>(h, lane * 4, rec, nwin, rr);
+    }
 }
 
 // sum over the wave's active lanes of popcount(m): one ballot per bit, so lanes that idle (items
@@ -182,22 +200,22 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
                                                const uint4 *__restrict__ recs,
                                                const uint32_t *__restrict__ samples,
                                                uint32_t *__restrict__ queue,
-                                               uint32_t *__restrict__ scratch, uint32_t region_cells,
+                                               uint32_t *__restrict__ scratch, uint32_t region_bytes,
                                                uint64_t *__restrict__ stats, uint32_t max_passes,
-                                               uint32_t prog_words, uint32_t item_words,
+                                               uint32_t prog_words, uint32_t skip_sink,
                                                uint64_t *__restrict__ lams,
                                                const uint64_t *__restrict__ states,
                                                const uint32_t *stop)
 {
     extern __shared__ uint32_t lds[];
     const uint32_t lane = threadIdx.x;
-    const uint32_t row = item_words * 4;
     // the hare (every instruction touches it) in LDS or HBM; the tortoise (Brent's compare and
     // copy only) in this wave's HBM region or after the hare in LDS
     // a wave's HBM region: hare and tortoise for the HBM class, the tortoise alone for an LDS
-    // class that keeps it in HBM, nothing when both states are in LDS
-    const size_t stride = (size_t)(LDS ? (TORT_LDS ? 0 : 1) : 2) * region_cells * item_words;
-    uint8_t *hbm = (uint8_t *)(scratch + (size_t)blockIdx.x * stride);
+    // class that keeps it in HBM, nothing when both states are in LDS; region_bytes holds the
+    // class's largest item
+    const size_t stride = (size_t)(LDS ? (TORT_LDS ? 0 : 1) : 2) * region_bytes;
+    uint8_t *hbm = (uint8_t *)scratch + (size_t)blockIdx.x * stride;
     // the host's stop flag (pinned host memory, read through to the host each time)
     auto stopped = [&]() {
         return stop && __hip_atomic_load(stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u;
@@ -211,15 +229,14 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
         item = __builtin_amdgcn_readfirstlane(__shfl(item, 0, 64));
         if (item >= nitems)
             break;
+        item += item_base;
+        const uint32_t pi = __builtin_amdgcn_readfirstlane(item_prog[item]);
+        const BatchProg P = progs[pi];
+        const uint32_t item_words = P.words, row = item_words * 4;
         if (lane < item_words) {  // an item narrower than the wave: the other lanes idle
-            item += item_base;
-            const uint32_t pi = __builtin_amdgcn_readfirstlane(item_prog[item]);
-            const BatchProg P = progs[pi];
             const uint32_t ncells = P.ncells;
             Cells hare{LDS ? (uint8_t *)lds : hbm, row};
-            Cells tort{TORT_LDS ? (uint8_t *)lds + (size_t)region_cells * row
-                                : LDS ? hbm : hbm + (size_t)region_cells * row,
-                       row};
+            Cells tort{TORT_LDS ? (uint8_t *)lds + region_bytes : LDS ? hbm : hbm + region_bytes, row};
             const uint4 *rec = recs + P.rec_off;
             uint64_t *st = stats + (size_t)pi * kStats;
 
@@ -234,7 +251,11 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
             // a caller's start state: group g's lines of cells 0..ncells-1, then its RR line
             const bool given = P.state_off != kNoLam && word < prog_words;
             const uint64_t *gs = states + (given ? P.state_off + g * (ncells + 1) * 8 + w64 : 0);
-            for (uint32_t c = 2; c < ncells; c++) {
+            // only cells the program reads or stores: the others never change and never
+            // matter (equal in hare and tortoise alike)
+            const uint32_t *used = samples + P.cells_off;
+            for (uint32_t i = 0; i < P.ncells_used; i++) {
+                const uint32_t c = used[i];
                 uint64_t k = g * per_group + (uint64_t)(c - 2) * 8 + w64;
                 uint64_t v = given ? gs[(size_t)c * 8] : mix64(s0 + (k + 1) * 0x9E3779B97F4A7C15ull);
                 hare.st(c, lane, hi ? (uint32_t)(v >> 32) : (uint32_t)v);
@@ -249,10 +270,12 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
             }
 
             // Brent: the tortoise jumps to the hare at powers of two (reference src/tester.c:96-164)
-            for (uint32_t c = 0; c < ncells; c++)
-                tort.st(c, lane, hare.ld(c, lane));
+            // Brent's state compares and tortoise copies cover the stored cells and RR: the
+            // state is equal exactly when those are
+            for (uint32_t i = 0; i < P.nstored8; i++)
+                tort.st(used[i], lane, hare.ld(used[i], lane));
             uint32_t trr = rr;
-            run_pass(hare, lane, rec, P.nrec, P.win, rr);
+            run_pass(hare, lane, rec, P.nrec, P.win, skip_sink, rr);
             uint64_t lam = 1, power = 1, steps = 1;
             uint32_t active = word < prog_words ? 0xFFFFFFFFu : 0u;
             const uint32_t *smp = samples + P.sample_off;
@@ -277,20 +300,17 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
                     for (int k = 0; k < 8; k++)
                         d |= x[k];
                 }
-                if (P.nsample < ncells - 2 && __any(active & ~d)) {
+                if (P.nsample < P.nstored && __any(active & ~d)) {
                     d = rr ^ trr;
-                    uint32_t c = 2;
-                    for (; c + 8 <= ncells; c += 8) {
+                    for (uint32_t i = 0; i < P.nstored8; i += 8) {
                         uint32_t x[8];
 #pragma unroll
                         for (int k = 0; k < 8; k++)
-                            x[k] = hare.ld(c + k, lane) ^ tort.ld(c + k, lane);
+                            x[k] = hare.ld(used[i + k], lane) ^ tort.ld(used[i + k], lane);
 #pragma unroll
                         for (int k = 0; k < 8; k++)
                             d |= x[k];
                     }
-                    for (; c < ncells; c++)
-                        d |= hare.ld(c, lane) ^ tort.ld(c, lane);
                 }
                 const uint32_t found = active & ~d;
                 if (__any(found)) {
@@ -318,13 +338,13 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
                     break;
                 }
                 if (power == lam) {
-                    for (uint32_t c = 2; c < ncells; c++)
-                        tort.st(c, lane, hare.ld(c, lane));
+                    for (uint32_t i = 0; i < P.nstored8; i++)
+                        tort.st(used[i], lane, hare.ld(used[i], lane));
                     trr = rr;
                     power *= 2;
                     lam = 0;
                 }
-                run_pass(hare, lane, rec, P.nrec, P.win, rr);
+                run_pass(hare, lane, rec, P.nrec, P.win, skip_sink, rr);
                 lam++;
                 steps++;
             }
@@ -341,27 +361,53 @@ __global__ void __launch_bounds__(64) ts_batch(const BatchProg *__restrict__ pro
 }
 
 struct Encoded {
-    std::vector<uint32_t> words;  // windows of 2 x win words: win load offsets (| NAND << 31), then win store offsets
+    std::vector<uint32_t> words;  // windows of 2 x win words: win load offsets (| 1 << 31 for a
+                                  // NAND), then win store offsets (| 1 << 31: nothing stored)
     uint32_t nwin = 0, win = 0;
+    std::vector<uint32_t> stored;     // cells a live STO writes, ascending
+    std::vector<uint32_t> read_only;  // writable cells a NAND reads that no live STO writes, ascending
 };
+
+// STOs whose value no one sees: the next access to the cell later in the same pass is another
+// STO. The state after each pass (all the interpreter and Brent's compares ever observe) is the
+// same without them. A STO whose cell is not accessed again in the pass stays: its value is the
+// cell's end-of-pass value. Random programs lose 42-53 % of their STOs this way.
+std::vector<uint8_t> dead_stores(const ts_program *p)
+{
+    std::vector<uint8_t> dead(p->len, 0);
+    std::vector<uint8_t> stored_next(p->ncells, 0);  // the next access later in the pass is a STO
+    for (size_t i = p->len; i-- > 0;) {
+        const ts_instr in = p->code[i];
+        if (in.op == TS_OP_NAND) {
+            stored_next[in.addr] = 0;
+        } else {
+            dead[i] = stored_next[in.addr];
+            stored_next[in.addr] = 1;
+        }
+    }
+    return dead;
+}
 
 // Windows of up to `win` slots in which no NAND reads a cell stored earlier in the window.
 // Offsets are bytes into an item's cells (cell * row). A slot is a NAND (or nothing: a load of the
 // sink cell) followed by a STO (or nothing: a store to the sink): a STO right after a NAND shares
 // its slot, since the slot stores rr after the NAND, so a random program of NANDs and STOs in
-// equal parts takes about three slots per four instructions. A STO to a read-only cell (ignored
-// by the machine) takes no slot; padding loads and stores the sink.
+// equal parts takes about three slots per four instructions, and about 0.6 once dead STOs are
+// dropped. A STO to a read-only cell (ignored by the machine) takes no slot; padding loads and
+// stores the sink.
 Encoded encode(const ts_program *p, uint32_t row, unsigned win_slots)
 {
     Encoded e;
     e.win = win_slots;
     const uint32_t sink = p->ncells * row;
+    const std::vector<uint8_t> dead = dead_stores(p);
     std::vector<uint32_t> lds_, sts, stored;  // the open window's load and store offsets
     bool open_nand = false;                   // the last slot is a NAND that stores nothing yet
+    const uint32_t no_store = sink | 0x80000000u;  // a slot that stores nothing
     auto flush = [&]() {
         while (lds_.size() < win_slots) {
             lds_.push_back(sink);
-            sts.push_back(sink);
+            sts.push_back(no_store);
         }
         e.words.insert(e.words.end(), lds_.begin(), lds_.end());
         e.words.insert(e.words.end(), sts.begin(), sts.end());
@@ -378,11 +424,12 @@ Encoded encode(const ts_program *p, uint32_t row, unsigned win_slots)
                 std::find(stored.begin(), stored.end(), in.addr) != stored.end())
                 flush();
             lds_.push_back(in.addr * row | 0x80000000u);
-            sts.push_back(sink);
+            sts.push_back(no_store);
             open_nand = true;
             continue;
         }
-        if (in.addr < TS_CELL_FIRST_FREE || (p->special && p->special[in.addr].kind != TS_CK_PLAIN))
+        if (in.addr < TS_CELL_FIRST_FREE || (p->special && p->special[in.addr].kind != TS_CK_PLAIN) ||
+            dead[i])
             continue;
         if (open_nand) {
             sts.back() = in.addr * row;
@@ -397,6 +444,20 @@ Encoded encode(const ts_program *p, uint32_t row, unsigned win_slots)
     }
     if (!lds_.empty())
         flush();
+    std::vector<uint8_t> use(p->ncells, 0);  // 1 read, 2 stored
+    for (size_t i = 0; i < p->len; i++) {
+        const ts_instr in = p->code[i];
+        if (in.op == TS_OP_NAND)
+            use[in.addr] |= 1;
+        else if (!dead[i] && in.addr >= TS_CELL_FIRST_FREE &&
+                 !(p->special && p->special[in.addr].kind != TS_CK_PLAIN))
+            use[in.addr] |= 2;
+    }
+    for (uint32_t c = TS_CELL_FIRST_FREE; c < p->ncells; c++)
+        if (use[c] & 2)
+            e.stored.push_back(c);
+        else if (use[c])
+            e.read_only.push_back(c);
     return e;
 }
 
@@ -487,9 +548,9 @@ extern "C" int gpu_batch_records(const ts_program *p, unsigned item_words, unsig
                                  uint32_t *out, size_t cap, size_t *nwords, char *err,
                                  size_t errlen)
 {
-    if (!p || !nwords || (win != kWinSmall && win != kWinLarge) || !item_words ||
-        item_words > kLanes) {
-        say(err, errlen, "%s", "gpu_batch_records: a program, 8 or 16 slots and 1-64 item words");
+    if (!p || !nwords || (win != kWinSmall && win != kWinLarge) ||
+        (item_words != 16 && item_words != 32 && item_words != kLanes)) {
+        say(err, errlen, "%s", "gpu_batch_records: a program, 8 or 16 slots and 16, 32 or 64 item words");
         return 0;
     }
     if (!check_program(p, item_words * 4, err, errlen))
@@ -527,6 +588,10 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
             say(err, errlen, "%s", "a batch item without a program");
             return 0;
         }
+        if (items[i].residency > 5) {
+            say(err, errlen, "%s", "a batch item's residency is 0-5");
+            return 0;
+        }
         if (!check_program(p, kLanes * 4, err, errlen))
             return 0;
         if (items[i].lam_groups &&
@@ -546,14 +611,26 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     }
     BCHK(hipSetDevice(e->device));
     const uint32_t prog_words = tune.groups * 16;
-    // items as wide as the program needs, up to the wave: 16 words for one group, 32 for two
+    // items as wide as the program needs, up to the wave (16 words for one group, 32 for two), or
+    // as wide as the caller asks per program
     const uint32_t item_words = tune.groups == 1 ? 16 : tune.groups == 2 ? 32 : kLanes;
-    const uint32_t row = item_words * 4;  // bytes of one cell of an item
+    const uint32_t row = item_words * 4;  // bytes of one cell of a default-width item
+    std::vector<uint32_t> words(n), per_prog(n);  // per program: words per item, items
+    uint64_t items_total = 0;
+    for (size_t i = 0; i < n; i++) {
+        const unsigned w = items[i].words;
+        if (w && w != 16 && w != 32 && w != kLanes) {
+            say(err, errlen, "%s", "a batch item's words per work item must be 16, 32 or 64");
+            return 0;
+        }
+        words[i] = w ? w : item_words;
+        per_prog[i] = (prog_words + words[i] - 1) / words[i];
+        items_total += per_prog[i];
+    }
     const size_t lds_max = e->lds_per_block;
     if (!tune_in || !tune_in->lds_cells) {
         // as many bytes per item as 130 cells of a full wave; more when so few items run that
         // each CU holds only a few (a search of few large programs keeps its hare in LDS)
-        const uint64_t items_total = (uint64_t)n * ((prog_words + item_words - 1) / item_words);
         const uint64_t per_cu_needed = std::max<uint64_t>(1, (items_total + e->cus - 1) / e->cus);
         const uint64_t fit = lds_max / per_cu_needed / row;  // cells, the sink included
         tune.lds_cells = (unsigned)std::max<uint64_t>(130ull * kLanes / item_words,
@@ -568,32 +645,42 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
 
     // residency classes: 0 = LDS (<= 32 cells with the sink), 1 = LDS (<= lds_cells), 2 = LDS
     // (<= lds_large_cells, as few as two waves per CU), 3 = HBM; each class is one persistent
-    // launch sized by what a CU can hold
-    const uint32_t limit[kClasses] = {std::min<uint32_t>(32, tune.lds_cells), tune.lds_cells,
-                                      tune.lds_large_cells, 0xFFFFFFFFu};
+    // launch sized by what a CU can hold. The thresholds count cells of a default-width item; a
+    // program's need is the bytes of one of its items (cells and sink x 4 x its words)
+    const size_t limit[kClasses] = {std::min<uint32_t>(32, tune.lds_cells) * (size_t)row,
+                                    tune.lds_cells * (size_t)row, tune.lds_large_cells * (size_t)row,
+                                    ~(size_t)0};
+    auto need_of = [&](size_t i) { return (size_t)(items[i].prog->ncells + 1) * 4 * words[i]; };
     // a caller's residency: 1 = the smallest LDS class with room (up to what two items per CU
-    // allow), 2 = HBM
+    // allow), 2 = HBM, 3-5 = LDS class 0-2 whatever the thresholds say (the class's launch then
+    // sizes its LDS for its largest item)
     auto class_of_item = [&](size_t i) {
-        const uint32_t need = items[i].prog->ncells + 1;
-        if (items[i].residency == 2)
+        const size_t need = need_of(i);
+        const unsigned r = items[i].residency;
+        if (r == 2)
             return kHbm;
-        if (items[i].residency == 1)
-            return need <= limit[0] ? 0 : need <= limit[1] ? 1 : need <= two_fit ? 2 : kHbm;
+        if (r >= 3)
+            return (int)(r - 3);
+        if (r == 1)
+            return need <= limit[0] ? 0 : need <= limit[1] ? 1 : need <= lds_max / 2 ? 2 : kHbm;
         return need <= limit[0] ? 0 : need <= limit[1] ? 1 : need <= limit[2] ? 2 : kHbm;
     };
-    const uint32_t per_prog = (prog_words + item_words - 1) / item_words;  // items per program
     std::vector<BatchProg> progs(n);
     std::vector<uint32_t> recs, samples;
     std::vector<uint64_t> states;  // callers' start states
     std::vector<uint32_t> item_of[kClasses];
-    uint32_t maxcells[kClasses] = {0, 0, 0, 0};
+    size_t maxbytes[kClasses] = {0, 0, 0, 0};  // the largest item of each class, in bytes
     uint64_t class_items[kClasses] = {0, 0, 0, 0};
     uint64_t ops = 0, lam_total = 0;
     std::vector<int> cls_of(n);
     for (size_t i = 0; i < n; i++) {
         const int c = cls_of[i] = class_of_item(i);
-        class_items[c] += per_prog;
-        maxcells[c] = std::max(maxcells[c], items[i].prog->ncells + 1);
+        if (c != kHbm && need_of(i) > lds_max) {
+            say(err, errlen, "%s", "a batch item asked for LDS needs more than a CU has");
+            return 0;
+        }
+        class_items[c] += per_prog[i];
+        maxbytes[c] = std::max(maxbytes[c], need_of(i));
     }
     // per class: persistent waves per CU (LDS bounds the LDS classes; the HBM regions of all
     // waves stay within 16 GiB) and its HBM regions (hare and tortoise per wave)
@@ -603,19 +690,18 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     constexpr size_t kRegionBudget = (size_t)16 << 30;
     for (int c = 0; c < kClasses; c++) {
         per_cu[c] = tune.waves_per_cu;
-        if (c != kHbm && maxcells[c]) {
+        if (c != kHbm && maxbytes[c]) {
             per_cu[c] = (unsigned)std::max<size_t>(
-                1, std::min<size_t>(per_cu[c], lds_max / ((size_t)maxcells[c] * row)));
+                1, std::min<size_t>(per_cu[c], lds_max / maxbytes[c]));
             // the tortoise joins the hare in LDS when every item of the class still finds a
             // wave slot (Brent compares the whole state each pass once the sample matches)
-            const size_t both = std::min<size_t>(tune.waves_per_cu,
-                                                 lds_max / (2 * (size_t)maxcells[c] * row));
+            const size_t both = std::min<size_t>(tune.waves_per_cu, lds_max / (2 * maxbytes[c]));
             if (both && class_items[c] <= (size_t)e->cus * both) {
                 tort_lds[c] = true;
                 per_cu[c] = (unsigned)both;
             }
         }
-        const size_t region = (size_t)(c == kHbm ? 2 : tort_lds[c] ? 0 : 1) * maxcells[c] * row;
+        const size_t region = (size_t)(c == kHbm ? 2 : tort_lds[c] ? 0 : 1) * maxbytes[c];
         if (region)
             per_cu[c] = (unsigned)std::max<size_t>(
                 1, std::min<size_t>(per_cu[c], kRegionBudget / ((size_t)e->cus * region)));
@@ -629,8 +715,9 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     std::vector<Encoded> encs(n);  // the records of each program, encoded on host threads
     parallel_for(n, [&](size_t i) {
         const unsigned w = items[i].win;
-        encs[i] = w == kWinSmall || w == kWinLarge ? encode(items[i].prog, row, w)
-                                                   : pick_encoding(items[i].prog, row, wait[cls_of[i]]);
+        const uint32_t r = words[i] * 4;
+        encs[i] = w == kWinSmall || w == kWinLarge ? encode(items[i].prog, r, w)
+                                                   : pick_encoding(items[i].prog, r, wait[cls_of[i]]);
     });
     for (size_t i = 0; i < n; i++) {
         const ts_program *p = items[i].prog;
@@ -642,13 +729,23 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
         bp.win = enc.win;
         recs.insert(recs.end(), enc.words.begin(), enc.words.end());
         bp.ncells = p->ncells;
-        uint32_t writable = p->ncells - TS_CELL_FIRST_FREE;
-        bp.nsample = writable < kSample ? writable : kSample;
+        // Brent's sample: up to 64 of the stored cells, evenly spaced; the kernel compares 8 at a
+        // time, so both this list and the stored cells are padded with repeats of their last cell
+        const std::vector<uint32_t> &sc = enc.stored;
+        bp.nstored = (uint32_t)sc.size();
+        bp.nsample = std::min<uint32_t>(bp.nstored, kSample);
         bp.sample_off = (uint32_t)samples.size();
         for (uint32_t k = 0; k < bp.nsample; k++)
-            samples.push_back(TS_CELL_FIRST_FREE + (uint32_t)((uint64_t)k * writable / bp.nsample));
-        while ((samples.size() - bp.sample_off) % 8)  // the kernel compares 8 at a time
+            samples.push_back(sc[(uint64_t)k * bp.nstored / bp.nsample]);
+        while ((samples.size() - bp.sample_off) % 8)
             samples.push_back(samples.back());
+        bp.cells_off = (uint32_t)samples.size();
+        samples.insert(samples.end(), sc.begin(), sc.end());
+        while ((samples.size() - bp.cells_off) % 8)
+            samples.push_back(samples.back());
+        bp.nstored8 = (uint32_t)(samples.size() - bp.cells_off);
+        samples.insert(samples.end(), enc.read_only.begin(), enc.read_only.end());
+        bp.ncells_used = (uint32_t)(samples.size() - bp.cells_off);
         bp.seed = items[i].seed;
         bp.lam_off = kNoLam;
         bp.state_off = kNoLam;
@@ -670,7 +767,8 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
             lam_total += (uint64_t)(bp.lam_w1 - bp.lam_w0) * 32;
         }
         bp.first_item = (uint32_t)item_of[cls].size();
-        for (uint32_t k = 0; k < per_prog; k++)
+        bp.words = words[i];
+        for (uint32_t k = 0; k < per_prog[i]; k++)
             item_of[cls].push_back((uint32_t)i);
         ops += (uint64_t)p->len * tune.max_passes * tune.groups * 512ull;
     }
@@ -732,27 +830,29 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
     }
 
     uint64_t launches = 0;
+    auto sink_branch = [&](int cls) -> uint32_t { return (tune.sink_branch >> cls) & 1u; };
     auto launch_class = [&](int cls, hipStream_t st, uint32_t *queue, uint32_t first,
                             uint32_t count) -> int {
         const uint32_t *ip = d_items[cls].p;
         unsigned grid = std::min<unsigned>((unsigned)e->cus * per_cu[cls], count);
         uint32_t *sc = scratch + scratch_off[cls];
         if (cls != kHbm && tort_lds[cls]) {
-            const size_t lds_bytes = 2 * (size_t)maxcells[cls] * row;
+            const size_t lds_bytes = 2 * maxbytes[cls];
             hipLaunchKernelGGL((ts_batch<true, true>), dim3(grid), dim3(kLanes), lds_bytes, st,
                                d_progs.p, ip, first, count, d_recs.p, d_samples.p, queue, sc,
-                               maxcells[cls], d_stats.p, tune.max_passes, prog_words, item_words, d_lams.p, d_states.p,
+                               (uint32_t)maxbytes[cls], d_stats.p, tune.max_passes, prog_words, sink_branch(cls), d_lams.p, d_states.p,
                                stop_dev);
         } else if (cls != kHbm) {
-            const size_t lds_bytes = (size_t)maxcells[cls] * row;
+            const size_t lds_bytes = maxbytes[cls];
             hipLaunchKernelGGL((ts_batch<true, false>), dim3(grid), dim3(kLanes), lds_bytes, st,
                                d_progs.p, ip, first, count, d_recs.p, d_samples.p, queue, sc,
-                               maxcells[cls], d_stats.p, tune.max_passes, prog_words, item_words, d_lams.p, d_states.p,
+                               (uint32_t)maxbytes[cls], d_stats.p, tune.max_passes, prog_words, sink_branch(cls), d_lams.p, d_states.p,
                                stop_dev);
         } else {
             hipLaunchKernelGGL((ts_batch<false, false>), dim3(grid), dim3(kLanes), 0, st, d_progs.p,
-                               ip, first, count, d_recs.p, d_samples.p, queue, sc, maxcells[cls],
-                               d_stats.p, tune.max_passes, prog_words, item_words, d_lams.p, d_states.p,
+                               ip, first, count, d_recs.p, d_samples.p, queue, sc,
+                               (uint32_t)maxbytes[cls], d_stats.p, tune.max_passes, prog_words,
+                               sink_branch(cls), d_lams.p, d_states.p,
                                stop_dev);
         }
         launches++;
@@ -813,7 +913,7 @@ extern "C" int gpu_batch_run(gpu_engine *e, const gpu_batch_item *items, size_t 
         // one launch per program, in program order (the baseline the batch is compared with)
         for (size_t i = 0; i < n; i++) {
             int cls = cls_of[i];
-            if (!launch_class(cls, s, d_queue.p + kClasses + i, progs[i].first_item, per_prog)) {
+            if (!launch_class(cls, s, d_queue.p + kClasses + i, progs[i].first_item, per_prog[i])) {
                 say(err, errlen, "%s", "batch launch failed");
                 return 0;
             }

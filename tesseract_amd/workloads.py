@@ -66,61 +66,122 @@ def t1_start_group(prog, seed, g):
     return mem, z[-8:].copy()
 
 
-def autotune_batch(engine, progs, seeds, groups, sample=24, sample_passes=4):
-    """The batch autotuner: per machine size of the mix, the slots per window (8, 16, or the host
-    cost model's choice per program) and where its machines live (LDS when two work items of the
-    program fit a CU's LDS, or HBM), picked by timing on samples (the first `sample` programs of
-    each size, `sample_passes` passes each) at their 1-bit ops/s.
-    Windows are a local choice: each size's sample runs alone under every window for each
-    residency. Residency is not: the classes run side by side and share HBM, so every assignment
-    of the sizes that fit LDS to LDS or HBM is timed on the mixed sample of all sizes together,
-    each size with its best window for that residency, and the fastest whole run wins.
-    Returns per-program residency and window lists for gpu_batch_run and the measured tables."""
-    import itertools
-    item_words = 16 if groups == 1 else 32 if groups == 2 else 64
-    lds_most = 160 * 1024 // (2 * item_words * 4)  # cells (the sink included) two items hold
+LDS_BYTES = 160 * 1024  # a CU's LDS on gfx950
+WAVES_PER_CU = 16       # the batch's default persistent waves per CU per class
+
+
+def lds_classes(needs):
+    """Sizes placed in LDS -> one of the three LDS classes (0, 1, 2), given each size's item
+    bytes. A class's launch sizes every workgroup's LDS for its largest item, so a size in a class
+    runs as many waves per CU as that item allows: sizes are sorted by need and, while there are
+    more than three, the two neighbours whose wave counts differ least share a class."""
+    slots = lambda n: min(WAVES_PER_CU, LDS_BYTES // n)
+    groups = [[c] for c in sorted(needs, key=lambda c: needs[c])]
+    while len(groups) > 3:
+        ratio = [slots(needs[groups[k][-1]]) / slots(needs[groups[k + 1][-1]])
+                 for k in range(len(groups) - 1)]
+        k = min(range(len(ratio)), key=lambda k: ratio[k])
+        groups[k:k + 2] = [groups[k] + groups[k + 1]]
+    return {c: k for k, g in enumerate(groups) for c in g}
+
+
+def autotune_batch(engine, progs, seeds, groups, sample=24, sample_passes=4, sweeps=2):
+    """The batch autotuner: per machine size of the mix, where its machines live (LDS or HBM),
+    the words of its work items (lanes per wave: 16, 32 or 64; narrower items need less LDS, so
+    more waves share a CU) and the slots per window (8, 16, or the host cost model's choice per
+    program: the unroll depth of the interpreter's load rounds), picked by timing samples (the
+    first `sample` programs of each size, `sample_passes` passes each) at their 1-bit ops/s.
+    Windows are a local choice: each size's sample runs alone under each of them for every
+    placement (HBM, or LDS at each width two items of which fit a CU). Placements are not: the
+    classes run side by side, share HBM, and sizes in one LDS class share its LDS size
+    (lds_classes), so placements are chosen on the mixed sample of all sizes together, starting
+    from each size's best alone and trying every other placement of each size in turn (largest
+    total instruction count first) for `sweeps` rounds, keeping what runs the whole sample faster.
+    Returns per-program residency (2 = HBM, 3-5 = an LDS class), window and width lists for
+    gpu_batch_run, and the measured tables."""
+    default_words = 16 if groups == 1 else 32 if groups == 2 else 64
+    widths = [w for w in (16, 32, 64) if w <= default_words]
+    need = lambda c, words: (c + 1) * 4 * words
     by_cells = {}
     for i, p in enumerate(progs):
         by_cells.setdefault(p.ncells, []).append(i)
     sizes = sorted(by_cells)
     samples = {c: by_cells[c][:sample] for c in sizes}
-    name = lambda res, w: "%s/%s" % ("LDS" if res == 1 else "HBM", w or "model")
+    name = lambda place, w: "%s%s/%s" % ("HBM" if place is None else "LDS", "" if place is None
+                                         else "/%dw" % place, w or "model")
     rate = lambda info: info["ops"] / (info["ms"] * 1e-3) if info["ms"] else 0.0
-    best_win, table = {}, []
+    # placements: None = HBM (default width), or an LDS width
+    places = {c: [None] + [w for w in widths if need(c, w) <= LDS_BYTES // 2] for c in sizes}
+    best_win, alone, table = {}, {}, []
     for c in sizes:
         idx = samples[c]
         sp, ss = [progs[i] for i in idx], [seeds[i] for i in idx]
         rates = {}
-        for res in ((1, 2) if c + 1 <= lds_most else (2,)):
+        for place in places[c]:
             for w in (0, 8, 16):
                 _, info = engine.batch_run(sp, ss, groups=groups, max_passes=sample_passes,
-                                           residency=[res] * len(sp), win=[w] * len(sp))
-                rates[(res, w)] = rate(info)
-            best_win[(c, res)] = max((w for w in (0, 8, 16)), key=lambda w: rates[(res, w)])
+                                           residency=[2 if place is None else 1] * len(sp),
+                                           win=[w] * len(sp),
+                                           words=[place or default_words] * len(sp))
+                rates[(place, w)] = rate(info)
+            best_win[(c, place)] = max((0, 8, 16), key=lambda w: rates[(place, w)])
+            alone[(c, place)] = rates[(place, best_win[(c, place)])]
         table.append({"cells": c - 2, "sample_programs": len(idx),
                       "alone": {name(*k): v for k, v in rates.items()}})
-    lds_able = [c for c in sizes if c + 1 <= lds_most]
     mixed = [i for c in sizes for i in samples[c]]
-    joint = []
-    for combo in itertools.product((1, 2), repeat=len(lds_able)):
-        res_of = {c: 2 for c in sizes}
-        res_of.update(zip(lds_able, combo))
+
+    def assignment(place_of):
+        lds = {c: need(c, place_of[c]) for c in sizes if place_of[c] is not None}
+        cls = lds_classes(lds)
+        res = {c: 2 if place_of[c] is None else 3 + cls[c] for c in sizes}
+        return res, cls
+
+    def joint(place_of):
+        res, _ = assignment(place_of)
         _, info = engine.batch_run([progs[i] for i in mixed], [seeds[i] for i in mixed],
                                    groups=groups, max_passes=sample_passes,
-                                   residency=[res_of[progs[i].ncells] for i in mixed],
-                                   win=[best_win[(progs[i].ncells, res_of[progs[i].ncells])]
-                                        for i in mixed])
-        joint.append((info["ms"], res_of))
-    ms, res_of = min(joint, key=lambda x: x[0])
-    residency, win = [0] * len(progs), [0] * len(progs)
+                                   residency=[res[progs[i].ncells] for i in mixed],
+                                   win=[best_win[(progs[i].ncells, place_of[progs[i].ncells])]
+                                        for i in mixed],
+                                   words=[place_of[progs[i].ncells] or default_words
+                                          for i in mixed])
+        return info["ms"]
+
+    place_of = {c: max(places[c], key=lambda pl: alone[(c, pl)]) for c in sizes}
+    tried = []
+    best_ms = joint(place_of)
+    tried.append({"placement": {c - 2: name(place_of[c], best_win[(c, place_of[c])]) for c in sizes},
+                  "mixed_sample_ms": best_ms})
+    order = sorted(sizes, key=lambda c: -sum(progs[i].len for i in by_cells[c]))
+    for _ in range(sweeps):
+        changed = False
+        for c in order:
+            for pl in places[c]:
+                if pl == place_of[c]:
+                    continue
+                trial = dict(place_of)
+                trial[c] = pl
+                ms = joint(trial)
+                tried.append({"placement": {x - 2: name(trial[x], best_win[(x, trial[x])])
+                                            for x in sizes}, "mixed_sample_ms": ms})
+                if ms < best_ms:
+                    best_ms, place_of, changed = ms, trial, True
+        if not changed:
+            break
+    res, cls = assignment(place_of)
+    residency, win, words = [0] * len(progs), [0] * len(progs), [0] * len(progs)
     for c in sizes:
         for i in by_cells[c]:
-            residency[i], win[i] = res_of[c], best_win[(c, res_of[c])]
+            residency[i] = res[c]
+            win[i] = best_win[(c, place_of[c])]
+            words[i] = place_of[c] or default_words
     for row in table:
         c = row["cells"] + 2
-        row["choice"] = name(res_of[c], best_win[(c, res_of[c])])
-    tried = [{"LDS": [x - 2 for x in lds_able if r[x] == 1], "mixed_sample_ms": m} for m, r in joint]
-    return residency, win, {"sizes": table, "residency_assignments": tried}
+        row["choice"] = name(place_of[c], best_win[(c, place_of[c])])
+        if place_of[c] is not None:
+            row["lds_class"] = cls[c]
+    return residency, win, words, {"sizes": table, "placements_tried": tried,
+                                   "mixed_sample_ms": best_ms}
 
 
 def run_batch(engine, n, groups, passes, seed, plain_too=True, lds_cells=0, waves_per_cu=0,
@@ -134,17 +195,17 @@ def run_batch(engine, n, groups, passes, seed, plain_too=True, lds_cells=0, wave
         lam_groups = lam_groups(progs)
     out = {"workload": "batch", "programs": n, "machines_per_program": groups * 512,
            "passes": passes, "instructions": sum(p.len for p in progs)}
-    residency = win = None
+    residency = win = words = None
     if tuned:
         t0 = time.perf_counter()
-        residency, win, out["autotuner"] = autotune_batch(engine, progs, seeds, groups)
+        residency, win, words, out["autotuner"] = autotune_batch(engine, progs, seeds, groups)
         out["autotune_s"] = time.perf_counter() - t0
     t0 = time.perf_counter()
     stats, info = engine.batch_run(progs, seeds, groups=groups, max_passes=passes,
                                    lds_cells=lds_cells, waves_per_cu=waves_per_cu,
                                    lds_large_cells=lds_large_cells,
                                    lams=lam_groups is not None, lam_groups=lam_groups,
-                                   residency=residency, win=win)
+                                   residency=residency, win=win, words=words)
     if keep is not None:
         keep.update(progs=progs, seeds=seeds, stats=stats, lam_groups=lam_groups)
     out["batched_wall_s"] = time.perf_counter() - t0
@@ -158,7 +219,7 @@ def run_batch(engine, n, groups, passes, seed, plain_too=True, lds_cells=0, wave
         stats_p, info_p = engine.batch_run(progs, seeds, groups=groups, max_passes=passes, plain=True,
                                            lds_cells=lds_cells, waves_per_cu=waves_per_cu,
                                            lds_large_cells=lds_large_cells,
-                                           residency=residency, win=win)
+                                           residency=residency, win=win, words=words)
         out["plain_ms"] = info_p["ms"]
         out["plain_ops_per_s"] = info_p["ops"] / (info_p["ms"] * 1e-3)
         out["launches_plain"] = info_p["launches"]

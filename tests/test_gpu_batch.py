@@ -235,6 +235,71 @@ def test_batch_program_without_instructions(engine):
     assert got[0]["hist"][0] == 4 * 512 and got[0]["passes"] == 4
 
 
+def stores_program(kind):
+    b = t.Builder()
+    x, y, z = b.cell("x"), b.cell("y"), b.cell("z")
+    b.cell("never")  # neither read nor stored
+    if kind == "dead":  # the first STO of x is overwritten before any read
+        ops = [("n", x), ("n", y), ("s", x), ("n", z), ("s", x), ("s", y), ("n", x), ("s", z)]
+    elif kind == "no_sto":  # RR alone changes
+        ops = [("n", x), ("n", y), ("n", 1)]
+    else:  # y is stored and never read in the pass: its end-of-pass value is all that counts
+        ops = [("s", x), ("s", y), ("n", x), ("n", z), ("s", z), ("s", y), ("s", y), ("n", 0),
+               ("n", z)]
+    for op, c in ops:
+        (b.nand if op == "n" else b.sto)(c)
+    return b.finish()
+
+
+@pytest.mark.parametrize("kind", ["dead", "no_sto", "unread"])
+def test_batch_store_rules(engine, kind):
+    # dead STOs are dropped and Brent compares, copies and starts only the cells the program
+    # stores or reads: the cycle lengths stay the oracle's, machine by machine
+    prog = stores_program(kind)
+    print("batch store rules: %s" % kind, flush=True)
+    for plain in (False, True):
+        got, _ = engine.batch_run([prog], [11], groups=4, max_passes=64, plain=plain, lams=True)
+        mem, rr = t.t1_start(prog, 11, 4)
+        want = np.concatenate([O.brent_lines(prog, mem[k], rr[k], 64) for k in range(4)])
+        assert np.array_equal(got[0].pop("lam"), want), plain
+        assert got[0] == expected(prog, 11, 4, 64), plain
+
+
+@pytest.mark.parametrize("residency", [1, 2])
+def test_batch_sink_branch(engine, residency):
+    # slots without a NAND or a STO branch around the sink's load and store in every class: the
+    # same cycle lengths as the oracle's
+    progs = programs()
+    print("batch sink branch, residency %d" % residency, flush=True)
+    for plain in (False, True):
+        got, _ = engine.batch_run([p for p, _ in progs], [s for _, s in progs], groups=4,
+                                  max_passes=64, plain=plain, lams=True,
+                                  residency=[residency] * len(progs), sink_branch=15)
+        for (p, s), g in zip(progs, got):
+            mem, rr = t.t1_start(p, s, 4)
+            want = np.concatenate([O.brent_lines(p, mem[k], rr[k], 64) for k in range(4)])
+            assert np.array_equal(g.pop("lam"), want), (p.ncells, plain)
+            assert g == expected(p, s, 4, 64), (p.ncells, plain)
+
+
+@pytest.mark.parametrize("residency", [0, 1, 2])
+def test_batch_item_widths(engine, residency):
+    # a caller's words per work item (16, 32, 64 for programs of 4 groups), mixed in one batch
+    # and in every residency: per-machine cycle lengths and statistics equal the oracle's
+    progs = programs()
+    words = [(16, 32, 64)[k % 3] for k in range(len(progs))]
+    print("batch item widths, residency %d" % residency, flush=True)
+    for plain in (False, True):
+        got, _ = engine.batch_run([p for p, _ in progs], [s for _, s in progs], groups=4,
+                                  max_passes=64, plain=plain, lams=True,
+                                  residency=[residency] * len(progs), words=words)
+        for (p, s), g, w in zip(progs, got, words):
+            mem, rr = t.t1_start(p, s, 4)
+            want = np.concatenate([O.brent_lines(p, mem[k], rr[k], 64) for k in range(4)])
+            assert np.array_equal(g.pop("lam"), want), (p.ncells, w, plain)
+            assert g == expected(p, s, 4, 64), (p.ncells, w, plain)
+
+
 def test_batch_matches_reference_t1_cycles(engine, ref):
     # group 0 of each program is exactly the reference's t1_cycles run (its own jit-compiled
     # code): same cycle lengths, same passes
@@ -282,3 +347,20 @@ def test_batch_mix_10000_matches_reference(engine, ref):
                 print("%d programs compared" % (k + 1), flush=True)
     assert not bad, "programs differing: %s" % bad[:10]
     assert info["ops"] == sum(p.len * g["passes"] * 512 for p, g in zip(progs, got))
+
+
+def test_batch_autotuned_mix_same_statistics(engine):
+    # the autotuner's placements (HBM or an LDS class, item widths, windows) change where and how
+    # programs run, never what they compute: the tuned batch, the tuned plain run and the untuned
+    # batch give the same statistics per program
+    from tesseract_amd.workloads import autotune_batch, batch_mix
+    progs, seeds = batch_mix(60, 3)
+    print("autotuned mix of 60 programs, 8 groups", flush=True)
+    residency, win, words, report = autotune_batch(engine, progs, seeds, 8, sample=4,
+                                                   sample_passes=2, sweeps=1)
+    assert all(r in (2, 3, 4, 5) for r in residency) and set(words) <= {16, 32, 64}
+    base, _ = engine.batch_run(progs, seeds, groups=8, max_passes=32)
+    for plain in (False, True):
+        got, _ = engine.batch_run(progs, seeds, groups=8, max_passes=32, plain=plain,
+                                  residency=residency, win=win, words=words)
+        assert got == base, plain

@@ -630,3 +630,18 @@ def test_t1_start_group_continues_the_stream():
     for g in range(5):
         m, r = t1_start_group(p, 1234, g)
         assert np.array_equal(m, mem[g]) and np.array_equal(r, rr[g]), g
+
+
+def test_batch_lds_classes():
+    # sizes in LDS share at most three classes; neighbours whose wave counts per CU differ least
+    # share one (a class sizes its LDS for its largest item)
+    from tesseract_amd.workloads import lds_classes
+    assert lds_classes({18: 19 * 256}) == {18: 0}
+    assert lds_classes({18: 4864, 66: 8576, 258: 16576, 1026: 65728}) == {18: 0, 66: 0, 258: 1,
+                                                                         1026: 2}
+    # 16, 9, 4 and 2 waves per CU: the first two are closest
+    assert lds_classes({18: 4864, 66: 17152, 258: 33152, 1026: 65728}) == {18: 0, 66: 0, 258: 1,
+                                                                          1026: 2}
+    got = lds_classes({c: 1000 * c for c in range(1, 9)})
+    assert sorted(set(got.values())) == [0, 1, 2]
+    assert all(got[a] <= got[b] for a in got for b in got if a < b)

@@ -54,6 +54,14 @@ typedef struct {
                               (x bits 0..4) reaches evaluated once per group, bit-sliced over its
                               words; 0 = auto (groups for one tick, at least 24 inputs and at
                               least 32 such LUTs) */
+    unsigned lds_links;    /* cubes of 128 to 512 words (2^12 to 2^14 machines): 1 = each cube
+                              runs in one workgroup of that many threads, reads across the waves
+                              of its cube through LDS (a barrier pair per level of such reads),
+                              the pass loop in-kernel with the state in registers; 2 = as larger
+                              cubes: segments at reads of values computed across waves (or the
+                              value recomputed when its cone is small), slices of threads per
+                              segment; 0 = the form a cost model of one cube's pass prefers (the
+                              LUTs one thread evaluates, and the launches) */
 } gpu_tune;
 
 typedef struct {
@@ -64,6 +72,8 @@ typedef struct {
     uint64_t values;       /* values the generated code materialises */
     double   compile_seconds;
     uint32_t dims;         /* cube dimensions */
+    uint64_t chain;        /* LUTs one thread evaluates in a pass: the resident kernel's pass, or
+                              the largest slice of each segment, summed */
 } gpu_code_info;
 
 /* Compiles p to gfx950 code through hipRTC: the counterpart of

@@ -49,7 +49,8 @@ class EnumResultStruct(C.Structure):
 class GpuTune(C.Structure):
     _fields_ = [("block", C.c_uint), ("slice_nodes", C.c_uint), ("lut3", C.c_uint), ("grid", C.c_uint),
                 ("graph_passes", C.c_uint), ("tape", C.c_uint), ("waves", C.c_uint),
-                ("max_waves", C.c_uint), ("sched", C.c_uint), ("group", C.c_uint)]
+                ("max_waves", C.c_uint), ("sched", C.c_uint), ("group", C.c_uint),
+                ("lds_links", C.c_uint)]
 
 
 class GpuT1Choice(C.Structure):
@@ -65,7 +66,7 @@ class GpuEnumInfo(C.Structure):
 class GpuCodeInfo(C.Structure):
     _fields_ = [("segments", C.c_uint32), ("resident", C.c_uint32), ("slices", C.c_uint32),
                 ("bitop3", C.c_uint64), ("values", C.c_uint64), ("compile_seconds", C.c_double),
-                ("dims", C.c_uint32)]
+                ("dims", C.c_uint32), ("chain", C.c_uint64)]
 
 
 class GpuBatchItem(C.Structure):

@@ -488,6 +488,106 @@ struct Emitter {
     }
 };
 
+// The resident pass of a cube that runs in one workgroup of B threads (thread = word of the cube):
+// a value read across the cube's waves goes through LDS. A node's level counts the reads across
+// waves on its longest path from the start of the pass; level 0 is emitted first, then for each
+// level k the values its cross-wave reads take are written to LDS (rows of B words, at most
+// `rows` at a time), a barrier, each read of the partner word's row (thread ^ the read's word
+// mask), a barrier before the rows are written again, and then the level's other nodes. Every
+// thread of the workgroup runs the same straight-line code, so every barrier is met by all.
+// `state` names the register holding a cell's value at the start of the pass (cross-wave reads of
+// those are level 1). Returns the LDS rows the code uses.
+template <class LeafFn>
+unsigned emit_levels(const Aig &g, const Mapping &m, Emitter &e, std::ostringstream &o, unsigned B,
+                     unsigned rows_max, const std::vector<Version> &versions, LeafFn leaf)
+{
+    const size_t N = g.nodes.size();
+    std::vector<int> lev(N, 0);
+    std::vector<uint8_t> cross(N, 0);
+    int top = 0;
+    for (size_t n = 1; n < N; n++) {
+        if (!m.in_cone[n])
+            continue;
+        const Node &nd = g.nodes[n];
+        if (nd.kind == N_AND) {
+            if (!m.is_lut[n])
+                continue;
+            const Cut &c = m.best[n];
+            int l = 0;
+            for (int i = 0; i < c.n; i++)
+                l = std::max(l, lev[c.leaf[i]]);
+            lev[n] = l;
+        } else if (nd.kind == N_PERM) {
+            cross[n] = (nd.b >> 5) >= 64u;
+            lev[n] = lev[nd.a >> 1] + cross[n];
+        } else if (nd.kind == N_LOAD) {
+            cross[n] = (nd.b >> 5) >= 64u;
+            lev[n] = cross[n];
+        }
+        top = std::max(top, lev[n]);
+    }
+    unsigned rows_used = 0;
+    std::vector<uint8_t> only(N, 0);
+    const std::vector<uint8_t> *saved = e.only;
+    for (int k = 0; k <= top; k++) {
+        if (k > 0) {
+            // the values this level's cross-wave reads take, each written once
+            std::vector<std::string> src;
+            std::unordered_map<std::string, unsigned> row_of;
+            std::vector<std::pair<uint32_t, unsigned>> reads;  // node, source index
+            for (size_t n = 1; n < N; n++) {
+                if (!m.in_cone[n] || !cross[n] || lev[n] != k)
+                    continue;
+                const Node &nd = g.nodes[n];
+                const std::string x = nd.kind == N_PERM ? e.var(nd.a >> 1)
+                                                        : "s" + std::to_string(versions[nd.a].cell);
+                auto it = row_of.find(x);
+                if (it == row_of.end()) {
+                    it = row_of.emplace(x, (unsigned)src.size()).first;
+                    src.push_back(x);
+                }
+                reads.push_back({(uint32_t)n, it->second});
+            }
+            for (size_t c0 = 0; c0 < src.size(); c0 += rows_max) {
+                const size_t c1 = std::min(src.size(), c0 + rows_max);
+                rows_used = std::max<unsigned>(rows_used, (unsigned)(c1 - c0));
+                for (size_t j = c0; j < c1; j++)
+                    o << "  ts_xs[" << (j - c0) * B << "u + tid] = " << src[j] << ";\n";
+                o << "  __syncthreads();\n";
+                for (const auto &r : reads) {
+                    if (r.second < c0 || r.second >= c1)
+                        continue;
+                    const Node &nd = g.nodes[r.first];
+                    const uint32_t inword = nd.b & 31u, words = nd.b >> 5;
+                    std::string x = "ts_xs[" + std::to_string((r.second - c0) * B) + "u + (tid ^ " +
+                                    std::to_string(words) + "u)]";
+                    if (inword)
+                        x = "ts_pw(" + x + ", " + std::to_string(inword) + "u)";
+                    // a PERM's operand row holds the operand's variable, which may hold its
+                    // complement (and the PERM may read the complemented literal)
+                    if (nd.kind == N_PERM && (((nd.a & 1u) != 0) != (m.inv[nd.a >> 1] != 0)))
+                        x = "(~" + x + ")";
+                    o << "  u32 " << e.var(r.first) << " = " << x << ";\n";
+                    e.values++;
+                    if (e.after) {
+                        auto it = e.after->find(r.first);
+                        if (it != e.after->end())
+                            for (const std::string &line : it->second)
+                                o << line;
+                    }
+                }
+                o << "  __syncthreads();\n";
+            }
+        }
+        for (size_t n = 0; n < N; n++)
+            only[n] = m.in_cone[n] && lev[n] == k && !cross[n];
+        e.only = &only;
+        e.emit(leaf);
+    }
+    e.only = saved;
+    return rows_used;
+}
+
 }  // namespace
 
 // ---- run kernels (streamed and resident) ------------------------------------------------------
@@ -523,6 +623,15 @@ void codegen_run(const Lowered &lw, const CodegenOptions &opt, KernelSource &ks)
             roots.push_back(out.v);
         Mapping m;
         map_luts(s.g, roots, opt.lut3, m);
+        // a whole cube in one workgroup (thread = word of the cube): reads across its waves go
+        // through LDS, level by level (emit_levels)
+        const bool in_wg = lw.wg_dims > lw.wave_dims;
+        const unsigned B = in_wg ? 1u << (lw.wg_dims - 5) : opt.block;
+        const unsigned rows_max = in_wg ? (64u * 1024u) / (4u * B) : 0u;
+        const std::string wg_bounds =
+            "__launch_bounds__(" + std::to_string(B) + ")" +
+            (waves > 1 ? " __attribute__((amdgpu_waves_per_eu(" + std::to_string(waves) + ")))"
+                       : std::string());
         // state variables carried from pass to pass: the cells whose value at the start of a pass
         // is read (version -1). A cell the pass writes before any read of it is no loop-carried
         // variable: its value of the last pass is stored as the last pass makes it, so no register
@@ -530,12 +639,19 @@ void codegen_run(const Lowered &lw, const CodegenOptions &opt, KernelSource &ks)
         std::vector<uint8_t> is_state(lw.ncells + 1, 0);
         for (const Version &v : lw.versions)
             is_state[v.cell] = 1;
-        o << "extern \"C\" __global__ void " << bounds
+        o << "extern \"C\" __global__ void " << (in_wg ? wg_bounds : bounds)
           << " ts_pass(const u32* __restrict__ prev, u32* __restrict__ cur, const u32* "
              "__restrict__ init, u32 nw, u32 passes) {\n"
              "  const u32 w = blockIdx.x * "
-          << opt.block << "u + threadIdx.x;\n  if (w >= nw) return;\n"
-          << "  const u32 cw = w & " << ((1u << cube_wl) - 1) << "u;\n  (void)cw;\n";
+          << B << "u + threadIdx.x;\n";
+        // (a workgroup is a whole cube and the state whole cubes: no thread is past nw, and all
+        // of them reach every barrier)
+        if (!in_wg)
+            o << "  if (w >= nw) return;\n";
+        o << "  const u32 cw = w & " << ((1u << cube_wl) - 1) << "u;\n  (void)cw;\n";
+        const size_t xs_at = o.tellp();  // the LDS rows are declared here once their count is known
+        if (in_wg)
+            o << "  const u32 tid = threadIdx.x;\n";
         o << "  if (passes == 0u) {  // the state as it was\n";
         for (const SegOut &out : s.outs)
             o << "    cur[(size_t)" << lw.final_slot[out.cell] << "u * nw + w] = prev[(size_t)"
@@ -572,10 +688,14 @@ void codegen_run(const Lowered &lw, const CodegenOptions &opt, KernelSource &ks)
         // overwritten by the next pass before anything reads it, and the compiler drops it. The
         // last pass runs apart, storing each cell's value the moment it exists, so that no value
         // waits in a register for the end of the pass.
+        unsigned rows = 0;
         o << "  for (u32 pass = 1; pass < passes; pass++) {\n";
         {
             Emitter e(s.g, m, o, "v");
-            e.emit(leaf);
+            if (in_wg)
+                rows = std::max(rows, emit_levels(s.g, m, e, o, B, rows_max, lw.versions, leaf));
+            else
+                e.emit(leaf);
             // all new values first, then the state update (every read above saw the old state)
             for (size_t i = 0; i < s.outs.size(); i++)
                 if (is_state[s.outs[i].cell])
@@ -597,13 +717,26 @@ void codegen_run(const Lowered &lw, const CodegenOptions &opt, KernelSource &ks)
                 after[out.v >> 1].push_back(line);
         }
         e.after = &after;
-        e.emit(leaf);
+        if (in_wg)
+            rows = std::max(rows, emit_levels(s.g, m, e, o, B, rows_max, lw.versions, leaf));
+        else
+            e.emit(leaf);
         for (const std::string &line : constant_outs)
             o << line;
         o << "  }\n}\n";
         ks.names.push_back("ts_pass");
+        if (in_wg) {
+            ks.block = B;
+            if (rows) {
+                std::string text = o.str();
+                text.insert(xs_at, "  __shared__ u32 ts_xs[" + std::to_string(rows * B) + "];\n");
+                o.str(text);
+                o.seekp(0, std::ios_base::end);
+            }
+        }
         ks.bitop3 += e.luts;
         ks.values += e.values;
+        ks.chain_luts = e.luts;
         ks.text = o.str();
         return;
     }
@@ -622,6 +755,7 @@ void codegen_run(const Lowered &lw, const CodegenOptions &opt, KernelSource &ks)
           << "  if (slice >= " << s.slices.size() << "u) return;\n"
           << "  const u32 cw = w & " << ((1u << cube_wl) - 1) << "u;\n  (void)cw;\n"
           << "  switch (slice) {\n";
+        size_t widest = 0;
         for (size_t k = 0; k < s.slices.size(); k++) {
             std::vector<Lit> roots;
             for (uint32_t oi : s.slices[k])
@@ -652,7 +786,9 @@ void codegen_run(const Lowered &lw, const CodegenOptions &opt, KernelSource &ks)
             o << "  } break;\n";
             ks.bitop3 += e.luts;
             ks.values += e.values;
+            widest = std::max(widest, e.luts);
         }
+        ks.chain_luts += widest;
         o << "  default: break;\n  }\n}\n";
     }
     ks.text = o.str();

@@ -265,23 +265,54 @@ extern "C" double gpu_last_ms(const gpu_engine *e) { return e->last_ms; }
 namespace {
 
 // program -> lowered passes -> HIP source (host only)
-int generate(const ts_program *p, const gpu_tune &tune, Lowered &lw, KernelSource &ks, char *err,
+int generate(const ts_program *p, gpu_tune &tune, Lowered &lw, KernelSource &ks, char *err,
              size_t errlen)
 {
     LowerOptions lo;
     unsigned dims = ts_program_dims(p);
     lo.cube_words_log2 = dims - 5;
     lo.slice_nodes = tune.slice_nodes;
-    std::string msg;
-    if (!lower_program(p, lo, lw, msg)) {
-        say(err, errlen, "%s", msg.c_str());
-        return 0;
-    }
     CodegenOptions co;
     co.block = tune.block;
     co.lut3 = tune.lut3 == 1;
     co.waves = std::min(tune.waves, 8u);
-    codegen_run(lw, co, ks);
+    auto make = [&](unsigned wg_log2, Lowered &l, KernelSource &k) {
+        lo.wg_words_log2 = wg_log2;
+        std::string msg;
+        if (!lower_program(p, lo, l, msg)) {
+            say(err, errlen, "%s", msg.c_str());
+            return false;
+        }
+        codegen_run(l, co, k);
+        return true;
+    };
+    // A cube of 128-512 words may run in one workgroup (links across its waves through LDS, the
+    // pass loop in-kernel) or segmented (slices of threads per segment, a launch per segment).
+    // By default the form with the shorter pass for one cube is taken, from the LUTs one thread
+    // evaluates in a pass (the workgroup form's whole pass; the segments' largest slices) and the
+    // launches: measured on one cube, rank on 12 dims (376 LUTs in one workgroup, 0.6 us per pass,
+    // against 2 segments of 1,744, 18 us) and gamma on 12 dims (6,311 LUTs in one workgroup,
+    // 94 us, against 2 segments of 5,945, 76 us: the workgroup form carries ~1,850 cells in
+    // registers and barriers between its levels), about 44 cycles per LUT in one workgroup, 35
+    // in a segment and 12,000 per launch.
+    const bool may_wg = dims >= 12 && dims <= 14 && tune.lds_links != 2;
+    if (!make(may_wg ? 9 : 0, lw, ks))
+        return 0;
+    if (may_wg && tune.lds_links != 1 && lw.resident && lw.wg_dims > lw.wave_dims) {
+        Lowered lw2;
+        KernelSource ks2;
+        if (!make(0, lw2, ks2))
+            return 0;
+        const double cyc_wg = 44.0 * (double)ks.chain_luts;
+        const double cyc_seg = 12000.0 * (lw2.resident ? 0 : lw2.segs.size()) +
+                               35.0 * (double)ks2.chain_luts;
+        if (cyc_seg < cyc_wg) {
+            lw = std::move(lw2);
+            ks = std::move(ks2);
+        }
+    }
+    if (ks.block)  // the kernels need this workgroup size (a whole cube per workgroup)
+        tune.block = ks.block;
     return 1;
 }
 
@@ -296,6 +327,7 @@ void fill_info(const Lowered &lw, const KernelSource &ks, gpu_code_info &info)
     info.bitop3 = ks.bitop3;
     info.values = ks.values;
     info.dims = lw.dims;
+    info.chain = ks.chain_luts;
 }
 
 }  // namespace

@@ -175,8 +175,8 @@ struct PassBuilder {
                 v = value(c);
             else {
                 Lit base = value(c);
-                if ((mask >> lw.wave_dims) && g().derived(base)) {
-                    // a value computed in this segment, read from a machine in another wave:
+                if ((mask >> lw.wg_dims) && g().derived(base)) {
+                    // a value computed in this segment, read from a machine in another workgroup:
                     // computed again here from that machine's start-of-segment values if its cone
                     // is small, else the segment ends (the value crosses as a slot)
                     if (!remat(base, mask, v)) {
@@ -234,8 +234,8 @@ struct PassBuilder {
 
     Lit moved(Lit x, uint32_t mask, std::unordered_map<uint64_t, Lit> &memo)
     {
-        if (!(mask >> lw.wave_dims))
-            return g().Perm(x, mask);  // inside the wave: a cross-lane move of the value
+        if (!(mask >> lw.wg_dims))
+            return g().Perm(x, mask);  // inside the workgroup: a move of the value
         const uint32_t n = x >> 1;
         const Lit c = x & 1u;
         const uint64_t key = (uint64_t)n << 32 | mask;
@@ -436,6 +436,9 @@ bool lower_program(const ts_program *p, const LowerOptions &opt, Lowered &lw, st
     lw.dims = dims;
     unsigned wl = opt.cube_words_log2 < 6 ? opt.cube_words_log2 : 6;
     lw.wave_dims = 5 + wl;
+    lw.wg_dims = opt.cube_words_log2 > 6 && opt.cube_words_log2 <= opt.wg_words_log2
+                     ? 5 + opt.cube_words_log2
+                     : lw.wave_dims;
 
     PassBuilder pb(p, opt, lw);
     if (!pb.run(err))
@@ -508,10 +511,10 @@ bool lower_program(const ts_program *p, const LowerOptions &opt, Lowered &lw, st
             o.final_only = o.slot >= 0 && (int)s == lw.last_seg[o.cell] && !upward[o.cell] && !later;
         }
 
-    // resident: one segment and every read of a start-of-pass value inside the wave
+    // resident: one segment and every read of a start-of-pass value inside the workgroup
     lw.resident = lw.segs.size() == 1;
     for (const Node &n : lw.segs[0].g.nodes)
-        if (n.kind == N_LOAD && (n.b >> lw.wave_dims))
+        if (n.kind == N_LOAD && (n.b >> lw.wg_dims))
             lw.resident = false;
     for (Segment &s : lw.segs) {
         if (lw.resident || opt.slice_nodes)

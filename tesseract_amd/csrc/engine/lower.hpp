@@ -102,12 +102,18 @@ struct Lowered {
     std::vector<int32_t> final_slot; // per cell incl. RR: slot of the value after a pass, -1 = init
     std::vector<int32_t> init_slot;  // per cell incl. RR: slot in the init area, -1 = none
     uint32_t nslots = 0, ninit = 0;
-    bool     resident = false;       // one segment, every read in-wave: the pass loop runs in-kernel
+    bool     resident = false;       // one segment, every read in the workgroup: the pass loop
+                                     // runs in-kernel
+    // dims < wg_dims are inside one workgroup: wave_dims, or all the cube's dims when a whole cube
+    // runs in one workgroup (its reads across waves then go through LDS)
+    unsigned wg_dims = 0;
 };
 
 struct LowerOptions {
     unsigned cube_words_log2;   // log2(words per cube) = dims - 5
     unsigned slice_nodes;       // target AIG nodes per slice (0 = chosen per segment)
+    unsigned wg_words_log2 = 0; // a cube of more than 64 and at most 2^this words runs in one
+                                // workgroup (0 = never)
 };
 
 // Lowers one pass of p into segments. Returns false with a message for programs it cannot take.
@@ -117,6 +123,10 @@ bool lower_program(const ts_program *p, const LowerOptions &opt, Lowered &out, s
 
 struct KernelSource {
     std::string text;
+    unsigned block = 0;              // workgroup size the kernels need (0 = any)
+    // LUTs one thread evaluates in a pass: the resident kernel's pass, or over the segments the
+    // largest slice of each (what bounds a pass when few cubes run)
+    size_t chain_luts = 0;
     std::vector<std::string> names;  // kernel names in launch order (one per segment)
     size_t and_nodes = 0, bitop3 = 0, values = 0;
     unsigned group = 1;  // enumerations: words a thread takes per step of its loop (1 or 32)

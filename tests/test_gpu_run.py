@@ -73,6 +73,22 @@ def test_random_cube(engine, dims):
     check(engine, prog, runs=(2, 3))
 
 
+@pytest.mark.parametrize("lds_links", [1, 2])
+@pytest.mark.parametrize("dims", [12, 13, 14])
+def test_cube_in_one_workgroup(engine, dims, lds_links):
+    # cubes of 128-512 words: each cube in one workgroup, reads across its waves through LDS and
+    # the pass loop in-kernel (lds_links 1), or segments through HBM (2); three cubes per state,
+    # several calls, every machine against the oracle
+    prog = random_cube_program(24, 600, dims, seed=300 + dims, links=40)
+    print("check cube dims %d lds_links %d" % (dims, lds_links), flush=True)
+    code = check(engine, prog, groups=3 * prog.groups, tune={"lds_links": lds_links},
+                 runs=(1, 4, 2))
+    info = code.stats()
+    assert info["resident"] == (lds_links == 1)
+    if lds_links == 1:
+        assert info["segments"] == 1
+
+
 @pytest.mark.parametrize("dims", [9, 12])
 def test_rank(engine, dims):
     print("check rank", dims, flush=True)
@@ -100,7 +116,7 @@ def test_graph_replays_match_eager(engine, gp):
     # eager launches in between, each checked against the oracle. An odd gp flips the parity at
     # every replay, so both parities' graphs are captured and replayed within one call.
     prog = random_cube_program(20, 300, 12, seed=77)
-    code = engine.compile(prog, {"graph_passes": gp})
+    code = engine.compile(prog, {"graph_passes": gp, "lds_links": 2})
     assert code.stats()["segments"] > 1
     mem, rr = O.random_lines(prog, prog.groups, 5)
     st = code.state()
@@ -112,7 +128,7 @@ def test_graph_replays_match_eager(engine, gp):
         gm, gr = st.get()
         wm, wr = O.run_lines(prog, mem, rr, done)
         assert np.array_equal(gm, wm) and np.array_equal(gr, wr), "after %d passes" % done
-    code1 = engine.compile(prog, {"graph_passes": 1})
+    code1 = engine.compile(prog, {"graph_passes": 1, "lds_links": 2})
     st1 = code1.state()
     st1.put(mem, rr)
     st1.run(done)

@@ -611,6 +611,31 @@ def test_generated_kernels_compile_for_gfx950():
     t.rtc_check(t.enum_source(t.EN_EQUIV, x, x))
 
 
+def test_cube_in_one_workgroup_source():
+    # a cube of 128-512 words compiles to one resident kernel whose workgroup is the cube (its
+    # reads across waves through LDS, a barrier pair per chunk of each level); larger cubes and
+    # lds_links = 2 keep the segmented form; every form compiles for gfx950
+    from tests.test_gpu_run import random_cube_program
+    for dims in (12, 14):
+        p = random_cube_program(16, 300, dims, seed=dims, links=30)
+        src, info = t.source(p)
+        assert info["resident"] == 1 and info["segments"] == 1
+        assert "__launch_bounds__(%d)" % (1 << (dims - 5)) in src
+        assert "ts_xs[" in src and src.count("__syncthreads();") % 2 == 0
+        t.rtc_check(src)
+        src2, info2 = t.source(p, {"lds_links": 2})
+        assert info2["resident"] == 0 and "ts_xs[" not in src2
+    p = random_cube_program(16, 300, 15, seed=15, links=30)
+    src, info = t.source(p)
+    assert info["resident"] == 0 and "ts_xs[" not in src
+    # the default weighs one cube's pass in both forms: gamma on 12 dims (6,311 LUTs one thread
+    # evaluates in one workgroup, 5,945 in the segments' largest slices) stays segmented
+    gamma = golden("gamma_4_4_4")
+    assert t.source(gamma)[1]["segments"] == 2
+    one = t.source(gamma, {"lds_links": 1})[1]
+    assert one["resident"] == 1 and one["chain"] == one["bitop3"]
+
+
 def test_enum_validation_messages():
     big = t.net_parse("\n".join("INPUT(i%d)" % k for k in range(47)) + "\nOUTPUT(o)\no = AND(i0, i1)\n")
     with pytest.raises(ValueError, match="47 inputs is too many"):

@@ -436,10 +436,48 @@ def run_serve(engine, n, rate, seed, streams=4, chunk_words=1 << 24, check=True,
     return out
 
 
+def run_cube_forms(engine, names=("rank12", "gamma_4_4_4"), counts=(1, 16, 256, 1024, 4096),
+                   seed=5):
+    """Cubes of 2^12 machines in both forms: one workgroup per cube (links across its waves
+    through LDS, the pass loop in-kernel; gpu_tune.lds_links 1) and segmented (2), and what the
+    cost model picks (0): microseconds per pass for each number of cubes, from the same random
+    states (put ignores the read-only cells), and whether the two forms' states agree."""
+    from .programs import golden
+    rng = np.random.default_rng(seed)
+    out = {"workload": "cubeforms", "programs": []}
+    for name in names:
+        p = golden(name)
+        codes = {ll: engine.compile(p, {"lds_links": ll}) for ll in (1, 2)}
+        row = {"program": name, "instructions": p.len, "dims": p.dims,
+               "default": "workgroup" if engine.compile(p).stats()["resident"] else "segmented",
+               "chain_luts": {k: codes[k].stats()["chain"] for k in (1, 2)},
+               "segments": codes[2].stats()["segments"], "cubes": []}
+        for cubes in counts:
+            groups = cubes * p.groups
+            mem = rng.integers(0, 1 << 64, size=(groups, p.ncells, 8), dtype=np.uint64)
+            mem[:, 0, :] = 0
+            mem[:, 1, :] = np.uint64(0xFFFFFFFFFFFFFFFF)
+            rr = rng.integers(0, 1 << 64, size=(groups, 8), dtype=np.uint64)
+            us, states = {}, {}
+            for ll in (1, 2):
+                st = codes[ll].state(groups)
+                st.put(mem, rr)
+                st.run(4)
+                n = 256 if cubes <= 16 else 64
+                us[ll] = st.run(n) * 1e3 / n
+                states[ll] = st.get()
+            row["cubes"].append({"cubes": cubes, "workgroup_us_per_pass": us[1],
+                                 "segmented_us_per_pass": us[2],
+                                 "same_states": all(np.array_equal(a, b) for a, b in
+                                                    zip(states[1], states[2]))})
+        out["programs"].append(row)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("workload", choices=["batch", "life", "hash", "benes", "serve", "polymul",
-                                         "tape"])
+                                         "tape", "cubeforms"])
     ap.add_argument("--requests", type=int, default=200)
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--streams", type=int, default=4)
@@ -476,6 +514,8 @@ def main():
         out = run_polymul(eng, a.cubes, a.seed)
     elif a.workload == "tape":
         out = run_tape(eng)
+    elif a.workload == "cubeforms":
+        out = run_cube_forms(eng)
     else:
         out = run_hash(eng)
     out["device"] = eng.describe()

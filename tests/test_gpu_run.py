@@ -89,6 +89,33 @@ def test_cube_in_one_workgroup(engine, dims, lds_links):
         assert info["segments"] == 1
 
 
+def wide_cross_program(dims=14, cells=48):
+    """Every cell read across the top dim at the start of the pass and again after it was
+    written: more values cross the waves at one level than one chunk of LDS rows holds."""
+    b = t.Builder()
+    plain = [b.cell("c%d" % i) for i in range(cells)]
+    far = [b.link("f%d" % i, plain[i], dims - 1) for i in range(cells)]
+    near = [b.link("n%d" % i, plain[i], dims - 2) for i in range(cells)]
+    for i in range(cells):  # start-of-pass values from the other half of the cube
+        b.nand(far[i])
+        b.nand(plain[(i + 1) % cells])
+        b.sto(plain[i])
+    for i in range(cells):  # values computed in the pass, read across the waves
+        b.nand(near[(i * 7) % cells])
+        b.nand(far[(i * 5 + 3) % cells])
+        b.sto(plain[(i + 11) % cells])
+    return b.finish()
+
+
+def test_cube_in_one_workgroup_many_rows(engine):
+    # 48 start-of-pass values cross the waves at one level: with 512-word cubes a chunk holds 32
+    # LDS rows, so the level takes two chunks (write, barrier, reads, barrier each)
+    prog = wide_cross_program()
+    print("check wide cross-wave program, dims 14", flush=True)
+    code = check(engine, prog, groups=2 * prog.groups, tune={"lds_links": 1}, runs=(1, 3))
+    assert code.stats()["resident"] == 1
+
+
 @pytest.mark.parametrize("dims", [9, 12])
 def test_rank(engine, dims):
     print("check rank", dims, flush=True)

@@ -628,6 +628,12 @@ def test_cube_in_one_workgroup_source():
     p = random_cube_program(16, 300, 15, seed=15, links=30)
     src, info = t.source(p)
     assert info["resident"] == 0 and "ts_xs[" not in src
+    # more values crossing the waves at one level than 64 KB of rows holds (48 rows of 512
+    # words, 32 at a time): the level is written and read in two chunks
+    from tests.test_gpu_run import wide_cross_program
+    src, info = t.source(wide_cross_program(), {"lds_links": 1})
+    assert "__shared__ u32 ts_xs[%d];" % (32 * 512) in src
+    t.rtc_check(src)
     # the default weighs one cube's pass in both forms: gamma on 12 dims (6,311 LUTs one thread
     # evaluates in one workgroup, 5,945 in the segments' largest slices) stays segmented
     gamma = golden("gamma_4_4_4")

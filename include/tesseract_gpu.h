@@ -54,7 +54,7 @@ typedef struct {
                               (x bits 0..4) reaches evaluated once per group, bit-sliced over its
                               words; 0 = auto (groups for one tick, at least 24 inputs and at
                               least 32 such LUTs) */
-    unsigned lds_links;    /* cubes of 128 to 512 words (2^12 to 2^14 machines): 1 = each cube
+    unsigned lds_links;    /* cubes of 128 to 1,024 words (2^12 to 2^15 machines): 1 = each cube
                               runs in one workgroup of that many threads, reads across the waves
                               of its cube through LDS (a barrier pair per level of such reads),
                               the pass loop in-kernel with the state in registers; 2 = as larger

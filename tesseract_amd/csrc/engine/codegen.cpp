@@ -628,10 +628,12 @@ void codegen_run(const Lowered &lw, const CodegenOptions &opt, KernelSource &ks)
         const bool in_wg = lw.wg_dims > lw.wave_dims;
         const unsigned B = in_wg ? 1u << (lw.wg_dims - 5) : opt.block;
         const unsigned rows_max = in_wg ? (64u * 1024u) / (4u * B) : 0u;
+        // a workgroup's waves share the 4 SIMDs of its CU: at least B / 256 of them per SIMD
+        const unsigned wg_waves = in_wg ? std::max(waves, B / 256u) : waves;
         const std::string wg_bounds =
             "__launch_bounds__(" + std::to_string(B) + ")" +
-            (waves > 1 ? " __attribute__((amdgpu_waves_per_eu(" + std::to_string(waves) + ")))"
-                       : std::string());
+            (wg_waves > 1 ? " __attribute__((amdgpu_waves_per_eu(" + std::to_string(wg_waves) + ")))"
+                          : std::string());
         // state variables carried from pass to pass: the cells whose value at the start of a pass
         // is read (version -1). A cell the pass writes before any read of it is no loop-carried
         // variable: its value of the last pass is stored as the last pass makes it, so no register

@@ -286,7 +286,7 @@ int generate(const ts_program *p, gpu_tune &tune, Lowered &lw, KernelSource &ks,
         codegen_run(l, co, k);
         return true;
     };
-    // A cube of 128-512 words may run in one workgroup (links across its waves through LDS, the
+    // A cube of 128-1,024 words may run in one workgroup (links across its waves through LDS, the
     // pass loop in-kernel) or segmented (slices of threads per segment, a launch per segment).
     // By default the form with the shorter pass for one cube is taken, from the LUTs one thread
     // evaluates in a pass (the workgroup form's whole pass; the segments' largest slices) and the
@@ -295,8 +295,8 @@ int generate(const ts_program *p, gpu_tune &tune, Lowered &lw, KernelSource &ks,
     // 94 us, against 2 segments of 5,945, 76 us: the workgroup form carries ~1,850 cells in
     // registers and barriers between its levels), about 44 cycles per LUT in one workgroup, 35
     // in a segment and 12,000 per launch.
-    const bool may_wg = dims >= 12 && dims <= 14 && tune.lds_links != 2;
-    if (!make(may_wg ? 9 : 0, lw, ks))
+    const bool may_wg = dims >= 12 && dims <= 15 && tune.lds_links != 2;
+    if (!make(may_wg ? 10 : 0, lw, ks))
         return 0;
     if (may_wg && tune.lds_links != 1 && lw.resident && lw.wg_dims > lw.wave_dims) {
         Lowered lw2;

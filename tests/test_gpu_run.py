@@ -74,9 +74,9 @@ def test_random_cube(engine, dims):
 
 
 @pytest.mark.parametrize("lds_links", [1, 2])
-@pytest.mark.parametrize("dims", [12, 13, 14])
+@pytest.mark.parametrize("dims", [12, 13, 14, 15])
 def test_cube_in_one_workgroup(engine, dims, lds_links):
-    # cubes of 128-512 words: each cube in one workgroup, reads across its waves through LDS and
+    # cubes of 128-1,024 words: each cube in one workgroup, reads across its waves through LDS and
     # the pass loop in-kernel (lds_links 1), or segments through HBM (2); three cubes per state,
     # several calls, every machine against the oracle
     prog = random_cube_program(24, 600, dims, seed=300 + dims, links=40)

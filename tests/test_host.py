@@ -612,11 +612,11 @@ def test_generated_kernels_compile_for_gfx950():
 
 
 def test_cube_in_one_workgroup_source():
-    # a cube of 128-512 words compiles to one resident kernel whose workgroup is the cube (its
+    # a cube of 128-1,024 words compiles to one resident kernel whose workgroup is the cube (its
     # reads across waves through LDS, a barrier pair per chunk of each level); larger cubes and
     # lds_links = 2 keep the segmented form; every form compiles for gfx950
     from tests.test_gpu_run import random_cube_program
-    for dims in (12, 14):
+    for dims in (12, 14, 15):
         p = random_cube_program(16, 300, dims, seed=dims, links=30)
         src, info = t.source(p)
         assert info["resident"] == 1 and info["segments"] == 1
@@ -625,7 +625,7 @@ def test_cube_in_one_workgroup_source():
         t.rtc_check(src)
         src2, info2 = t.source(p, {"lds_links": 2})
         assert info2["resident"] == 0 and "ts_xs[" not in src2
-    p = random_cube_program(16, 300, 15, seed=15, links=30)
+    p = random_cube_program(16, 300, 16, seed=16, links=30)
     src, info = t.source(p)
     assert info["resident"] == 0 and "ts_xs[" not in src
     # more values crossing the waves at one level than 64 KB of rows holds (48 rows of 512

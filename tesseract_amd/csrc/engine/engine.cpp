@@ -268,6 +268,13 @@ namespace {
 int generate(const ts_program *p, gpu_tune &tune, Lowered &lw, KernelSource &ks, char *err,
              size_t errlen)
 {
+    if (tune.sched == 2) {
+        // the compiler itself faults on some pass kernels under this strategy (the reference's
+        // PERM8 program, Life's segment kernel) and takes the process down with it
+        say(err, errlen, "the iterative-ilp scheduling strategy (sched 2) is not used for pass "
+                         "kernels: the compiler faults on some of them; take 0/1, 3 or 4");
+        return 0;
+    }
     LowerOptions lo;
     unsigned dims = ts_program_dims(p);
     lo.cube_words_log2 = dims - 5;

@@ -642,6 +642,16 @@ def test_cube_in_one_workgroup_source():
     assert one["resident"] == 1 and one["chain"] == one["bitop3"]
 
 
+def test_pass_kernels_refuse_iterative_ilp():
+    # the compiler faults (the process dies) on some pass kernels scheduled iterative-ilp, the
+    # reference's PERM8 among them: the engine refuses that strategy for them with a message, and
+    # still takes it for enumeration kernels, where it compiles large kernels fastest
+    with pytest.raises(RuntimeError, match="iterative-ilp"):
+        t.source(golden("perm8"), {"sched": 2})
+    t.source(golden("perm8"), {"sched": 3})
+    t.enum_source(t.EN_VERIFY, golden("mul8"), spec="p = a * b", tune={"sched": 2})
+
+
 def test_enum_validation_messages():
     big = t.net_parse("\n".join("INPUT(i%d)" % k for k in range(47)) + "\nOUTPUT(o)\no = AND(i0, i1)\n")
     with pytest.raises(ValueError, match="47 inputs is too many"):

@@ -33,9 +33,12 @@ def test_beta_on_the_device(engine, ref):
     assert [beta_fields(g) for g in got] == [beta_fields(b) for b in betas]
 
 
-@pytest.mark.parametrize("name,ticks", [("gamma_3_3_3", 9), ("gamma_4_4_4", 12)])
-def test_gamma_on_the_device(engine, ref, name, ticks):
-    # a soup with predators on the torus; ticks in two calls against as many gamma_step ticks
+@pytest.mark.parametrize("name,ticks,tune", [("gamma_3_3_3", 9, None), ("gamma_4_4_4", 12, None),
+                                             ("gamma_4_4_4", 12, {"lds_links": 1})])
+def test_gamma_on_the_device(engine, ref, name, ticks, tune):
+    # a soup with predators on the torus; ticks in two calls against as many gamma_step ticks.
+    # gamma_4_4_4 is a cube of 128 words: segmented by default, and once in one workgroup per
+    # cube (its links across the two waves through LDS, ~1,850 carried cells spilled)
     prog = golden(name)
     cells = prog.extra["cells"]
     geo = [int(v) for v in prog.extra["geometry"]]
@@ -44,8 +47,11 @@ def test_gamma_on_the_device(engine, ref, name, ticks):
     assert L.gamma_soup(world.buf, 0.3, 60, 11) > 0
     L.gamma_predators(world.buf, 1 << (sum(geo) - 4), 12)
     mem, rr = ref.gamma_lines(prog.ptr, prog.ncells, cells.ctypes.data, world)
-    print("compiling %s (%d instructions, %d dims)" % (name, prog.len, prog.dims), flush=True)
-    code = engine.compile(prog)
+    print("compiling %s (%d instructions, %d dims), tune %s" % (name, prog.len, prog.dims, tune),
+          flush=True)
+    code = engine.compile(prog, tune)
+    if tune:
+        assert code.stats()["resident"] == 1
     st = code.state()
     st.put(mem, rr)
     st.run(ticks // 3)

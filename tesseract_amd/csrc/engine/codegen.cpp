@@ -832,6 +832,9 @@ struct SpecCode {
     // left shifts by constants, bit i of the result depends on bits <= i of the operands, so bits
     // below the lowest vector bit fed by a lane (x bits 0..4) are the same in all 32 lanes.
     int lane_free = 0;
+    // By the same rule, bits below the lowest vector bit fed by x bits 5..9 (the word within a
+    // group of 32 words) are the same in every word of the group.
+    int group_free = 0;
 };
 
 bool lower_spec(const ts_spec *sp, SpecCode &sc, std::string &err)
@@ -853,12 +856,17 @@ bool lower_spec(const ts_spec *sp, SpecCode &sc, std::string &err)
         if (rpn[i].op == TS_T_SHR || (rpn[i].op == TS_T_SHL && (i == 0 || rpn[i - 1].op != TS_T_NUM)))
             low_monotone = false;
     if (low_monotone) {
-        int m = 64;
+        int m = 64, mg = 64;
         for (int k = 0; k < nv; k++)
-            for (int b = 0; b < ts_spec_var_bits(sp, k); b++)
-                if (ts_spec_var_pos(sp, k, b) < 5)
+            for (int b = 0; b < ts_spec_var_bits(sp, k); b++) {
+                const int pos = ts_spec_var_pos(sp, k, b);
+                if (pos < 5)
                     m = std::min(m, b);
+                else if (pos < 10)
+                    mg = std::min(mg, b);
+            }
         sc.lane_free = std::min(m, width);
+        sc.group_free = std::min(mg, width);
     }
     const char *T = u32 ? "u32" : "u64";
     std::ostringstream pre;
@@ -1313,6 +1321,7 @@ bool codegen_enum(const ts_program *a, const ts_program *b, const int *in_map, c
     };
 
     std::vector<std::string> oa, ob;  // expressions of the compared output words
+    std::vector<uint8_t> hoisted(el.width, 0);  // verify bits compared once per group (gmiss)
     if (grouped) {
         o << "  const " << W << " g0 = i0 >> 5, g1 = (i1 + 31u) >> 5;\n"
           << "  for (" << W << " gbase = g0 + (" << W << ")blockIdx.x * " << opt.block
@@ -1368,6 +1377,41 @@ bool codegen_enum(const ts_program *a, const ts_program *b, const int *in_map, c
             }
             o << "    }\n";
         }
+        // verify bits that no lane and no word of the group reaches (below lin_free and
+        // group_free), against output planes that are the same in every word of the group: one
+        // compare per group instead of one per word
+        if (lin) {
+            const char *T = sc.u32 ? "u32" : "u64";
+            const int gf = std::min(lin_free, sc.group_free);
+            std::vector<int> bits;
+            for (int bit = 0; bit < gf; bit++)
+                if (cls[out_a[bit] >> 1] == 0)
+                    bits.push_back(bit);
+            if (!bits.empty()) {
+                o << "    u32 gmiss = 0u;\n    {\n"
+                  << "    const u64 xb = (u64)gwb << 5;\n"
+                  << "    const u64 xr = __builtin_bitreverse64(xb);\n    (void)xr;\n"
+                  << sc.prelude << "    const " << T << " bg = " << sl.b << ";\n";
+                for (int bit : bits) {
+                    o << "    gmiss = __builtin_amdgcn_bitop3_b32(gmiss, (u32)__builtin_amdgcn_sbfe("
+                         "(int)(u32)(bg"
+                      << (bit >= 32 ? " >> 32" : "") << "), " << bit % 32 << "u, 1u), "
+                      << eg.val(out_a[bit]) << ", 0xF6u);\n";
+                    hoisted[bit] = 1;
+                }
+                o << "    }\n";
+            }
+        }
+        // the words of the group inside [i0, i1) and inside the space: bit w for word ib + w
+        o << "    u32 gvalid;\n    {\n"
+          << "    const u64 sw = (space + 31ull) >> 5;  // words of the space\n"
+          << "    const u64 lo = (u64)i0 > (u64)ib ? (u64)i0 - (u64)ib : 0ull;\n"
+          << "    u64 hi = (u64)i1 > (u64)ib ? (u64)i1 - (u64)ib : 0ull;\n"
+          << "    const u64 hs = sw > (u64)gwb ? sw - (u64)gwb : 0ull;\n"
+          << "    hi = hi < hs ? hi : hs;\n"
+          << "    hi = hi < 32ull ? hi : 32ull;\n"
+          << "    gvalid = hi > lo ? (u32)(((1ull << hi) - 1ull) & ~((1ull << lo) - 1ull)) : 0u;\n"
+          << "    }\n";
         // the group values a word reads: LUT leaves of its own part and compared outputs
         std::vector<uint8_t> import(g.nodes.size(), 0);
         for (size_t k = 1; k < g.nodes.size(); k++)
@@ -1383,10 +1427,9 @@ bool codegen_enum(const ts_program *a, const ts_program *b, const int *in_map, c
                 import[v >> 1] = 1;
         o << "    #pragma unroll 1\n"
           << "    for (u32 w = 0; w < 32u; w++) {\n"
-          << "    const " << W << " i = ib + w;\n"
           << "    const " << W << " gw = gwb + w;\n"
           << "    const u64 xb = (u64)gw << 5;\n"
-          << "    const u32 valid = i >= i0 && i < i1 && xb < space ? 0xffffffffu : 0u;\n";
+          << "    const u32 valid = (u32)__builtin_amdgcn_sbfe((int)gvalid, w, 1u);\n";
         for (size_t k = 1; k < g.nodes.size(); k++)
             if (import[k])
                 o << "    const u32 v" << k << " = "
@@ -1504,7 +1547,8 @@ bool codegen_enum(const ts_program *a, const ts_program *b, const int *in_map, c
             // each expected bit plane, compared as it is made
             o << "    const " << T << " bw = " << sl.b << ";\n";
             for (int bit = 0; bit < lin_free; bit++)
-                gather(bit_of("bw", bit), oa[bit]);
+                if (!hoisted[bit])
+                    gather(bit_of("bw", bit), oa[bit]);
             for (int bit = lin_free; bit < el.width; bit++) {
                 const std::string tp = "tp" + std::to_string(bit), rb = "r" + std::to_string(bit);
                 o << "    const u32 " << rb << " = " << bit_of("bw", bit) << ";\n";
@@ -1520,7 +1564,8 @@ bool codegen_enum(const ts_program *a, const ts_program *b, const int *in_map, c
                     gather("s" + std::to_string(bit), oa[bit]);
                 }
             }
-            o << "    hit = (miss0 | miss1 | miss2 | miss3) & valid;\n";
+            o << "    hit = (miss0 | miss1 | miss2 | miss3"
+              << (std::count(hoisted.begin(), hoisted.end(), 1) ? " | gmiss" : "") << ") & valid;\n";
         } else {
             for (int l = 0; l < 32; l++)
                 o << "    const " << T << " e" << l << " = " << sc.lane_expr[l] << ";\n";

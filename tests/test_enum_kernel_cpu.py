@@ -58,17 +58,22 @@ def test_mul16_shares_against_the_reference(ref, part, part_end):
 
 @pytest.mark.parametrize("spec", ["p = b * a", "p = ~(0 - a * b) + 1",
                                   "p = a * b + (a << 3) - a * 8", "p = (a + 1) * b - b",
-                                  "p = a * b + 12345", "p = a * b ^ 1"])
+                                  "p = a * b + 12345", "p = a * b ^ 1", "p = a * b + (a & 1)",
+                                  "p = a * b + (b & 64)"])
 def test_mul16_spec_forms(spec):
     # specs equal to a * b in several forms, some kept linear in the lane bits (the grouped
-    # kernel's bit-sliced add) and some not (its per-lane values); one differs: a * b ^ 1 in every
-    # assignment, a * b + 12345 in every assignment
+    # kernel's bit-sliced add) and some not (its per-lane values); some differ: a * b ^ 1 and
+    # a * b + 12345 in every assignment (bit 0, which no lane and no word of a group reaches, is
+    # compared once per group), + (a & 1) where a is odd (a group's whole 32 words, from bit 0
+    # up), + (b & 64) where b bit 6 (x bit 9, a word bit of the group) is set
     prog = golden("mul16")
     plain, grouped = _both(t.EN_VERIFY, prog, spec=spec, part=5, part_end=5, parts=1024)
     hp, xp = K.run(plain, 32, part=5, part_end=5, parts=1024)
     hg, xg = K.run(grouped, 32, part=5, part_end=5, parts=1024)
     assert (hp, xp) == (hg, xg)
-    assert hg == ((1 << 22) if spec in ("p = a * b + 12345", "p = a * b ^ 1") else 0)
+    want = {"p = a * b + 12345": 1 << 22, "p = a * b ^ 1": 1 << 22,
+            "p = a * b + (a & 1)": 1 << 21, "p = a * b + (b & 64)": 1 << 21}
+    assert hg == want.get(spec, 0)
 
 
 def test_ranges_inside_groups():

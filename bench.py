@@ -354,8 +354,9 @@ def run_workload(args):
 def profiled(source_sha16):
     """From the committed counter summary of the ts_enum kernel: HBM bytes per launch
     (FETCH_SIZE + WRITE_SIZE runs scaled by the copy-kernel factor) and VALU instructions per
-    launch (SQ_INSTS_VALU). Each only if the summary was measured on the kernel source this run
-    compiles (its sha256 prefix), else None."""
+    launch (SQ_INSTS_VALU; the same under each scheduling strategy the timed choice takes). Each
+    only if the summary was measured on the kernel source this run compiles (its sha256 prefix),
+    else None."""
     path = os.path.join(ROOT, "profiles", "ts_enum_mul16_verify_counters.txt")
     got = {}
     try:
@@ -381,6 +382,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-parts", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sched", type=int, default=0, choices=[0, 1, 2, 3],
+                    help="mul16: the kernel's scheduling strategy (gpu_tune.sched; 0 = timed "
+                         "choice among 1-3)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="afterwards copy a MIB-MiB device buffer (copy_probe): a kernel of known "
                          "bytes whose FETCH_SIZE/WRITE_SIZE readings scale the kernel's")
@@ -424,7 +428,8 @@ def main():
     from tesseract_amd.programs import golden
     prog = golden("mul16")  # the reference's gen_mul(16), a program file
     t0 = time.perf_counter()
-    code = t.EnumCode(eng, t.EN_VERIFY, prog, spec="p = a * b")
+    tune = {"sched": args.sched} if args.sched else None
+    code = t.EnumCode(eng, t.EN_VERIFY, prog, spec="p = a * b", tune=tune)
     compile_s = time.perf_counter() - t0
     for _ in range(args.warmup):
         r = code.run()

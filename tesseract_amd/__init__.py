@@ -575,9 +575,12 @@ def enum_source(mode, a, b=None, spec=None, ticks=1, part=0, part_end=0, parts=1
     return _take_string(out)
 
 
+TAPE_GROUP = 16  # entries per tape group (gpu_tape)
+
+
 def enum_tape(mode, a, b=None, spec=None, max_slots=0):
     """The interpreter tier's tape of an equiv/search enumeration (no device): a dict of numpy
-    arrays ops (groups x 5 x 16: fields a, b, d, ma, mb), count, leaves and root, slots,
+    arrays ops (groups x 5 x TAPE_GROUP: fields a, b, d, ma, mb), count, leaves and root, slots,
     and_nodes."""
     import numpy as np
     en = _enum_struct(mode, a, b, spec, 1, 0, 0, 1, None)
@@ -587,8 +590,8 @@ def enum_tape(mode, a, b=None, spec=None, max_slots=0):
         raise ValueError(e.value.decode())
     try:
         g = t.ngroups
-        out = {"ops": np.ctypeslib.as_array(t.ops, (g * 80,)).reshape(g, 5, 16).copy() if g else
-               np.zeros((0, 5, 16), np.uint32),
+        out = {"ops": np.ctypeslib.as_array(t.ops, (g * 5 * TAPE_GROUP,)).reshape(
+                   g, 5, TAPE_GROUP).copy() if g else np.zeros((0, 5, TAPE_GROUP), np.uint32),
                "count": np.ctypeslib.as_array(t.count, (g,)).copy() if g else np.zeros(0, np.uint32),
                "leaves": np.ctypeslib.as_array(t.leaves, (t.nleaves,)).copy() if t.nleaves else
                np.zeros(0, np.uint32),

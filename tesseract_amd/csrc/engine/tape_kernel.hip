@@ -6,7 +6,7 @@
 // word at byte slot * 256 + 4 l of the workgroup's LDS file, so every slot access of a wave is one
 // conflict-free 256-byte row, and the waves split each group's 16 entries (4 each) with one
 // barrier per group: four times the waves per CU that one wave per file allows. A group's
-// entries are described by 80 words (the tape format in lower.hpp), each wave loading its share
+// entries are described by 5 x kGroup words (the tape format in lower.hpp), each wave loading its share
 // one group ahead. Each lane enumerates one word (32 assignments) at a time exactly like the
 // compiled ts_enum kernel — same word order, same share rule, same valid mask, same hit list of
 // (word, mask) pairs — so the host's collection of hits is shared by both tiers.
@@ -42,12 +42,12 @@ constexpr unsigned kQ = kShare / 4;           // 16-byte words per field and wav
 __device__ inline void load_share(uint4 (&d)[5 * kQ], const uint4 *__restrict__ ops, uint32_t g,
                                  uint32_t wave, uint32_t lane_zero)
 {
-    const uint4 *p = ops + (size_t)g * 20 + wave * kQ + lane_zero;
+    const uint4 *p = ops + (size_t)g * (ts::Tape::kWords / 4) + wave * kQ + lane_zero;
 #pragma unroll
     for (unsigned f = 0; f < 5; f++)
 #pragma unroll
         for (unsigned q = 0; q < kQ; q++)
-            d[kQ * f + q] = p[4 * f + q];
+            d[kQ * f + q] = p[(kGroup / 4) * f + q];
 }
 
 // word j of field f of the wave's share

@@ -32,29 +32,38 @@ __device__ inline uint32_t lane_pattern(uint32_t j)
 
 constexpr unsigned kWaves = 4;               // waves per workgroup, over one slot file
 constexpr unsigned kShare = kGroup / kWaves;  // entries per wave
-constexpr unsigned kQ = kShare / 4;           // 16-byte words per field and wave
+static_assert(kShare == 2 || kShare % 4 == 0, "a wave's share loads as 8- or 16-byte vectors");
 
-// A wave's share of a group's descriptors: fields a, b, d, ma, mb (16 words each), entries
-// kShare h .. kShare h + kShare - 1, as 5 kQ 16-byte vector loads. `zero` is 0 at run time; adding (lane & zero) makes
-// the address per-lane for the compiler, so the words arrive through vector memory (counted by
-// vmcnt) rather than scalar loads, whose counter is shared with LDS reads and would make every
-// LDS wait also wait for the next group's descriptors.
-__device__ inline void load_share(uint4 (&d)[5 * kQ], const uint4 *__restrict__ ops, uint32_t g,
+// A wave's share of a group's descriptors: fields a, b, d, ma, mb, kShare words each (entries
+// kShare h .. kShare h + kShare - 1), loaded as 8- or 16-byte vectors. `zero` is 0 at run time;
+// adding (lane & zero) makes the address per-lane for the compiler, so the words arrive through
+// vector memory (counted by vmcnt) rather than scalar loads, whose counter is shared with LDS
+// reads and would make every LDS wait also wait for the next group's descriptors.
+struct Share {
+    uint32_t w[5][kShare];
+};
+
+__device__ inline void load_share(Share &d, const uint32_t *__restrict__ ops, uint32_t g,
                                  uint32_t wave, uint32_t lane_zero)
 {
-    const uint4 *p = ops + (size_t)g * (ts::Tape::kWords / 4) + wave * kQ + lane_zero;
+    const uint32_t *p = ops + (size_t)g * ts::Tape::kWords + wave * kShare + lane_zero;
 #pragma unroll
-    for (unsigned f = 0; f < 5; f++)
+    for (unsigned f = 0; f < 5; f++) {
+        if constexpr (kShare == 2) {
+            const uint2 x = *(const uint2 *)(p + kGroup * f);
+            d.w[f][0] = x.x;
+            d.w[f][1] = x.y;
+        } else {
 #pragma unroll
-        for (unsigned q = 0; q < kQ; q++)
-            d[kQ * f + q] = p[(kGroup / 4) * f + q];
-}
-
-// word j of field f of the wave's share
-__device__ inline uint32_t word(const uint4 (&d)[5 * kQ], unsigned f, unsigned j)
-{
-    const uint4 &q = d[kQ * f + (j >> 2)];
-    return (j & 3) == 0 ? q.x : (j & 3) == 1 ? q.y : (j & 3) == 2 ? q.z : q.w;
+            for (unsigned q = 0; q < kShare / 4; q++) {
+                const uint4 x = *(const uint4 *)(p + kGroup * f + 4 * q);
+                d.w[f][4 * q] = x.x;
+                d.w[f][4 * q + 1] = x.y;
+                d.w[f][4 * q + 2] = x.z;
+                d.w[f][4 * q + 3] = x.w;
+            }
+        }
+    }
 }
 
 // Hits, reported a wave at a time (every lane of the wave takes part). Each lane counts its hits in
@@ -109,22 +118,22 @@ __device__ inline void count_hits(uint64_t *out, uint32_t acc)
 }
 
 // a wave's entries of one group: every read, then every write
-__device__ inline void run_share(const uint4 (&d)[5 * kQ], char *my)
+__device__ inline void run_share(const Share &d, char *my)
 {
     uint32_t v[kShare];
 #pragma unroll
     for (unsigned j = 0; j < kShare; j++) {
-        const uint32_t x = *(const uint32_t *)(my + word(d, 0, j)) ^ word(d, 3, j);
-        const uint32_t y = *(const uint32_t *)(my + word(d, 1, j)) ^ word(d, 4, j);
+        const uint32_t x = *(const uint32_t *)(my + d.w[0][j]) ^ d.w[3][j];
+        const uint32_t y = *(const uint32_t *)(my + d.w[1][j]) ^ d.w[4][j];
         v[j] = x & y;
     }
 #pragma unroll
     for (unsigned j = 0; j < kShare; j++)
-        *(uint32_t *)(my + word(d, 2, j)) = v[j];
+        *(uint32_t *)(my + d.w[2][j]) = v[j];
 }
 
 __global__ void __launch_bounds__(64 * kWaves)
-    ts_enum_tape(const uint4 *__restrict__ ops, uint32_t ngroups, const uint32_t *__restrict__ leaves,
+    ts_enum_tape(const uint32_t *__restrict__ ops, uint32_t ngroups, const uint32_t *__restrict__ leaves,
                  uint32_t nleaves, uint32_t root, uint32_t zero, uint64_t *__restrict__ out,
                  uint64_t *__restrict__ list, uint64_t cap, uint64_t i0, uint64_t i1, uint64_t space,
                  uint32_t parts, uint32_t part, uint32_t nshare)
@@ -158,7 +167,7 @@ __global__ void __launch_bounds__(64 * kWaves)
             }
         __syncthreads();
         // two descriptor buffers: the next group's loads are in flight while this one runs
-        uint4 da[5 * kQ], db[5 * kQ];
+        Share da, db;
         if (ngroups)
             load_share(da, ops, 0, wave, lane_zero);
         for (uint32_t g = 0; g < ngroups; g += 2) {
@@ -216,7 +225,7 @@ bool tape_launch(const TapeDev &t, hipStream_t stream, unsigned grid, uint64_t *
                  uint64_t *list, uint64_t cap, uint64_t i0, uint64_t i1, uint64_t space,
                  uint32_t parts, uint32_t part, uint32_t nshare)
 {
-    ts_enum_tape<<<grid, 64 * kWaves, tape_lds_bytes(t), stream>>>((const uint4 *)t.ops, t.ngroups,
+    ts_enum_tape<<<grid, 64 * kWaves, tape_lds_bytes(t), stream>>>(t.ops, t.ngroups,
                                                            t.leaves, t.nleaves, t.root, 0u, out,
                                                            list, cap, i0, i1, space, parts, part,
                                                            nshare);

@@ -45,13 +45,12 @@ typedef struct {
     unsigned max_waves;    /* enumerations: at most this many waves per SIMD run at once (each
                               workgroup reserves LDS it does not use; 0 = no cap) */
     unsigned sched;        /* the compiler's instruction scheduling: 1 = its default,
-                              2 = iterative-ilp, 3 = max-ilp, 4 = iterative-minreg; 0 = auto: an
-                              enumeration of at least 2^40 1-bit ops is compiled with 1, 2 and 3,
-                              each timed on a sample of its words, and the fastest kept;
-                              otherwise 2, which compiles fastest on large kernels. Pass kernels
-                              (gpu_compile) take 0 (= 1), 1, 3 or 4: the compiler faults on some
-                              of them under 2 (the reference's PERM8, Life's segment kernel), so
-                              gpu_compile refuses it */
+                              3 = max-ilp, 4 = iterative-minreg; 0 = auto: an enumeration of at
+                              least 2^40 1-bit ops is compiled with 1 and 3, each timed on a sample
+                              of its words, and the faster kept; otherwise 1. 2 (iterative-ilp) is
+                              refused with an error: the compiler itself faults on some kernels
+                              under it (the reference's PERM8, Life's segment kernel, the mul16
+                              verify against a * b + (a & 1)) */
     unsigned group;        /* enumerations: 1 = a thread takes one word of 32 assignments at a
                               time; 2 = groups of 32 words, the part of the program no lane bit
                               (x bits 0..4) reaches evaluated once per group, bit-sliced over its

@@ -106,6 +106,11 @@ const char *ts_internal::sched_strategy(unsigned sched)
 int ts_internal::rtc_compile(const std::string &src, const std::string &arch_name, std::vector<char> &code,
                 double *seconds, char *err, size_t errlen, unsigned sched)
 {
+    if (sched == 2) {
+        say(err, errlen, "the iterative-ilp scheduling strategy (sched 2) is not used: the compiler "
+                         "faults on some kernels under it; take 0/1, 3 or 4");
+        return 0;
+    }
     auto t0 = std::chrono::steady_clock::now();
     hiprtcProgram prog;
     if (hiprtcCreateProgram(&prog, src.c_str(), "tesseract_gen.hip", 0, nullptr, nullptr) !=
@@ -1034,21 +1039,22 @@ int enum_buffers(gpu_enum_code *c, char *err, size_t errlen)
     return 1;
 }
 
-// Compiles the enumeration kernel with scheduling strategies 1, 2 and 3 (in parallel host
-// threads), times each on the first `sample` words of the share (a warm-up launch, then the best
-// of two) and keeps the fastest module in c.
+// Compiles the enumeration kernel with scheduling strategies 1 and 3 (in parallel host threads),
+// times each on the first `sample` words of the share (a warm-up launch, then the best of two)
+// and keeps the fastest module in c.
 int enum_autotune(gpu_enum_code *c, const std::string &src, uint64_t sample, char *err,
                   size_t errlen)
 {
     gpu_engine *e = c->e;
-    const unsigned cand[3] = {1, 2, 3};
-    std::vector<char> code[3];
-    std::string msg[3];
-    double secs[3] = {0, 0, 0};
-    int ok[3] = {0, 0, 0};
+    constexpr int kCand = 2;
+    const unsigned cand[kCand] = {1, 3};
+    std::vector<char> code[kCand];
+    std::string msg[kCand];
+    double secs[kCand] = {0, 0};
+    int ok[kCand] = {0, 0};
     {
         std::vector<std::thread> th;
-        for (int k = 0; k < 3; k++)
+        for (int k = 0; k < kCand; k++)
             th.emplace_back([&, k] {
                 char m[1024] = "";
                 ok[k] = ts_internal::rtc_compile(src, e->arch, code[k], &secs[k], m, sizeof m, cand[k]);
@@ -1060,11 +1066,11 @@ int enum_autotune(gpu_enum_code *c, const std::string &src, uint64_t sample, cha
     HIPCHK(hipSetDevice(e->device));
     if (!enum_buffers(c, err, errlen))
         return 0;
-    hipModule_t mods[3] = {nullptr, nullptr, nullptr};
+    hipModule_t mods[kCand] = {nullptr, nullptr};
     double best = 0;
     int keep = -1;
     ShareInfo si = ts_internal::shares(c->n, c->part, c->part_end, c->parts);
-    for (int k = 0; k < 3; k++) {
+    for (int k = 0; k < kCand; k++) {
         if (!ok[k] || hipModuleLoadData(&mods[k], code[k].data()) != hipSuccess)
             continue;
         hipFunction_t fn;
@@ -1096,7 +1102,7 @@ int enum_autotune(gpu_enum_code *c, const std::string &src, uint64_t sample, cha
             keep = k;
         }
     }
-    for (int k = 0; k < 3; k++)
+    for (int k = 0; k < kCand; k++)
         if (mods[k] && k != keep)
             (void)hipModuleUnload(mods[k]);
     if (keep < 0) {
@@ -1106,7 +1112,7 @@ int enum_autotune(gpu_enum_code *c, const std::string &src, uint64_t sample, cha
     c->mod = mods[keep];
     c->sched = cand[keep];
     c->sample_words = sample;
-    c->compile_seconds = secs[0] + secs[1] + secs[2];
+    c->compile_seconds = secs[0] + secs[1];
     HIPCHK(hipModuleGetFunction(&c->fn, c->mod, "ts_enum"));
     return 1;
 }

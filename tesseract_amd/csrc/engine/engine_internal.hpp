@@ -64,14 +64,15 @@ int enum_device(const gpu_enum_code *c);
 int static_scratch_init(int device, char *err, size_t errlen);
 
 // HIP source -> code object for arch (no device needed). sched picks the compiler's instruction
-// scheduling strategy: 0 or 1 its default, 2 iterative-ilp, 3 max-ilp, 4 iterative-minreg.
+// scheduling strategy: 0 or 1 its default, 3 max-ilp, 4 iterative-minreg; 2 (iterative-ilp) is
+// refused: the compiler itself faults on some kernels under it (the reference's PERM8, Life's
+// segment kernel, the mul16 verify against a * b + (a & 1)) and takes the process with it.
 int rtc_compile(const std::string &src, const std::string &arch, std::vector<char> &code,
                 double *seconds, char *err, size_t errlen, unsigned sched = 0);
 const char *sched_strategy(unsigned sched);  // the strategy's name, nullptr for the default
 // The strategy enumeration kernels are compiled with when none is timed (short enumerations and
-// the server): iterative-ilp compiled the serving stream's largest kernel (13,823 LUTs) in 1.86 s
-// against 2.66 s with the default, and mul16's timed samples came within 3 % of each other.
-constexpr unsigned kEnumSched = 2;
+// the server): the compiler's default.
+constexpr unsigned kEnumSched = 1;
 
 // validation with the reference's messages, spec parsing and code generation of an enumeration
 int enum_prepare(const gpu_engine *e, const gpu_enum *en, gpu_tune &tune, ts::KernelSource &ks,

@@ -158,6 +158,24 @@ def test_mul16_counts_every_assignment(engine):
     assert r["hit"] == [(1 << 31) + (1 << 15) + k for k in range(16)]
 
 
+def test_enumerations_refuse_iterative_ilp(engine):
+    # the compiler itself faults on some enumeration kernels scheduled iterative-ilp (this one
+    # among them): the compile refuses the strategy with a message, and the timed choice of long
+    # enumerations takes 1 or 3
+    with pytest.raises((ValueError, RuntimeError), match="iterative-ilp"):
+        engine.enum_run(t.EN_VERIFY, golden("mul16"), spec="p = a * b + (a & 1)", tune={"sched": 2})
+
+
+def test_mul16_difference_in_group_compared_bits(engine):
+    # a * b + (a & 1) keeps the linear form: the difference is in bit 0 (compared once per group
+    # of 32 words, no lane and no word of a group reaching it) and above, wherever a is odd (x bit
+    # 31): 2^31 assignments, the smallest 2^31 + 0..15
+    print("mul16 against a * b + (a & 1) over 2^32", flush=True)
+    r = engine.enum_run(t.EN_VERIFY, golden("mul16"), spec="p = a * b + (a & 1)")
+    assert r["tried"] == 1 << 32 and r["hits"] == 1 << 31
+    assert r["hit"] == [(1 << 31) + k for k in range(16)]
+
+
 def test_toyhash32_search_random_target(engine, ref):
     # a seeded target with a known preimage: every hit maps back to the target, and the count
     # and the 16 smallest equal the reference's enum_run over all 2^32 assignments

@@ -382,9 +382,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-parts", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--sched", type=int, default=0, choices=[0, 1, 2, 3],
+    ap.add_argument("--sched", type=int, default=0, choices=[0, 1, 3, 4],
                     help="mul16: the kernel's scheduling strategy (gpu_tune.sched; 0 = timed "
-                         "choice among 1-3)")
+                         "choice of 1 or 3)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="afterwards copy a MIB-MiB device buffer (copy_probe): a kernel of known "
                          "bytes whose FETCH_SIZE/WRITE_SIZE readings scale the kernel's")

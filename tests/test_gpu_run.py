@@ -129,7 +129,7 @@ def test_perm8_sort8(engine):
 
 
 def test_life_cube_small(engine):
-    prog, board = life_cube(4, 4, 3, 3)  # 64 machines? no: dims = max(9, 6) -> one group
+    prog, board = life_cube(4, 4, 3, 3)  # 6 dims of board, run on dims max(9, 6) = 9: one group
     print("check life cube 4x4 on 3+3 dims", flush=True)
     check(engine, prog, runs=(2,))
     prog, board = life_cube(4, 4, 6, 6)  # dims 12: cross-wave links, segments

@@ -132,9 +132,11 @@ def test_life_cube_small(engine):
     prog, board = life_cube(4, 4, 3, 3)  # 6 dims of board, run on dims max(9, 6) = 9: one group
     print("check life cube 4x4 on 3+3 dims", flush=True)
     check(engine, prog, runs=(2,))
-    prog, board = life_cube(4, 4, 6, 6)  # dims 12: cross-wave links, segments
-    print("check life cube 4x4 on 6+6 dims", flush=True)
-    check(engine, prog, runs=(1, 2))
+    prog, board = life_cube(4, 4, 6, 6)  # dims 12: links across the cube's two waves
+    print("check life cube 4x4 on 6+6 dims, one workgroup per cube", flush=True)
+    assert check(engine, prog, runs=(1, 2)).stats()["resident"] == 1
+    print("check life cube 4x4 on 6+6 dims, streamed (cross-wave values recomputed)", flush=True)
+    assert check(engine, prog, tune={"lds_links": 2}, runs=(1, 2)).stats()["resident"] == 0
 
 
 @pytest.mark.parametrize("gp", [4, 3, 5])
